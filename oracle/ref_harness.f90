@@ -1,0 +1,124 @@
+! ref_harness.f90 -- drives the compiled Ndyn reference to write golden vectors.
+!
+! TEST INFRASTRUCTURE ONLY.  oracle/Makefile compiles the reference's own
+! src/{mods,domtype,potentials,read_input,nd}.f90 where they lie (never copied),
+! renames the reference's main program symbol in its nd.o, and links this harness
+! in its place.  The harness sets the reference's module state directly (so the
+! thread-dependent Velocity RNG, src/nd.f90:1500-1624, is bypassed), calls the
+! reference routines, and dumps their results at full precision.
+!
+! Input (stream binary, little endian):
+!   int32  mode, np, ntypes, npairs, ptsize, neimax, nsteps, outsteps, step0,
+!          renei, renei_freq, isokinetic, ke_freq, isobaric, p_freq,
+!          isoenergetic, e_freq, nthreads
+!   real64 box(3), dt, skin, ptrmin, ketarget, keparm, ptarget(3), pparm, eparm
+!   real64 mass(ntypes); int32 pairmat(ntypes,ntypes); int32 kind(npairs)
+!   real64 pairparams(4,npairs); int32 typ(np); real64 pos(3,np), vel(3,np), acc(3,np)
+! Modes: 1 tables (buildPairTables), 2 half list (buildCellList),
+!        3 forces (buildCellList + compute0(T,T)), 4 'dynamics' run.
+program ref_harness
+  use omp_lib
+  use mostCommon
+  use interpot
+  use control
+  use listData
+  use groupStuff
+  implicit none
+  character(len=1024) :: fin, fout
+  integer :: mode, nthr, i
+  integer :: iv(18)
+  real(8) :: dv(13)
+  integer, allocatable :: kinds(:)
+
+  call get_command_argument(1, fin)
+  call get_command_argument(2, fout)
+  open(20, file=trim(fin), access='stream', form='unformatted', status='old')
+  read(20) iv
+  mode = iv(1); np = iv(2); Ntypes = iv(3); NPairs = iv(4); PTsize = iv(5)
+  NeiMax = iv(6); step_num = iv(7); OutSteps = iv(8); step = iv(9)
+  ReNei = iv(10) /= 0; ReNeiFreq = iv(11)
+  isokinetic = iv(12) /= 0; KEfreq = iv(13)
+  isobaric = iv(14) /= 0; Pfreq = iv(15)
+  isoenergetic = iv(16) /= 0; Efreq = iv(17)
+  nthr = iv(18)
+  read(20) dv
+  box = dv(1:3); dt = dv(4); skin = dv(5); PTrMin = dv(6)
+  KEtarget = dv(7); KEparm = dv(8); Ptarget = dv(9:11); Pparm = dv(12); Eparm = dv(13)
+
+  call omp_set_num_threads(nthr)
+  thread_num = nthr
+  proc_num = omp_get_num_procs()
+
+  allocate(mass(Ntypes), typeName(Ntypes), charge(Ntypes), NatomsPerType(0:Ntypes))
+  read(20) mass
+  charge = 0.d0
+  do i = 1, Ntypes
+    write(typeName(i), '(A,i0)') 'T', i
+  enddo
+  allocate(PairMat(Ntypes,Ntypes), PairName(0:NPairs), PairParams(4,NPairs), &
+           PairChargeProd(NPairs), PTdr2(NPairs), PTrc2(NPairs), kinds(NPairs))
+  read(20) PairMat
+  read(20) kinds
+  read(20) PairParams
+  PairName = 'N/A'; PairChargeProd = 0.d0; PTdr2 = 0.d0; PTrc2 = 0.d0
+  do i = 1, NPairs
+    select case (kinds(i))
+      case (1); PairName(i) = 'morse'
+      case (2); PairName(i) = 'buck'
+      case (3); PairName(i) = 'LJ'
+      case default; stop 'harness: unknown pair kind'
+    end select
+  enddo
+  asympots = .false.; springpots = .false.; potOuts = .false.
+
+  allocate(typ(np), pos(3,np), vel(3,np), acc(3,np), force(3,np), pene(np))
+  read(20) typ
+  read(20) pos
+  read(20) vel
+  read(20) acc
+  close(20)
+  force = 0.d0; pene = 0.d0
+  grpBuff = 100; holdBuff = 100; Ngrp = 0; Nhold = 0
+  allocate(grp(np), groupName(grpBuff), hold(np), holdName(holdBuff), &
+           bcTyp(holdBuff), bcParm(3,holdBuff), bcDir(3,holdBuff))
+  grp = 0; hold = 0; bcTyp = 'X'; bcParm = 0.d0; bcDir = .false.
+  boxLoad = .false.; boxVel = 0.d0
+  Model = 'NA'; ModelInType = 'NA'; ModelOutType = 'dump'; ConfOuts = 0
+  potential = 0.d0; kinetic = 0.d0; virial = 0.d0; e0 = 0.d0
+
+  call buildPairTables
+
+  open(21, file=trim(fout), access='stream', form='unformatted', status='replace')
+  select case (mode)
+  case (1)
+    write(21) PTrc2, PTdr2, PPairTable, FPairTable
+  case (2, 3)
+    ! the prologue of 'dynamics' (src/nd.f90:77-107)
+    if (skin < 0.d0) skin = rcut*0.2d0
+    MCELL(1) = int(box(1)/(rcut+skin))
+    MCELL(2) = int(box(2)/(rcut+skin))
+    MCELL(3) = int(box(3)/(rcut+skin))
+    NCELL = MCELL(1)*MCELL(2)*MCELL(3)
+    if (NeiMax == 0) then
+      NeiMax = int(4./3.*PI*((rcut+skin)*1.1)**3*np/(box(1)*box(2)*box(3)))
+    endif
+    allocate(HEAD(NCELL), VNL(1:NeiMax+1,1:np), List(np), dispList(3,np), &
+             ftmp(1:3,thread_num*np))
+    skinsq = skin*skin
+    call buildCellList(.false.)
+    if (mode == 2) then
+      write(21) MCELL, NeiMax
+      write(21) VNL
+    else
+      call compute0(.true., .true.)
+      write(21) force, potential, virial
+    endif
+  case (4)
+    call dynamics
+    write(21) pos, vel, acc, force, potential, kinetic, virial, box, e0, &
+              dble(NlistUpdates), dble(step)
+  case default
+    stop 'harness: unknown mode'
+  end select
+  close(21)
+end program ref_harness

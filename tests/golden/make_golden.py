@@ -1,0 +1,188 @@
+"""Writes the golden fixtures in tests/golden/ from the compiled Ndyn reference.
+
+Run in the build container only (it needs oracle/_ref/ndyn_ref_harness, built by
+`make -C oracle ref` from the reference sources):
+
+    python tests/golden/make_golden.py
+
+Each fixture is an .npz holding the inputs and the reference's outputs at full
+precision.  Systems come from ndyn_omp_amd.workloads (seeded), so the inputs can also
+be regenerated; they are stored anyway so the fixtures stand alone.  The harness runs
+the reference with one OpenMP thread (its force sum order then equals a serial sweep).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+from ndyn_omp_amd import workloads as W  # noqa: E402
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ndyn_ref_harness")
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def write_input(path, mode, s, nsteps=0, outsteps=1, step0=0, neimax=0, skin=-1.0, dt=0.0,
+                renei=True, renei_freq=1, iso_ke=None, iso_p=None, iso_e=None, acc=None):
+    iv = np.array([mode, s.np, s.ntypes, s.npairs, s.ptsize, neimax, nsteps, outsteps, step0,
+                   int(renei), renei_freq,
+                   1 if iso_ke else 0, iso_ke[2] if iso_ke else 1,
+                   1 if iso_p else 0, iso_p[2] if iso_p else 1,
+                   1 if iso_e else 0, iso_e[1] if iso_e else 1, 1], dtype="<i4")
+    dv = np.zeros(13, dtype="<f8")
+    dv[0:3] = s.box
+    dv[3] = dt
+    dv[4] = skin
+    dv[5] = s.ptrmin
+    if iso_ke:
+        dv[6], dv[7] = iso_ke[0], iso_ke[1]
+    if iso_p:
+        dv[8:11] = iso_p[0]
+        dv[11] = iso_p[1]
+    if iso_e:
+        dv[12] = iso_e[0]
+    with open(path, "wb") as f:
+        f.write(iv.tobytes())
+        f.write(dv.tobytes())
+        f.write(np.asarray(s.masses, "<f8").tobytes())
+        f.write(np.asarray(s.pairmat.T, "<i4").tobytes())          # column-major
+        f.write(np.asarray(s.kinds, "<i4").tobytes())
+        f.write(np.asarray(s.params, "<f8").tobytes())             # (npairs,4) -> 4 x npairs col-major
+        f.write(np.asarray(s.typ, "<i4").tobytes())
+        f.write(np.asarray(s.pos, "<f8").tobytes())                # (np,3) -> 3 x np col-major
+        f.write(np.asarray(s.vel, "<f8").tobytes())
+        a = np.zeros_like(s.pos) if acc is None else acc
+        f.write(np.asarray(a, "<f8").tobytes())
+
+
+def run_harness(mode, s, **kw):
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        write_input(fin, mode, s, **kw)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        p = subprocess.run([HARNESS, fin, fout], cwd=d, env=env, capture_output=True,
+                           text=True, timeout=3600)
+        if p.returncode != 0:
+            raise RuntimeError(p.stdout + p.stderr)
+        raw = open(fout, "rb").read()
+        return raw, p.stdout
+
+
+def sys_arrays(s):
+    return dict(pos=s.pos, vel=s.vel, typ=s.typ, box=s.box, masses=s.masses,
+                pairmat=s.pairmat, kinds=s.kinds, params=s.params,
+                ptrmin=np.float64(s.ptrmin), ptsize=np.int64(s.ptsize))
+
+
+def thermo_lines(stdout):
+    rows = []
+    for line in stdout.splitlines():
+        parts = line.split()
+        if len(parts) == 6 and not line.startswith("#"):
+            try:
+                rows.append([float(x) for x in parts])
+            except ValueError:
+                pass
+    return np.array(rows)
+
+
+def golden_tables():
+    out = {}
+    for name, kind, prm in [("morse", W.MORSE, W.CU_MORSE), ("lj", W.LJ, W.AR_LJ),
+                            ("buck", W.BUCK, W.ALLOY_BUCK)]:
+        s = W.cu_morse(1, temp_k=0.0)
+        s.kinds = np.array([kind], np.int32)
+        s.params = np.array([prm], np.float64)
+        raw, _ = run_harness(1, s)
+        n = s.ptsize + 1
+        a = np.frombuffer(raw, "<f8")
+        out[f"{name}_params"] = np.array(prm)
+        out[f"{name}_rc2"], out[f"{name}_dr2"] = a[0], a[1]
+        out[f"{name}_P"] = a[2:2 + n]
+        out[f"{name}_F"] = a[2 + n:2 + 2 * n]
+    out["ptrmin"], out["ptsize"] = np.float64(0.25), np.int64(5000)
+    np.savez_compressed(os.path.join(OUT, "tables.npz"), **out)
+
+
+def golden_list(fname, s, skin):
+    raw, _ = run_harness(2, s, skin=skin)
+    hdr = np.frombuffer(raw[:16], "<i4")
+    mcell, neimax = hdr[:3], int(hdr[3])
+    vnl = np.frombuffer(raw[16:], "<i4").reshape(s.np, neimax + 1)
+    counts = vnl[:, 0] - 1
+    ent = np.concatenate([vnl[i, 1:1 + counts[i]] - 1 for i in range(s.np)]).astype(np.int32)
+    np.savez_compressed(os.path.join(OUT, fname), **sys_arrays(s), skin=np.float64(skin),
+                        mcell=mcell, neimax=np.int64(neimax), counts=counts.astype(np.int32),
+                        entries=ent)
+
+
+def golden_force(fname, s, skin):
+    raw, _ = run_harness(3, s, skin=skin)
+    a = np.frombuffer(raw, "<f8")
+    n = s.np
+    np.savez_compressed(os.path.join(OUT, fname), **sys_arrays(s), skin=np.float64(skin),
+                        force=a[:3 * n].reshape(n, 3), pe=a[3 * n], virial=a[3 * n + 1:3 * n + 7])
+
+
+def golden_run(fname, s, nsteps, dt, skin, outsteps, **kw):
+    raw, stdout = run_harness(4, s, nsteps=nsteps, dt=dt, skin=skin, outsteps=outsteps, **kw)
+    a = np.frombuffer(raw, "<f8")
+    n = s.np
+    o = 0
+    res = {}
+    for k in ("pos", "vel", "acc", "force"):
+        res["out_" + k] = a[o:o + 3 * n].reshape(n, 3)
+        o += 3 * n
+    res["out_pe"], res["out_ke"] = a[o], a[o + 1]
+    res["out_virial"] = a[o + 2:o + 8]
+    res["out_box"] = a[o + 8:o + 11]
+    res["out_e0"], res["out_nupdates"], res["out_step"] = a[o + 11], a[o + 12], a[o + 13]
+    meta = dict(nsteps=np.int64(nsteps), dt=np.float64(dt), skin=np.float64(skin),
+                outsteps=np.int64(outsteps))
+    for k, v in kw.items():
+        if v is None or isinstance(v, bool):
+            meta[k] = np.array(-1 if v is None else int(v))
+        else:
+            meta[k] = np.array(v, dtype=object if isinstance(v, tuple) and
+                               any(isinstance(x, tuple) for x in v) else None)
+    if "iso_p" in kw and kw["iso_p"]:
+        meta["iso_p"] = np.array(list(kw["iso_p"][0]) + [kw["iso_p"][1], kw["iso_p"][2]])
+    res["thermo"] = thermo_lines(stdout)
+    np.savez_compressed(os.path.join(OUT, fname), **sys_arrays(s), **meta, **res)
+
+
+def main():
+    if not os.path.exists(HARNESS):
+        raise SystemExit("build the reference harness first: make -C oracle ref")
+    golden_tables()
+    # half lists: Cu 7^3 (MCELL 3), c0-size Cu 10^3 (MCELL 4), Buckingham alloy (3 pairs)
+    golden_list("list_cu7.npz", W.cu_morse(7, jitter_a=0.05, seed=101), 0.5)
+    golden_list("list_cu10.npz", W.cu_morse(10, jitter_a=0.05, seed=102), 0.5)
+    golden_list("list_alloy5.npz", W.buck_alloy((5, 5, 5), jitter_a=0.1, seed=103), 0.5)
+    golden_list("list_ar6.npz", W.ar_lj(6, seed=104), 0.4)
+    # forces
+    golden_force("force_cu10.npz", W.cu_morse(10, jitter_a=0.05, seed=105), 0.5)
+    golden_force("force_ar6.npz", W.ar_lj(6, jitter_a=0.05, seed=106), 0.4)
+    golden_force("force_alloy5.npz", W.buck_alloy((5, 5, 5), jitter_a=0.1, seed=107), 0.5)
+    # dynamics: NVE Cu (config-0 system, 100 steps), NVT Ar, NPT Cu, isoenergetic alloy
+    golden_run("run_cu10_nve.npz", W.cu_morse(10, temp_k=300.0, seed=12345), 100, W.DT_1FS,
+               0.5, 25)
+    golden_run("run_ar6_nvt.npz", W.ar_lj(6, seed=12346), 100, W.DT_2FS, 0.4, 20,
+               iso_ke=(1.5 * W.KB_EV * 90.0, W.DT_2FS, 1))
+    golden_run("run_cu7_npt.npz", W.cu_morse(7, temp_k=600.0, seed=12347), 100, W.DT_1FS,
+               0.5, 20, iso_ke=(1.5 * W.KB_EV * 600.0, 100 * W.DT_1FS, 1),
+               iso_p=((0.0, 0.0, 0.0), 1000 * W.DT_1FS, 10))
+    golden_run("run_alloy5_nve.npz", W.buck_alloy((5, 5, 5), temp_k=900.0, seed=12348), 60,
+               W.DT_1FS, 0.5, 20, iso_e=(50 * W.DT_1FS, 5))
+    for f in sorted(os.listdir(OUT)):
+        if f.endswith(".npz"):
+            print(f"{f:28s} {os.path.getsize(os.path.join(OUT, f)) / 1024:8.1f} KiB")
+
+
+if __name__ == "__main__":
+    main()
