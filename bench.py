@@ -1,0 +1,232 @@
+"""Benchmark of the MD-step engine on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c1|c2|c4]
+
+Prints one JSON line.  Default workload (the one the metric is quoted on): 512
+independent periodic systems of uneven size (atom counts log-uniform in [500, 20000],
+whole FCC cells; Morse Cu or Buckingham binary alloy; T uniform in [100, 1200] K; dt 1 fs;
+NVE), all advanced by the batched mixed-precision engine.  value = atom-steps/s over the
+K timed steps with all state resident in HBM.  The plain FP64 one-system-per-launch GPU
+path is timed on the same mix; the force kernel is timed with HIP events on the engine
+stream for the roofline; the reference (compiled Ndyn, oracle/_ref) is timed on a bounded
+sample on the host for cpu_baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from ndyn_omp_amd import engine as G  # noqa: E402
+from ndyn_omp_amd import workloads as W  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "atom-steps/s vs plain FP64 GPU path at <=1e-5 rel force err; p50/p99 job latency"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def force_bytes_per_atom(mean_nbr):
+    """Algorithmic HBM bytes of one force evaluation per atom: list entries (4 B each),
+    own coordinate+type (16 B), list length (4 B), force+energy out (16 B)."""
+    return 4.0 * mean_nbr + 36.0
+
+
+def workload_systems(name):
+    if name == "c3":
+        systems = W.replica_mix(512)
+        desc = ("512 uneven replicas: atoms log-uniform [500,20000] in whole FCC cells, "
+                "Morse Cu (a=3.615) or Buckingham alloy (a=5.6, 50/50 27/63.5 amu), "
+                "T~U[100,1200] K, dt 1 fs, NVE, skin 0.5 A")
+        return systems, desc, dict(dt=W.DT_1FS, skin=0.5)
+    if name == "c1":
+        s = W.ar_lj(40)
+        return [s], ("LJ argon FCC 40^3 (256000 atoms), rc 2.5 sigma, skin 0.4, 90 K, "
+                     "velocity rescale every step, dt 2 fs"), dict(dt=W.DT_2FS, skin=0.4,
+                                                                   iso_ke=(1.5 * W.KB_EV * 90.0, W.DT_2FS, 1))
+    if name == "c2":
+        s = W.cu_morse(80, temp_k=600.0, seed=12347)
+        return [s], ("Morse Cu FCC 80^3 (2048000 atoms), skin 0.5, 600 K, Berendsen "
+                     "thermostat tau 100 fs + isotropic barostat 0 Pa tau 1 ps every 10 steps, "
+                     "dt 1 fs"), dict(dt=W.DT_1FS, skin=0.5,
+                                      iso_ke=(1.5 * W.KB_EV * 600.0, 100 * W.DT_1FS, 1),
+                                      iso_p=((0.0, 0.0, 0.0), 1000 * W.DT_1FS, 10))
+    raise SystemExit(f"unknown workload {name}")
+
+
+def make_rp(cfg, nsteps):
+    return G.run_params(nsteps, cfg["dt"], out_steps=nsteps, skin=cfg["skin"],
+                        trigger=G.TRIGGER_SKIN, isokinetic=cfg.get("iso_ke"),
+                        isobaric=cfg.get("iso_p"))
+
+
+def run_engine(systems, cfg, path, warmup, steps, timing_steps, graph_steps=0, tpa=0):
+    eng = G.BatchEngine(path=path, tpa=tpa, graph_steps=graph_steps)
+    total = warmup + steps + timing_steps
+    slots = [eng.admit(s, make_rp(cfg, total)) for s in systems]
+    atoms = sum(s.np for s in systems)
+    eng.step(warmup)
+    t0 = time.perf_counter()
+    eng.step(steps)
+    t1 = time.perf_counter()
+    res = dict(atom_steps_per_s=atoms * steps / (t1 - t0), ms_per_step=1e3 * (t1 - t0) / steps,
+               atoms=atoms)
+    if timing_steps:
+        eng.timing(True)
+        eng.kernel_time(reset=True)
+        eng.step(timing_steps)
+        kt = eng.kernel_time()
+        eng.timing(False)
+        mean_nbr, _ = eng.list_stats()
+        res.update(force_ms=kt["avg_ms"], force_launches=kt["count"], mean_nbr=mean_nbr)
+    return eng, slots, res
+
+
+def force_error_mixed_vs_fp64(eng, slots, systems, k=6):
+    """RMS relative force error of the mixed path against the FP64 path on snapshots of
+    k replicas taken from the running engine."""
+    errs = []
+    idx = np.linspace(0, len(slots) - 1, k).astype(int)
+    for r in idx:
+        s = systems[r].copy()
+        st = eng.fetch(slots[r], s.np)
+        s.pos, s.vel, s.box = st["pos"], st["vel"], st["box"]
+        f = {}
+        for path in (G.PATH_FP64, G.PATH_MIXED):
+            sim = G.Simulation(s, path=path)
+            sim.build_lists(0.5)
+            sim.compute(False, False)
+            f[path] = sim.state()["force"]
+            sim.close()
+        errs.append(float(np.linalg.norm(f[1] - f[0]) / np.linalg.norm(f[0])))
+    return max(errs)
+
+
+def cpu_baseline(systems, cfg, budget_s):
+    """The compiled reference (oracle/_ref) on host cores: its own step-loop timer
+    ('Elapsed time for main dynamics') over a bounded sample of the workload."""
+    from oracle import refrun
+    if not refrun.available():
+        return None
+    threads = min(8, os.cpu_count() or 1)   # ptmp(8)/vtmp(6,8) cap the reference at 8
+    order = np.argsort([s.np for s in systems])
+    pick = [systems[i] for i in order[:: max(1, len(order) // 12)]][:12]
+    steps = 20
+    tot_as, tot_t = 0.0, 0.0
+    t_start = time.perf_counter()
+    for s in pick:
+        _, out = refrun.run_harness(refrun.MODE_RUN, s, nsteps=steps, outsteps=steps,
+                                    dt=cfg["dt"], skin=cfg["skin"], nthreads=threads,
+                                    iso_ke=cfg.get("iso_ke"), iso_p=cfg.get("iso_p"))
+        tot_t += refrun.step_loop_seconds(out)
+        tot_as += s.np * steps
+        if time.perf_counter() - t_start > budget_s:
+            break
+    # scale the step count to fill about budget_s of step loop
+    if tot_t > 0 and tot_t < 0.5 * budget_s:
+        steps2 = int(min(2000, steps * max(1.0, 0.8 * budget_s / tot_t)))
+        tot_as, tot_t = 0.0, 0.0
+        for s in pick:
+            _, out = refrun.run_harness(refrun.MODE_RUN, s, nsteps=steps2, outsteps=steps2,
+                                        dt=cfg["dt"], skin=cfg["skin"], nthreads=threads,
+                                        iso_ke=cfg.get("iso_ke"), iso_p=cfg.get("iso_p"))
+            tot_t += refrun.step_loop_seconds(out)
+            tot_as += s.np * steps2
+        steps = steps2
+    return dict(value=tot_as / tot_t, unit="atom-steps/s", cores=threads, kind="reference",
+                sample=f"{len(pick)} systems of the workload ({sum(s.np for s in pick)} atoms), "
+                       f"{steps} steps each, reference step-loop timer, OMP_NUM_THREADS={threads}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", default="c3")
+    ap.add_argument("--timing-steps", type=int, default=100)
+    ap.add_argument("--plain-steps", type=int, default=100)
+    ap.add_argument("--graph-steps", type=int, default=0)
+    ap.add_argument("--tpa", type=int, default=0)
+    ap.add_argument("--cpu-budget-s", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is "
+                                   "designed (DESIGN) but not built, --gpus must be 1"}))
+        sys.exit(2)
+    G.load_library()
+    systems, desc, cfg = workload_systems(args.workload)
+    atoms = sum(s.np for s in systems)
+    log(f"bench: workload {args.workload}: {len(systems)} systems, {atoms} atoms")
+
+    eng, slots, mixed = run_engine(systems, cfg, G.PATH_MIXED, args.warmup, args.steps,
+                                   args.timing_steps, args.graph_steps, args.tpa)
+    log(f"bench: mixed {mixed['atom_steps_per_s']:.4e} atom-steps/s, "
+        f"{mixed['ms_per_step']:.3f} ms/step, force {mixed.get('force_ms', 0):.4f} ms")
+    ferr = force_error_mixed_vs_fp64(eng, slots, systems)
+    log(f"bench: mixed vs fp64 rms rel force err {ferr:.3e}")
+    eng.close()
+
+    plain = None
+    if not args.no_plain:
+        engp, _, plain = run_engine(systems, cfg, G.PATH_FP64, 5, args.plain_steps, 0)
+        engp.close()
+        log(f"bench: plain fp64 {plain['atom_steps_per_s']:.4e} atom-steps/s")
+
+    bpa = force_bytes_per_atom(mixed["mean_nbr"])
+    achieved = bpa * mixed["atoms"] / (mixed["force_ms"] * 1e-3) / 1e9
+    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                kernel="k_force_mixed", bytes_per_atom=round(bpa, 1),
+                atoms_per_launch=mixed["atoms"], avg_launch_ms=round(mixed["force_ms"], 5))
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(systems, cfg, args.cpu_budget_s)
+        except Exception as ex:  # the reference harness is a baseline only
+            log(f"bench: cpu baseline skipped: {ex}")
+    out = {
+        "metric": METRIC,
+        "value": round(mixed["atom_steps_per_s"], 1),
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair math / f64 integration",
+        "data": "synthetic (seeded FCC lattices, Maxwell-Boltzmann velocities)",
+        "config": {
+            "workload": f"{args.workload}: {desc}",
+            "systems": len(systems),
+            "atoms": atoms,
+            "mean_full_list": round(mixed["mean_nbr"], 2),
+            "plain_fp64_atom_steps_per_s": None if plain is None else round(plain["atom_steps_per_s"], 1),
+            "speedup_vs_plain_fp64": None if plain is None else round(
+                mixed["atom_steps_per_s"] / plain["atom_steps_per_s"], 3),
+            "force_rms_rel_err_vs_fp64": ferr,
+            "graph_steps": args.graph_steps,
+        },
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

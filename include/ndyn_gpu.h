@@ -1,0 +1,201 @@
+/*
+ * ndyn_gpu.h -- C ABI of the MI355X MD-step engine (libndyn_gpu.so).
+ *
+ * Drop-in for the engine layer of Ndyn (stewmasterj/Ndyn_OMP, src/): the reference
+ * routines take no data arguments and share module globals (src/mods.f90:2-95); here
+ * every entry point takes an opaque handle and plain pointers/sizes, callable from
+ * Fortran through iso_c_binding (see ndyn_omp_amd/fortran/ndg_iface.f90).
+ *
+ * Layouts at the boundary are the reference's: pos/vel/acc/force are (3,np)
+ * column-major (AoS x,y,z per atom) in scaled coordinates for pos; typ is 1-based;
+ * pairmat is (ntypes,ntypes) column-major with 1-based pair ids, 0 = no interaction;
+ * tables are (ptsize+1, npairs) column-major.  The handle owns device state
+ * (positions, velocities, lists, thermostat state stay resident between calls);
+ * host arrays are only read or written during a call.
+ *
+ * Return codes map the reference's STOPs: NDG_OK, NDG_ERR_NEIMAX
+ * ('too many neighbours in list', src/nd.f90:1378,1421), NDG_ERR_OUTSIDE
+ * ('Particle outside of box', src/nd.f90:1494), NDG_ERR_HIP (runtime error), NDG_ERR_ARG,
+ * NDG_ERR_NOMEM (HBM budget exhausted).  ndg_last_error() gives the message.
+ * One host thread per handle; calls on one handle are not re-entrant.
+ */
+#ifndef NDYN_GPU_H
+#define NDYN_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  NDG_OK = 0,
+  NDG_ERR_NEIMAX = 1,
+  NDG_ERR_OUTSIDE = 2,
+  NDG_ERR_HIP = 3,
+  NDG_ERR_ARG = 4,
+  NDG_ERR_NOMEM = 5
+};
+
+/* precision / path selection (ndg_create flags, ndg_engine_create precision) */
+enum {
+  NDG_PATH_FP64 = 0,   /* plain path: FP64 positions, tables, accumulation; one system per launch */
+  NDG_PATH_MIXED = 1   /* FP32 pair math on cell-anchored FP32 coordinates, FP64 integration */
+};
+
+/* list-rebuild trigger (ndg_run_params.trigger) */
+enum {
+  NDG_TRIGGER_COMPAT = 0, /* reference criterion: top-2 |disp|^2 in scaled units vs skin^2 */
+  NDG_TRIGGER_SKIN = 1    /* d1 + d2 (A) + (rcut+skin)*box strain > skin; re-grid at rebuild */
+};
+
+/* pair kinds for ndg_pair_table (reference potentials.f90 names) */
+enum { NDG_MORSE = 1, NDG_BUCK = 2, NDG_LJ = 3, NDG_MORSE_PHYS = 4 };
+
+typedef struct ndg_system ndg_t;
+typedef struct ndg_engine ndg_engine_t;
+
+/* One 'Run n dt' (src/read_input.f90:198-200) with the deck's control settings. */
+typedef struct {
+  int nsteps;            /* Run n */
+  double dt;             /* Run n dt */
+  int out_steps;         /* OutFreq (src/read_input.f90:193) */
+  double skin;           /* Skin; < 0 -> 0.2*rcut (src/nd.f90:77-79) */
+  int renei, renei_freq; /* ReNeighbor (src/read_input.f90:93-94) */
+  int trigger;           /* NDG_TRIGGER_* */
+  int isokinetic; double ke_target, ke_tau; int ke_freq;       /* Isokinetic */
+  int isobaric; double p_target[3], p_tau; int p_freq;         /* Isobaric */
+  int isoenergetic; double e_tau; int e_freq;                  /* Isoenergetic */
+  int box_load; double box_vel[3];                             /* BoxVel */
+} ndg_run_params;
+
+/* One thermo line (write_thermo, src/read_input.f90:433-446) plus virial and box. */
+typedef struct {
+  int step;
+  int pad_;
+  double pe, ke, eerr, pressure, volume;
+  double virial[6];
+  double box[3];
+} ndg_thermo;
+
+/* ------------------------------------------------------------------ tables */
+
+/* Pair table as buildPairTables (src/potentials.f90:2-56): P[i] = V(r_i),
+ * F[i] = r_i*dpot(r_i) on r_i^2 = ptrmin + i*dr2, i = 0..ptsize.  params = {rc,p2,p3,p4}
+ * in deck order.  Host-only helper for callers without their own table builder. */
+int ndg_pair_table(int kind, const double params[4], double ptrmin, int ptsize,
+                   double *P, double *F, double *rc2, double *dr2);
+
+/* ------------------------------------------------------------------ single system */
+
+/* Allocate a system of np atoms.  neimax = full-list capacity per atom (0: the
+ * reference estimate at the first ndg_run_begin/ndg_build_lists, src/nd.f90:84-88).
+ * flags = NDG_PATH_*.  Replaces the allocations of read_model/dynamics
+ * (src/read_input.f90:349-351, src/nd.f90:90-94). */
+int ndg_create(ndg_t **h, int np, int ntypes, int npairs, int ptsize, int neimax, int flags);
+
+/* Upload tables and type data: PPairTable/FPairTable (src/potentials.f90:17,31-32),
+ * PTrc2/PTdr2, PTrMin, PairMat, PairParams (4 x npairs), mass (src/mods.f90:51-58). */
+int ndg_set_tables(ndg_t *h, const double *P, const double *F, const double *rc2,
+                   const double *dr2, double ptrmin, const int *pairmat,
+                   const double *pairparams, const double *mass);
+
+/* Upload state: pos (scaled), vel, acc (may be NULL = 0, as read_model sets,
+ * src/read_input.f90:354-355), typ (1-based), box (A). */
+int ndg_set_state(ndg_t *h, const double *pos, const double *vel, const double *acc,
+                  const int *typ, const double box[3]);
+
+/* Copy state back in the caller's atom order; any pointer may be NULL.  force is the
+ * last evaluated force, pene the per-atom energy 1/2 sum_j V_ij of the last energy step. */
+int ndg_get_state(ndg_t *h, double *pos, double *vel, double *acc, double *force,
+                  double *pene, double box[3]);
+
+int ndg_set_box(ndg_t *h, const double box[3]);
+int ndg_get_box(ndg_t *h, double box[3]);
+
+/* buildCellList + buildVerletList (src/nd.f90:1214-1446) at the current positions, with
+ * MCELL = int(box/(rcut+skin)) (src/nd.f90:80-83).  Lists are full (both i-j and j-i). */
+int ndg_build_lists(ndg_t *h, double skin);
+
+/* Neighbour list in caller order: counts[np] and, if nbr != NULL, nbr[np*cap] (0-based,
+ * row i holds counts[i] entries, sorted ascending). */
+int ndg_get_neighbours(ndg_t *h, int *counts, int *nbr, int cap);
+
+/* CheckListUpdate (src/nd.f90:1447-1499) followed by a rebuild when it fires.
+ * *rebuilt = 1 if rebuilt. */
+int ndg_check_rebuild(ndg_t *h, double skin, int trigger, int *rebuilt);
+
+/* integrate1 (src/nd.f90:984-1046). */
+int ndg_integrate1(ndg_t *h, double dt);
+
+/* compute0 (src/nd.f90:200-363): forces; potential when cu, virial[6] when cv. */
+int ndg_compute(ndg_t *h, int cv, int cu, double *pe, double virial[6]);
+
+/* integrate2 (src/nd.f90:1081-1121): kick; kinetic energy when ck. */
+int ndg_integrate2(ndg_t *h, double dt, int ck, double *ke);
+
+/* mkIsoKE / mkIsoE velocity scaling (src/nd.f90:1122-1182): v *= lambda. */
+int ndg_scale_velocities(ndg_t *h, double lambda);
+
+/* The 'dynamics' loop (src/nd.f90:60-199), resident on the device.
+ * ndg_run_begin: prologue (grid, NeiMax, list build, e0 reset).
+ * ndg_run_steps: advance nsteps (<= remaining); thermo records for step 1 of the Run
+ *   and every out_steps are returned in rec (up to max_rec), *nrec = count.
+ * ndg_run_end: applies the last pending thermostat scale; *nupdates = list rebuilds. */
+int ndg_run_begin(ndg_t *h, const ndg_run_params *rp, int step0);
+int ndg_run_steps(ndg_t *h, int nsteps, ndg_thermo *rec, int max_rec, int *nrec);
+int ndg_run_end(ndg_t *h, int *nupdates, int *step);
+
+int ndg_destroy(ndg_t *h);
+const char *ndg_last_error(void);
+
+/* ------------------------------------------------------------------ batched engine */
+
+/* An engine advances many independent systems ("replicas") per launch.
+ * hbm_budget = device bytes the engine may hold (0 = 80% of free HBM). */
+int ndg_engine_create(ndg_engine_t **e, int path, size_t hbm_budget);
+int ndg_engine_destroy(ndg_engine_t *e);
+
+/* Bytes a system of this shape needs on the device (admission by footprint). */
+size_t ndg_footprint(int np, int ntypes, int npairs, int ptsize, int neimax);
+
+/* Admit a system into the engine: tables/state as ndg_set_tables/ndg_set_state and a
+ * Run.  Returns NDG_ERR_NOMEM if the budget cannot hold it.  *slot identifies it. */
+int ndg_engine_admit(ndg_engine_t *e, int *slot, int np, int ntypes, int npairs, int ptsize,
+                     int neimax, const double *P, const double *F, const double *rc2,
+                     const double *dr2, double ptrmin, const int *pairmat,
+                     const double *pairparams, const double *mass, const double *pos,
+                     const double *vel, const int *typ, const double box[3],
+                     const ndg_run_params *rp);
+
+/* Advance every admitted, unfinished system by up to nsteps steps in batched launches.
+ * Systems stop at the end of their Run.  *ndone = systems that are finished. */
+int ndg_engine_step(ndg_engine_t *e, int nsteps, int *ndone);
+
+/* Status of a slot: steps done, finished flag, last thermo record, error code. */
+int ndg_engine_status(ndg_engine_t *e, int slot, int *steps_done, int *finished,
+                      ndg_thermo *last, int *err);
+
+/* Thermo records of a slot produced since the last call (up to max_rec). */
+int ndg_engine_thermo(ndg_engine_t *e, int slot, ndg_thermo *rec, int max_rec, int *nrec);
+
+/* Copy a slot's state back (caller order), then optionally release it. */
+int ndg_engine_fetch(ndg_engine_t *e, int slot, double *pos, double *vel, double *force,
+                     double box[3]);
+int ndg_engine_release(ndg_engine_t *e, int slot);
+
+/* Launch configuration and timing.  tpa = threads per atom in the force kernel (0 =
+ * per-size-class table); graph_steps = steps captured per hipGraph (0 = eager launches). */
+int ndg_engine_config(ndg_engine_t *e, int tpa, int graph_steps);
+/* Average device time (ms) of the force kernel over launches since the last reset,
+ * measured with HIP events on the engine's stream; count = launches timed. */
+int ndg_engine_kernel_time(ndg_engine_t *e, int reset, double *avg_ms, long long *count,
+                           double *pairs_per_launch, double *atoms_per_launch);
+int ndg_engine_timing(ndg_engine_t *e, int enable);
+/* Mean full-list length over admitted systems at their last build. */
+int ndg_engine_list_stats(ndg_engine_t *e, double *mean_nbr, long long *atoms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1690 @@
+// ndg_engine.hip -- host side of the MD-step engine and the C ABI (include/ndyn_gpu.h).
+//
+// An engine owns a HIP stream, a set of replica slots and the device tables that
+// describe them (RepParams / RepState / Tile arrays).  Each replica lives in one device
+// block; pair tables are shared between replicas with identical tables.  A single-system
+// handle (ndg_t) is an engine with one slot.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "ndg_internal.h"
+
+namespace ndg {
+void launch_drift(const EngineDev &E, int path, int t0, int nt, int xflags, hipStream_t s);
+void launch_kick(const EngineDev &E, int path, int t0, int nt, int xflags, hipStream_t s);
+void launch_force(const EngineDev &E, int path, int tpa, bool full, int t0, int nt, int xflags,
+                  hipStream_t s);
+void launch_reduce(const EngineDev &E, int r0, int nr, int mode, int xflags, int trigger,
+                   hipStream_t s);
+void launch_apply_pending(const EngineDev &E, int r0, int nr, hipStream_t s);
+void launch_rebuild(const EngineDev &E, int t0, int nt, int r0, int nr, bool any_large,
+                    hipStream_t s);
+enum : int { RED_FORCE = 1, RED_KICK = 2, RED_FLIP = 4, RED_STEP = 8, RED_CHECK = 16 };
+constexpr int kSmallSortHost = 65536;
+}  // namespace ndg
+
+using namespace ndg;
+
+// ------------------------------------------------------------------ errors
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                    \
+  do {                                                                               \
+    hipError_t e_ = (x);                                                             \
+    if (e_ != hipSuccess)                                                            \
+      return fail(NDG_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+static const char *err_text(int code) {
+  switch (code) {
+    case NDG_ERR_NEIMAX: return "too many neighbours in list";
+    case NDG_ERR_OUTSIDE: return "Particle outside of box";
+    default: return "engine error";
+  }
+}
+
+// ------------------------------------------------------------------ pair tables (host)
+// buildPairTables (src/potentials.f90:2-56) with pot/dpot (src/potentials.f90:59-247):
+// Morse (:94-110, :184-204, the /r of :198 kept), Buckingham (:112-122, :206-216),
+// Lennard-Jones (:124-133, :218-227); NDG_MORSE_PHYS tabulates -r V'(r) for Morse.
+
+static double pair_v(int kind, double r, const double *p) {
+  switch (kind) {
+    case NDG_MORSE:
+    case NDG_MORSE_PHYS: {
+      double x = p[3] <= 0.0 ? 0.0 : r - p[3];
+      double e = std::exp(-p[2] * x);
+      return p[1] * (e * e - 2.0 * e);
+    }
+    case NDG_BUCK: {
+      double r6 = r * r * r * r * r * r;
+      return p[2] <= 0.0 ? p[1] - p[3] / r6 : p[1] * std::exp(-r / p[2]) - p[3] / r6;
+    }
+    case NDG_LJ: {
+      double q = p[2] / r, q6 = q * q * q * q * q * q;
+      return 4.0 * p[1] * (q6 * q6 - q6);
+    }
+  }
+  return 0.0;
+}
+
+static double pair_dv(int kind, double r, const double *p) {  // -dV/dr (reference Morse: /r)
+  switch (kind) {
+    case NDG_MORSE:
+    case NDG_MORSE_PHYS: {
+      double x = p[3] <= 0.0 ? 0.0 : r - p[3];
+      double e = std::exp(-p[2] * x);
+      double d = 2.0 * p[1] * p[2] * (e * e - e);
+      return kind == NDG_MORSE ? d / r : d;
+    }
+    case NDG_BUCK: {
+      double r7 = r * r * r * r * r * r * r;
+      return p[2] <= 0.0 ? p[1] - 6.0 * p[3] / r7
+                         : p[1] * std::exp(-r / p[2]) / p[2] - 6.0 * p[3] / r7;
+    }
+    case NDG_LJ: {
+      double q = p[2] / r, q6 = q * q * q * q * q * q;
+      return 24.0 * p[1] * (2.0 * q6 * q6 / r - q6 / r);
+    }
+  }
+  return 0.0;
+}
+
+extern "C" int ndg_pair_table(int kind, const double params[4], double ptrmin, int ptsize,
+                              double *P, double *F, double *rc2, double *dr2) {
+  if (kind < NDG_MORSE || kind > NDG_MORSE_PHYS || ptsize < 2 || !P || !F)
+    return fail(NDG_ERR_ARG, "ndg_pair_table: bad arguments");
+  const double rc = params[0];
+  const double c2 = rc * rc;
+  const double h = (c2 - ptrmin) / (double)(ptsize - 1);
+  for (int i = 0; i <= ptsize; ++i) {
+    const double s = ptrmin + (double)i * h;
+    const double r = std::sqrt(s);
+    P[i] = pair_v(kind, r, params);
+    F[i] = r * pair_dv(kind, r, params);
+  }
+  if (rc2) *rc2 = c2;
+  if (dr2) *dr2 = h;
+  return NDG_OK;
+}
+
+// ------------------------------------------------------------------ engine structures
+
+namespace {
+
+struct TableBuf {       // device copy of one replica's pair tables (shared when identical)
+  int refs = 0;
+  void *block = nullptr;
+  size_t bytes = 0;
+  double *F64 = nullptr, *P64 = nullptr;
+  float2 *F2 = nullptr, *P2 = nullptr;
+  std::vector<std::vector<float>> F32;  // per pair, host copy for the LDS table set
+  std::vector<uint64_t> phash;          // per pair content hash
+};
+
+struct HostState {      // caller-provided data for one replica (caller order)
+  int np = 0, ntypes = 0, npairs = 0, ptsize = 0;
+  std::vector<double> P, F, rc2, dr2, pairparams, mass;
+  std::vector<int> pairmat, typ;
+  double ptrmin = 0.25;
+  std::vector<double> pos, vel, acc;
+  double box[3] = {0, 0, 0};
+  bool have_tables = false, have_state = false;
+};
+
+struct Slot {
+  bool used = false;
+  int np = 0, ntypes = 0, npairs = 0, ptsize = 0, neimax = 0, tpa = 1, ncell_cap = 0;
+  int ntiles = 0, tile_base = 0;
+  void *block = nullptr;
+  size_t bytes = 0;
+  uint64_t table_key = 0;
+  RepParams P{};
+  ndg_run_params rp{};
+  // host mirror of the step schedule
+  int step = 0, run_step0 = 0, run_nsteps = 0;
+  bool finished = true;
+  int nrec_seen = 0;
+  std::vector<ndg_thermo> pending_rec;
+  ndg_thermo last{};
+  int err = 0;
+  long long nbr_sum = 0;
+};
+
+}  // namespace
+
+struct ndg_engine {
+  int path = NDG_PATH_MIXED;
+  size_t budget = 0, used = 0;
+  hipStream_t stream = nullptr;
+  std::vector<Slot> slots;
+  RepParams *d_reps = nullptr;
+  RepState *d_st = nullptr;
+  Tile *d_tiles = nullptr;
+  int cap_slots = 0, cap_tiles = 0;
+  std::vector<Tile> h_tiles;
+  std::vector<std::pair<int, int>> tpa_ranges;  // (tpa, t0) in tile order, nt follows
+  std::vector<int> tpa_counts;
+  bool tiles_dirty = true;
+  bool any_large = false;
+  std::map<uint64_t, TableBuf> tables;
+  std::multimap<size_t, void *> free_blocks;
+  int tpa_override = 0;
+  int graph_steps = 0;
+  // timing of the force kernel
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending;
+  std::vector<hipEvent_t> ev_pool;
+  double t_ms = 0.0;
+  long long t_count = 0;
+  double t_pairs = 0.0, t_atoms = 0.0;
+  // graphs
+  std::map<std::string, hipGraphExec_t> graphs;
+  // distinct FP32 force tables of all replicas (staged in LDS by the force kernel)
+  float *d_tabset = nullptr;
+  size_t tabset_cap = 0;
+  int tabset_n = 0;
+};
+
+struct ndg_system {
+  ndg_engine *e = nullptr;
+  int slot = -1;
+  HostState hs;
+  int neimax_req = 0;
+  int path = NDG_PATH_MIXED;
+};
+
+// ------------------------------------------------------------------ helpers
+
+static uint64_t fnv1a(const void *p, size_t n, uint64_t h = 1469598103934665603ull) {
+  const unsigned char *c = (const unsigned char *)p;
+  for (size_t i = 0; i < n; ++i) {
+    h ^= c[i];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+struct Layout {
+  size_t off[64];
+  int n = 0;
+  size_t total = 0;
+  size_t add(size_t bytes) {
+    size_t o = total;
+    off[n++] = o;
+    total = align_up(total + bytes, 256);
+    return o;
+  }
+};
+
+static int default_tpa(int np) {
+  (void)np;
+  return 1;
+}
+
+static int estimate_neimax(double rcut, double skin, int np, const double box[3]) {
+  // src/nd.f90:84-88 (single-precision literals 4./3. and 1.1 as in the reference)
+  const double c = (double)(4.0f / 3.0f) * 3.141592653589793;
+  const double rr = (rcut + skin) * (double)1.1f;
+  return (int)(c * (rr * rr * rr) * (double)np / (box[0] * box[1] * box[2]));
+}
+
+static void grid_for_box(const double box[3], double rcut, double skin, int m[3], int &brute) {
+  for (int a = 0; a < 3; ++a) m[a] = (int)(box[a] / (rcut + skin));
+  brute = (m[0] < 3 || m[1] < 3 || m[2] < 3) ? 1 : 0;
+  if (brute) m[0] = m[1] = m[2] = 1;
+}
+
+// carve a replica block; returns total bytes (block == nullptr: size only)
+static size_t carve(Slot &s, int path, char *block) {
+  const int np = s.np;
+  const int ntiles = (np + kTile - 1) / kTile;
+  const int C = (s.neimax + s.tpa - 1) / s.tpa;
+  Layout L;
+  size_t o_mass = L.add(sizeof(double) * s.ntypes);
+  size_t o_pm = L.add(sizeof(int) * s.ntypes * s.ntypes);
+  size_t o_rcp = L.add(sizeof(double) * s.npairs);
+  size_t o_rc2d = L.add(sizeof(double) * s.npairs);
+  size_t o_rc2f = L.add(sizeof(float) * s.npairs);
+  size_t o_vdr2 = L.add(sizeof(float) * s.npairs);
+  size_t o_d[16];
+  for (int q = 0; q < 16; ++q) o_d[q] = L.add(sizeof(double) * np);  // px..pbz (16 arrays)
+  size_t o_f32a = L.add(sizeof(float4) * np);
+  size_t o_f32b = L.add(sizeof(float4) * np);
+  size_t o_x64a = L.add(sizeof(double4) * np);
+  size_t o_x32b = L.add(sizeof(float4) * np);
+  size_t o_anc = L.add(sizeof(int4) * np);
+  size_t o_typ = L.add(sizeof(int) * np);
+  size_t o_oid = L.add(sizeof(int) * np);
+  size_t o_nnb = L.add(sizeof(int) * np);
+  size_t o_list = L.add(sizeof(unsigned) * (size_t)ntiles * C * kTile * s.tpa);
+  size_t o_cc = L.add(sizeof(int) * s.ncell_cap);
+  size_t o_cs = L.add(sizeof(int) * (s.ncell_cap + 1));
+  size_t o_cf = L.add(sizeof(int) * s.ncell_cap);
+  size_t o_key = L.add(sizeof(int) * np);
+  size_t o_src = L.add(sizeof(int) * np);
+  size_t o_tmpd = L.add(sizeof(double) * 9 * (size_t)np);
+  size_t o_tmpf = L.add(sizeof(float4) * 2 * (size_t)np);
+  size_t o_tmpi = L.add(sizeof(int) * 2 * (size_t)np);
+  size_t o_part = L.add(sizeof(double) * kPartials * ntiles);
+  size_t o_ring = L.add(sizeof(ndg_thermo) * kRing);
+  if (block) {
+    RepParams &P = s.P;
+    P.mass = (const double *)(block + o_mass);
+    P.pairmat = (const int *)(block + o_pm);
+    P.rc_pair = (const double *)(block + o_rcp);
+    P.rc2d = (const double *)(block + o_rc2d);
+    P.rc2f = (const float *)(block + o_rc2f);
+    P.vdr2 = (const float *)(block + o_vdr2);
+    double **dp[16] = {&P.px, &P.py, &P.pz, &P.vx, &P.vy, &P.vz, &P.ax, &P.ay,
+                       &P.az, &P.fx, &P.fy, &P.fz, &P.pene, &P.pbx, &P.pby, &P.pbz};
+    for (int q = 0; q < 16; ++q) *dp[q] = (double *)(block + o_d[q]);
+    P.f32a = (float4 *)(block + o_f32a);
+    P.f32b = (float4 *)(block + o_f32b);
+    P.x64a = (double4 *)(block + o_x64a);
+    P.x32b = (float4 *)(block + o_x32b);
+    P.anchor = (int4 *)(block + o_anc);
+    P.typ = (int *)(block + o_typ);
+    P.oid = (int *)(block + o_oid);
+    P.nnb = (int *)(block + o_nnb);
+    P.list = (unsigned *)(block + o_list);
+    P.cell_count = (int *)(block + o_cc);
+    P.cell_start = (int *)(block + o_cs);
+    P.cell_fill = (int *)(block + o_cf);
+    P.key = (int *)(block + o_key);
+    P.slot_src = (int *)(block + o_src);
+    P.tmpd = (double *)(block + o_tmpd);
+    P.tmpf = (float4 *)(block + o_tmpf);
+    P.tmpi = (int *)(block + o_tmpi);
+    P.partials = (double *)(block + o_part);
+    P.ring = (ndg_thermo *)(block + o_ring);
+    P.np = np;
+    P.ntiles = ntiles;
+    P.tpa = s.tpa;
+    P.neimax = s.neimax;
+    P.list_chunks = C;
+    P.ncell_cap = s.ncell_cap;
+    P.ntypes = s.ntypes;
+    P.npairs = s.npairs;
+    P.ptsize = s.ptsize;
+    P.path = path;
+  }
+  (void)path;
+  return L.total;
+}
+
+static size_t table_bytes(int npairs, int ptsize) {
+  const size_t n = (size_t)npairs * (ptsize + 1);
+  return align_up(n * 8, 256) * 4;
+}
+
+static int dev_alloc(ndg_engine *e, size_t bytes, void **p) {
+  auto it = e->free_blocks.lower_bound(bytes);
+  if (it != e->free_blocks.end() && it->first <= bytes + bytes / 4 + (1 << 20)) {
+    *p = it->second;
+    e->free_blocks.erase(it);
+    return NDG_OK;
+  }
+  if (e->used + bytes > e->budget) {
+    // give cached blocks back before refusing
+    for (auto &fb : e->free_blocks) {
+      hipFree(fb.second);
+      e->used -= fb.first;
+    }
+    e->free_blocks.clear();
+    if (e->used + bytes > e->budget) return fail(NDG_ERR_NOMEM, "HBM budget exhausted");
+  }
+  hipError_t r = hipMalloc(p, bytes);
+  if (r != hipSuccess) return fail(NDG_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(r));
+  e->used += bytes;
+  return NDG_OK;
+}
+
+static void dev_release(ndg_engine *e, void *p, size_t bytes) {
+  if (p) e->free_blocks.emplace(bytes, p);
+}
+
+static int ensure_capacity(ndg_engine *e) {
+  int ns = (int)e->slots.size();
+  if (ns > e->cap_slots) {
+    int cap = std::max(ns, std::max(8, 2 * e->cap_slots));
+    RepParams *nr;
+    RepState *nst;
+    HIPCHK(hipMalloc(&nr, sizeof(RepParams) * cap));
+    HIPCHK(hipMalloc(&nst, sizeof(RepState) * cap));
+    HIPCHK(hipMemsetAsync(nst, 0, sizeof(RepState) * cap, e->stream));
+    HIPCHK(hipMemsetAsync(nr, 0, sizeof(RepParams) * cap, e->stream));
+    if (e->d_reps) {
+      HIPCHK(hipMemcpyAsync(nr, e->d_reps, sizeof(RepParams) * e->cap_slots, hipMemcpyDeviceToDevice, e->stream));
+      HIPCHK(hipMemcpyAsync(nst, e->d_st, sizeof(RepState) * e->cap_slots, hipMemcpyDeviceToDevice, e->stream));
+      HIPCHK(hipStreamSynchronize(e->stream));
+      hipFree(e->d_reps);
+      hipFree(e->d_st);
+    }
+    e->d_reps = nr;
+    e->d_st = nst;
+    e->cap_slots = cap;
+    for (auto &g : e->graphs) hipGraphExecDestroy(g.second);
+    e->graphs.clear();
+  }
+  return NDG_OK;
+}
+
+// Distinct FP32 force tables of all replicas, concatenated; each replica gets the offset
+// of each of its pairs (or -1 when the set exceeds the LDS budget).
+static int rebuild_tabset(ndg_engine *e) {
+  std::map<uint64_t, int> off;
+  std::vector<float> all;
+  for (auto &s : e->slots) {
+    if (!s.used || !s.table_key) continue;
+    auto it = e->tables.find(s.table_key);
+    if (it == e->tables.end()) continue;
+    for (int p = 0; p < s.npairs; ++p) {
+      const uint64_t h = it->second.phash[p];
+      if (!off.count(h)) {
+        off[h] = (int)all.size();
+        all.insert(all.end(), it->second.F32[p].begin(), it->second.F32[p].end());
+      }
+    }
+  }
+  const bool fits = !all.empty() && all.size() * sizeof(float) <= (size_t)kLdsTableBytes;
+  for (auto &s : e->slots) {
+    for (int p = 0; p < 16; ++p) s.P.ltab[p] = -1;
+    if (!s.used || !s.table_key || !fits) continue;
+    auto it = e->tables.find(s.table_key);
+    for (int p = 0; p < s.npairs; ++p) s.P.ltab[p] = off[it->second.phash[p]];
+  }
+  if (fits) {
+    if (all.size() > e->tabset_cap) {
+      if (e->d_tabset) {
+        HIPCHK(hipStreamSynchronize(e->stream));
+        hipFree(e->d_tabset);
+      }
+      HIPCHK(hipMalloc(&e->d_tabset, sizeof(float) * all.size()));
+      e->tabset_cap = all.size();
+    }
+    HIPCHK(hipMemcpyAsync(e->d_tabset, all.data(), sizeof(float) * all.size(), hipMemcpyHostToDevice, e->stream));
+  }
+  e->tabset_n = fits ? (int)all.size() : 0;
+  for (size_t r = 0; r < e->slots.size(); ++r)
+    if (e->slots[r].used && e->slots[r].block)
+      HIPCHK(hipMemcpyAsync(e->d_reps + r, &e->slots[r].P, sizeof(RepParams), hipMemcpyHostToDevice, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return NDG_OK;
+}
+
+static int rebuild_tiles(ndg_engine *e) {
+  if (!e->tiles_dirty) return NDG_OK;
+  int rc0 = rebuild_tabset(e);
+  if (rc0) return rc0;
+  e->h_tiles.clear();
+  e->tpa_ranges.clear();
+  e->tpa_counts.clear();
+  e->any_large = false;
+  const int tpas[4] = {1, 2, 4, 8};
+  for (int q = 0; q < 4; ++q) {
+    int t0 = (int)e->h_tiles.size();
+    for (size_t r = 0; r < e->slots.size(); ++r) {
+      Slot &s = e->slots[r];
+      if (!s.used || s.tpa != tpas[q]) continue;
+      s.tile_base = (int)e->h_tiles.size();
+      for (int lt = 0; lt < s.ntiles; ++lt) e->h_tiles.push_back(Tile{(int)r, lt * kTile, lt, 0});
+      if (s.np > kSmallSortHost) e->any_large = true;
+    }
+    int nt = (int)e->h_tiles.size() - t0;
+    if (nt > 0) {
+      e->tpa_ranges.push_back({tpas[q], t0});
+      e->tpa_counts.push_back(nt);
+    }
+  }
+  int nt = (int)e->h_tiles.size();
+  if (nt > e->cap_tiles) {
+    if (e->d_tiles) {
+      HIPCHK(hipStreamSynchronize(e->stream));
+      hipFree(e->d_tiles);
+    }
+    e->cap_tiles = std::max(nt, 2 * e->cap_tiles);
+    HIPCHK(hipMalloc(&e->d_tiles, sizeof(Tile) * e->cap_tiles));
+  }
+  if (nt > 0)
+    HIPCHK(hipMemcpyAsync(e->d_tiles, e->h_tiles.data(), sizeof(Tile) * nt, hipMemcpyHostToDevice, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  e->tiles_dirty = false;
+  for (auto &g : e->graphs) hipGraphExecDestroy(g.second);
+  e->graphs.clear();
+  return NDG_OK;
+}
+
+static EngineDev dev_view(ndg_engine *e) {
+  EngineDev E;
+  E.reps = e->d_reps;
+  E.st = e->d_st;
+  E.tiles = e->d_tiles;
+  E.ntiles = (int)e->h_tiles.size();
+  E.nreps = (int)e->slots.size();
+  E.tabset = e->d_tabset;
+  E.tabset_n = e->tabset_n;
+  return E;
+}
+
+static int upload_params(ndg_engine *e, int r) {
+  HIPCHK(hipMemcpyAsync(e->d_reps + r, &e->slots[r].P, sizeof(RepParams), hipMemcpyHostToDevice,
+                        e->stream));
+  return NDG_OK;
+}
+
+static int get_state_dev(ndg_engine *e, int r, RepState *S) {
+  HIPCHK(hipMemcpyAsync(S, e->d_st + r, sizeof(RepState), hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return NDG_OK;
+}
+
+static int put_state_dev(ndg_engine *e, int r, const RepState *S) {
+  HIPCHK(hipMemcpyAsync(e->d_st + r, S, sizeof(RepState), hipMemcpyHostToDevice, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return NDG_OK;
+}
+
+// Tables: dedupe identical tables between replicas.
+static int attach_tables(ndg_engine *e, Slot &s, const HostState &h) {
+  const size_t n = (size_t)h.npairs * (h.ptsize + 1);
+  uint64_t key = fnv1a(h.P.data(), n * 8);
+  key = fnv1a(h.F.data(), n * 8, key);
+  key = fnv1a(&h.npairs, sizeof(int), key);
+  key = fnv1a(&h.ptsize, sizeof(int), key);
+  auto it = e->tables.find(key);
+  if (it == e->tables.end()) {
+    TableBuf tb;
+    tb.bytes = table_bytes(h.npairs, h.ptsize);
+    int rc = dev_alloc(e, tb.bytes, &tb.block);
+    if (rc) return rc;
+    char *b = (char *)tb.block;
+    tb.F64 = (double *)b;
+    tb.P64 = (double *)(b + align_up(n * 8, 256));
+    tb.F2 = (float2 *)(b + 2 * align_up(n * 8, 256));
+    tb.P2 = (float2 *)(b + 3 * align_up(n * 8, 256));
+    // FP32 interpolation tables {T_k, T_k+1 - T_k} (difference formed in FP64)
+    std::vector<float2> f2(n), p2(n);
+    const int tab = h.ptsize + 1;
+    for (int p = 0; p < h.npairs; ++p)
+      for (int k = 0; k < tab; ++k) {
+        const size_t q = (size_t)p * tab + k;
+        const double dF = k + 1 < tab ? h.F[q + 1] - h.F[q] : 0.0;
+        const double dP = k + 1 < tab ? h.P[q + 1] - h.P[q] : 0.0;
+        f2[q] = make_float2((float)h.F[q], (float)dF);
+        p2[q] = make_float2((float)h.P[q], (float)dP);
+      }
+    HIPCHK(hipMemcpyAsync(tb.F64, h.F.data(), n * 8, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(tb.P64, h.P.data(), n * 8, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(tb.F2, f2.data(), n * 8, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(tb.P2, p2.data(), n * 8, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    tb.F32.resize(h.npairs);
+    tb.phash.resize(h.npairs);
+    for (int p = 0; p < h.npairs; ++p) {
+      tb.F32[p].resize(tab);
+      for (int k = 0; k < tab; ++k) tb.F32[p][k] = (float)h.F[(size_t)p * tab + k];
+      tb.phash[p] = fnv1a(tb.F32[p].data(), 4 * (size_t)tab);
+    }
+    it = e->tables.emplace(key, tb).first;
+  }
+  it->second.refs += 1;
+  s.table_key = key;
+  s.P.tabF64 = it->second.F64;
+  s.P.tabP64 = it->second.P64;
+  s.P.tabF2 = it->second.F2;
+  s.P.tabP2 = it->second.P2;
+  return NDG_OK;
+}
+
+static void detach_tables(ndg_engine *e, Slot &s) {
+  auto it = e->tables.find(s.table_key);
+  if (it == e->tables.end()) return;
+  if (--it->second.refs == 0) {
+    dev_release(e, it->second.block, it->second.bytes);
+    e->tables.erase(it);
+  }
+}
+
+static void set_run_params(Slot &s, const ndg_run_params &rp, int step0) {
+  RepParams &P = s.P;
+  s.rp = rp;
+  P.run_step0 = step0;
+  P.run_nsteps = rp.nsteps;
+  P.out_steps = rp.out_steps > 0 ? rp.out_steps : (rp.nsteps > 0 ? rp.nsteps : 1);
+  P.renei = rp.renei;
+  P.renei_freq = rp.renei_freq > 0 ? rp.renei_freq : 1;
+  P.trigger = rp.trigger;
+  P.isokinetic = rp.isokinetic;
+  P.ke_freq = rp.ke_freq > 0 ? rp.ke_freq : 1;
+  P.isobaric = rp.isobaric;
+  P.p_freq = rp.p_freq > 0 ? rp.p_freq : 1;
+  P.isoenergetic = rp.isoenergetic;
+  P.e_freq = rp.e_freq > 0 ? rp.e_freq : 1;
+  P.box_load = rp.box_load;
+  P.dt = rp.dt;
+  P.ke_target = rp.ke_target;
+  P.ke_tau = rp.ke_tau;
+  for (int a = 0; a < 3; ++a) {
+    P.p_target[a] = rp.p_target[a];
+    P.box_vel[a] = rp.box_vel[a];
+  }
+  P.p_tau = rp.p_tau;
+  P.e_tau = rp.e_tau;
+  s.step = step0;
+  s.run_step0 = step0;
+  s.run_nsteps = rp.nsteps;
+  s.finished = rp.nsteps <= 0;
+}
+
+// Upload caller data into a slot's block (sorted order == caller order until the first
+// build; the build sorts and keeps oid).
+static int upload_host(ndg_engine *e, Slot &s, const HostState &h, bool state_too) {
+  RepParams &P = s.P;
+  const int np = h.np;
+  std::vector<double> rcp(h.npairs);
+  std::vector<float> rc2f(h.npairs), vdr2(h.npairs);
+  for (int p = 0; p < h.npairs; ++p) {
+    rcp[p] = h.pairparams[4 * p];
+    rc2f[p] = (float)h.rc2[p];
+    vdr2[p] = (float)(1.0 / h.dr2[p]);
+  }
+  double rcut = 0.0;
+  for (double r : rcp) rcut = std::max(rcut, r);
+  P.rcut = rcut;
+  P.ptrmin = h.ptrmin;
+  hipStream_t st = e->stream;
+  HIPCHK(hipMemcpyAsync((void *)P.mass, h.mass.data(), 8 * h.ntypes, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync((void *)P.pairmat, h.pairmat.data(), 4 * h.ntypes * h.ntypes, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync((void *)P.rc_pair, rcp.data(), 8 * h.npairs, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync((void *)P.rc2d, h.rc2.data(), 8 * h.npairs, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync((void *)P.rc2f, rc2f.data(), 4 * h.npairs, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync((void *)P.vdr2, vdr2.data(), 4 * h.npairs, hipMemcpyHostToDevice, st));
+  if (state_too) {
+    std::vector<double> soa(3 * (size_t)np);
+    auto put3 = [&](const std::vector<double> &aos, double *dx, double *dy, double *dz) -> int {
+      for (int i = 0; i < np; ++i) {
+        soa[i] = aos.empty() ? 0.0 : aos[3 * i];
+        soa[np + i] = aos.empty() ? 0.0 : aos[3 * i + 1];
+        soa[2 * np + i] = aos.empty() ? 0.0 : aos[3 * i + 2];
+      }
+      HIPCHK(hipMemcpyAsync(dx, soa.data(), 8 * np, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(dy, soa.data() + np, 8 * np, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(dz, soa.data() + 2 * np, 8 * np, hipMemcpyHostToDevice, st));
+      HIPCHK(hipStreamSynchronize(st));
+      return NDG_OK;
+    };
+    int rc;
+    if ((rc = put3(h.pos, P.px, P.py, P.pz))) return rc;
+    if ((rc = put3(h.vel, P.vx, P.vy, P.vz))) return rc;
+    if ((rc = put3(h.acc, P.ax, P.ay, P.az))) return rc;
+    // mixed path keeps the latest force (F = m a) in FP32
+    std::vector<float4> f4(np);
+    for (int i = 0; i < np; ++i) {
+      const double m = h.mass[h.typ[i] - 1];
+      if (h.acc.empty())
+        f4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+      else
+        f4[i] = make_float4((float)(h.acc[3 * i] * m), (float)(h.acc[3 * i + 1] * m),
+                            (float)(h.acc[3 * i + 2] * m), 0.f);
+    }
+    std::vector<int> oid(np);
+    for (int i = 0; i < np; ++i) oid[i] = i;
+    HIPCHK(hipMemcpyAsync(P.f32a, f4.data(), 16 * np, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(P.f32b, f4.data(), 16 * np, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(P.typ, h.typ.data(), 4 * np, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(P.oid, oid.data(), 4 * np, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemsetAsync(P.fx, 0, 8 * np, st));
+    HIPCHK(hipMemsetAsync(P.fy, 0, 8 * np, st));
+    HIPCHK(hipMemsetAsync(P.fz, 0, 8 * np, st));
+    HIPCHK(hipMemsetAsync(P.pene, 0, 8 * np, st));
+    HIPCHK(hipMemsetAsync(P.nnb, 0, 4 * np, st));
+    HIPCHK(hipMemsetAsync(P.cell_count, 0, 4 * s.ncell_cap, st));
+    HIPCHK(hipMemsetAsync(P.cell_fill, 0, 4 * s.ncell_cap, st));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  return NDG_OK;
+}
+
+// Download the slot's state in caller order.
+static int download_state(ndg_engine *e, Slot &s, double *pos, double *vel, double *acc,
+                          double *force, double *pene, double box[3]) {
+  const int np = s.np;
+  RepParams &P = s.P;
+  hipStream_t st = e->stream;
+  RepState S;
+  int rc = get_state_dev(e, (int)(&s - &e->slots[0]), &S);
+  if (rc) return rc;
+  std::vector<int> oid(np), typ(np);
+  HIPCHK(hipMemcpyAsync(oid.data(), P.oid, 4 * np, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(typ.data(), P.typ, 4 * np, hipMemcpyDeviceToHost, st));
+  std::vector<double> buf(3 * (size_t)np);
+  auto get3 = [&](double *dx, double *dy, double *dz, double *out) -> int {
+    HIPCHK(hipMemcpyAsync(buf.data(), dx, 8 * np, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(buf.data() + np, dy, 8 * np, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(buf.data() + 2 * np, dz, 8 * np, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int i = 0; i < np; ++i) {
+      const int o = oid[i];
+      out[3 * o] = buf[i];
+      out[3 * o + 1] = buf[np + i];
+      out[3 * o + 2] = buf[2 * np + i];
+    }
+    return NDG_OK;
+  };
+  HIPCHK(hipStreamSynchronize(st));
+  if (pos && (rc = get3(P.px, P.py, P.pz, pos))) return rc;
+  if (vel && (rc = get3(P.vx, P.vy, P.vz, vel))) return rc;
+  if (P.path == NDG_PATH_FP64) {
+    if (acc && (rc = get3(P.ax, P.ay, P.az, acc))) return rc;
+    if (force && (rc = get3(P.fx, P.fy, P.fz, force))) return rc;
+    if (pene) {
+      HIPCHK(hipMemcpyAsync(buf.data(), P.pene, 8 * np, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      for (int i = 0; i < np; ++i) pene[oid[i]] = buf[i];
+    }
+  } else if (acc || force || pene) {
+    std::vector<float4> f4(np);
+    // latest forces: the buffer a force evaluation wrote if no kick has consumed it yet
+    const int latest = S.parity ^ S.fnew;
+    HIPCHK(hipMemcpyAsync(f4.data(), latest ? P.f32b : P.f32a, 16 * np, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<double> mass(s.ntypes);
+    HIPCHK(hipMemcpy(mass.data(), P.mass, 8 * s.ntypes, hipMemcpyDeviceToHost));
+    for (int i = 0; i < np; ++i) {
+      const int o = oid[i];
+      const double rm = 1.0 / mass[typ[i] - 1];
+      if (force) {
+        force[3 * o] = f4[i].x;
+        force[3 * o + 1] = f4[i].y;
+        force[3 * o + 2] = f4[i].z;
+      }
+      if (acc) {
+        acc[3 * o] = (double)f4[i].x * rm;
+        acc[3 * o + 1] = (double)f4[i].y * rm;
+        acc[3 * o + 2] = (double)f4[i].z * rm;
+      }
+      if (pene) pene[o] = f4[i].w;
+    }
+  }
+  if (box)
+    for (int a = 0; a < 3; ++a) box[a] = S.box[a];
+  return NDG_OK;
+}
+
+// Allocate (or grow) a slot's block for the given list capacity and cell grid.
+static int alloc_slot(ndg_engine *e, int r, int neimax, int ncell_cap, int tpa) {
+  Slot &s = e->slots[r];
+  Slot probe = s;
+  probe.neimax = neimax;
+  probe.ncell_cap = ncell_cap;
+  probe.tpa = tpa;
+  const size_t need = carve(probe, e->path, nullptr);
+  void *blk;
+  int rc = dev_alloc(e, need, &blk);
+  if (rc) return rc;
+  if (s.block) {
+    HIPCHK(hipStreamSynchronize(e->stream));
+    dev_release(e, s.block, s.bytes);
+  }
+  const RepParams keep = s.P;
+  s.neimax = neimax;
+  s.ncell_cap = ncell_cap;
+  s.tpa = tpa;
+  s.ntiles = (s.np + kTile - 1) / kTile;
+  s.block = blk;
+  s.bytes = need;
+  carve(s, e->path, (char *)blk);
+  // keep the table pointers and scalars that are not part of the block
+  s.P.tabF64 = keep.tabF64;
+  s.P.tabP64 = keep.tabP64;
+  s.P.tabF2 = keep.tabF2;
+  s.P.tabP2 = keep.tabP2;
+  e->tiles_dirty = true;
+  return NDG_OK;
+}
+
+// Launch a full rebuild of one slot now (run begin / drop-in build).
+static int build_slot_now(ndg_engine *e, int r, double skin) {
+  Slot &s = e->slots[r];
+  int rc = rebuild_tiles(e);
+  if (rc) return rc;
+  RepState S;
+  if ((rc = get_state_dev(e, r, &S))) return rc;
+  int m[3], brute;
+  grid_for_box(S.box, s.P.rcut, skin, m, brute);
+  if (!brute && (long long)m[0] * m[1] * m[2] > s.ncell_cap) return fail(NDG_ERR_ARG, "cell grid exceeds capacity");
+  for (int a = 0; a < 3; ++a) S.mcell[a] = m[a];
+  S.brute = brute;
+  S.rebuild = 1;
+  S.err = 0;
+  s.P.skin = skin;
+  if ((rc = upload_params(e, r))) return rc;
+  if ((rc = put_state_dev(e, r, &S))) return rc;
+  EngineDev E = dev_view(e);
+  launch_rebuild(E, s.tile_base, s.ntiles, r, 1, s.np > kSmallSortHost, e->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(e->stream));
+  if ((rc = get_state_dev(e, r, &S))) return rc;
+  S.rebuild = 0;
+  // no displacement yet
+  std::vector<double> part((size_t)s.ntiles * kPartials, 0.0);
+  for (int lt = 0; lt < s.ntiles; ++lt) {
+    part[(size_t)lt * kPartials + 8] = -1.0;
+    part[(size_t)lt * kPartials + 9] = -1.0;
+  }
+  HIPCHK(hipMemcpyAsync(s.P.partials, part.data(), 8 * part.size(), hipMemcpyHostToDevice, e->stream));
+  if ((rc = put_state_dev(e, r, &S))) return rc;
+  s.nbr_sum = S.nbr_sum;
+  if (S.err) return fail(S.err, err_text(S.err));
+  return NDG_OK;
+}
+
+static int new_slot(ndg_engine *e) {
+  for (size_t r = 0; r < e->slots.size(); ++r)
+    if (!e->slots[r].used) {
+      e->slots[r] = Slot();
+      e->slots[r].used = true;
+      return (int)r;
+    }
+  e->slots.push_back(Slot());
+  e->slots.back().used = true;
+  return (int)e->slots.size() - 1;
+}
+
+// ------------------------------------------------------------------ step scheduling
+
+struct TickPlan {
+  bool full = false, check = false, any = false;
+};
+
+static TickPlan plan_tick(ndg_engine *e, std::vector<int> *active) {
+  TickPlan tp;
+  for (size_t r = 0; r < e->slots.size(); ++r) {
+    Slot &s = e->slots[r];
+    if (!s.used || s.finished || s.err) continue;
+    RepState S{};
+    S.step = s.step;
+    const int f = step_flags(s.P, S);
+    if (!(f & F_ACTIVE)) continue;
+    tp.any = true;
+    if (f & (F_CU | F_CV)) tp.full = true;
+    if (f & F_CHECK) tp.check = true;
+    if (active) active->push_back((int)r);
+  }
+  return tp;
+}
+
+static void advance_mirror(ndg_engine *e) {
+  for (auto &s : e->slots) {
+    if (!s.used || s.finished || s.err) continue;
+    s.step += 1;
+    if (s.step - s.run_step0 >= s.run_nsteps) s.finished = true;
+  }
+}
+
+static void time_begin(ndg_engine *e, hipEvent_t *a) {
+  if (!e->timing) return;
+  if (e->ev_pool.size() < 2) {
+    hipEvent_t x, y;
+    hipEventCreate(&x);
+    hipEventCreate(&y);
+    e->ev_pool.push_back(x);
+    e->ev_pool.push_back(y);
+  }
+  *a = e->ev_pool.back();
+  e->ev_pool.pop_back();
+  hipEventRecord(*a, e->stream);
+}
+
+static void time_end(ndg_engine *e, hipEvent_t a) {
+  if (!e->timing) return;
+  hipEvent_t b = e->ev_pool.empty() ? nullptr : e->ev_pool.back();
+  if (!b) hipEventCreate(&b); else e->ev_pool.pop_back();
+  hipEventRecord(b, e->stream);
+  e->ev_pending.push_back({a, b});
+}
+
+static void time_collect(ndg_engine *e) {
+  for (auto &p : e->ev_pending) {
+    float ms = 0.f;
+    hipEventSynchronize(p.second);
+    hipEventElapsedTime(&ms, p.first, p.second);
+    e->t_ms += ms;
+    e->t_count += 1;
+    e->ev_pool.push_back(p.first);
+    e->ev_pool.push_back(p.second);
+  }
+  e->ev_pending.clear();
+}
+
+// Kernels of one step for the batched (mixed) path: every active replica in one launch
+// per kernel.
+static void enqueue_step_batched(ndg_engine *e, const TickPlan &tp) {
+  EngineDev E = dev_view(e);
+  const int nt = (int)e->h_tiles.size(), nr = (int)e->slots.size();
+  hipStream_t s = e->stream;
+  if (tp.check) launch_rebuild(E, 0, nt, 0, nr, e->any_large, s);
+  launch_drift(E, e->path, 0, nt, 0, s);
+  hipEvent_t a = nullptr;
+  time_begin(e, &a);
+  for (size_t q = 0; q < e->tpa_ranges.size(); ++q) {
+    launch_force(E, e->path, e->tpa_ranges[q].first, false, e->tpa_ranges[q].second,
+                 e->tpa_counts[q], 0, s);
+    if (tp.full)
+      launch_force(E, e->path, e->tpa_ranges[q].first, true, e->tpa_ranges[q].second,
+                   e->tpa_counts[q], 0, s);
+  }
+  time_end(e, a);
+  launch_kick(E, e->path, 0, nt, 0, s);
+  launch_reduce(E, 0, nr, RED_FORCE | RED_KICK | RED_STEP, 0, 0, s);
+}
+
+// Plain path: each replica is advanced with its own launches.
+static void enqueue_step_single(ndg_engine *e, int r, const TickPlan &tp) {
+  EngineDev E = dev_view(e);
+  Slot &sl = e->slots[r];
+  hipStream_t s = e->stream;
+  RepState S{};
+  S.step = sl.step;
+  const int f = step_flags(sl.P, S);
+  if (!(f & F_ACTIVE)) return;
+  if (f & F_CHECK) launch_rebuild(E, sl.tile_base, sl.ntiles, r, 1, sl.np > kSmallSortHost, s);
+  launch_drift(E, e->path, sl.tile_base, sl.ntiles, 0, s);
+  hipEvent_t a = nullptr;
+  time_begin(e, &a);
+  launch_force(E, e->path, sl.tpa, false, sl.tile_base, sl.ntiles, 0, s);
+  if (f & (F_CU | F_CV)) launch_force(E, e->path, sl.tpa, true, sl.tile_base, sl.ntiles, 0, s);
+  time_end(e, a);
+  launch_kick(E, e->path, sl.tile_base, sl.ntiles, 0, s);
+  launch_reduce(E, r, 1, RED_FORCE | RED_KICK | RED_STEP, 0, 0, s);
+  (void)tp;
+}
+
+static std::string tick_signature(const std::vector<TickPlan> &plans) {
+  std::string sig;
+  for (auto &p : plans) sig.push_back((char)('0' + (p.full ? 1 : 0) + (p.check ? 2 : 0)));
+  return sig;
+}
+
+// Drain device state and thermo rings after a chunk.
+static int drain(ndg_engine *e) {
+  const int nr = (int)e->slots.size();
+  if (nr == 0) return NDG_OK;
+  std::vector<RepState> S(nr);
+  HIPCHK(hipMemcpyAsync(S.data(), e->d_st, sizeof(RepState) * nr, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  int first_err = 0;
+  for (int r = 0; r < nr; ++r) {
+    Slot &s = e->slots[r];
+    if (!s.used) continue;
+    s.nbr_sum = S[r].nbr_sum;
+    if (S[r].err && !s.err) {
+      s.err = S[r].err;
+      if (!first_err) first_err = s.err;
+    }
+    const int have = S[r].nrec - s.nrec_seen;
+    if (have > 0) {
+      const int n = std::min(have, kRing);
+      std::vector<ndg_thermo> buf(kRing);
+      HIPCHK(hipMemcpy(buf.data(), s.P.ring, sizeof(ndg_thermo) * kRing, hipMemcpyDeviceToHost));
+      for (int k = S[r].nrec - n; k < S[r].nrec; ++k) s.pending_rec.push_back(buf[k % kRing]);
+      s.last = s.pending_rec.back();
+      s.nrec_seen = S[r].nrec;
+    }
+    // the device is the authority on the step counter
+    s.step = S[r].step;
+    if (S[r].done) s.finished = true;
+  }
+  time_collect(e);
+  if (first_err) return fail(first_err, err_text(first_err));
+  return NDG_OK;
+}
+
+// Advance all slots by up to nsteps steps.
+static int engine_steps(ndg_engine *e, int nsteps) {
+  int rc = rebuild_tiles(e);
+  if (rc) return rc;
+  int done_steps = 0;
+  const int chunk_cap = kRing / 2;  // keep thermo ring from wrapping between drains
+  while (done_steps < nsteps) {
+    int chunk = std::min(nsteps - done_steps, chunk_cap);
+    std::vector<TickPlan> plans;
+    for (int k = 0; k < chunk; ++k) {
+      TickPlan tp = plan_tick(e, nullptr);
+      if (!tp.any) break;
+      plans.push_back(tp);
+      advance_mirror(e);
+    }
+    if (plans.empty()) break;
+    const bool use_graph = e->path == NDG_PATH_MIXED && e->graph_steps > 0 && !e->timing;
+    if (e->path == NDG_PATH_MIXED) {
+      if (use_graph) {
+        size_t k = 0;
+        while (k < plans.size()) {
+          size_t len = std::min(plans.size() - k, (size_t)e->graph_steps);
+          std::vector<TickPlan> sub(plans.begin() + k, plans.begin() + k + len);
+          std::string sig = tick_signature(sub);
+          auto it = e->graphs.find(sig);
+          if (it == e->graphs.end()) {
+            hipGraph_t g;
+            HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+            for (auto &p : sub) enqueue_step_batched(e, p);
+            HIPCHK(hipStreamEndCapture(e->stream, &g));
+            hipGraphExec_t ge;
+            HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+            hipGraphDestroy(g);
+            it = e->graphs.emplace(sig, ge).first;
+          }
+          HIPCHK(hipGraphLaunch(it->second, e->stream));
+          k += len;
+        }
+      } else {
+        for (auto &p : plans) enqueue_step_batched(e, p);
+      }
+      HIPCHK(hipGetLastError());
+    }
+    done_steps += (int)plans.size();
+    if ((rc = drain(e))) return rc;
+  }
+  return NDG_OK;
+}
+
+// Plain FP64 path: one system per launch, replica after replica for each step.
+static int engine_steps_plain(ndg_engine *e, int nsteps) {
+  int rc = rebuild_tiles(e);
+  if (rc) return rc;
+  int done_steps = 0;
+  const int chunk_cap = kRing / 2;
+  while (done_steps < nsteps) {
+    int chunk = std::min(nsteps - done_steps, chunk_cap);
+    int issued = 0;
+    for (int k = 0; k < chunk; ++k) {
+      bool any = false;
+      for (size_t r = 0; r < e->slots.size(); ++r) {
+        Slot &s = e->slots[r];
+        if (!s.used || s.finished || s.err) continue;
+        TickPlan tp;
+        enqueue_step_single(e, (int)r, tp);
+        any = true;
+      }
+      if (!any) break;
+      advance_mirror(e);
+      ++issued;
+    }
+    HIPCHK(hipGetLastError());
+    if (issued == 0) break;
+    done_steps += issued;
+    if ((rc = drain(e))) return rc;
+  }
+  return NDG_OK;
+}
+
+static int run_engine(ndg_engine *e, int nsteps) {
+  return e->path == NDG_PATH_FP64 ? engine_steps_plain(e, nsteps) : engine_steps(e, nsteps);
+}
+
+// ------------------------------------------------------------------ engine lifecycle
+
+static int engine_new(ndg_engine **pe, int path, size_t budget) {
+  auto e = std::make_unique<ndg_engine>();
+  e->path = path;
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (ndev < 1) return fail(NDG_ERR_HIP, "no HIP device");
+  HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+  size_t fr = 0, tot = 0;
+  HIPCHK(hipMemGetInfo(&fr, &tot));
+  e->budget = budget ? budget : (size_t)(0.8 * (double)fr);
+  *pe = e.release();
+  return NDG_OK;
+}
+
+extern "C" int ndg_engine_create(ndg_engine_t **e, int path, size_t hbm_budget) {
+  if (!e || (path != NDG_PATH_FP64 && path != NDG_PATH_MIXED))
+    return fail(NDG_ERR_ARG, "ndg_engine_create: bad arguments");
+  return engine_new(e, path, hbm_budget);
+}
+
+extern "C" int ndg_engine_destroy(ndg_engine_t *e) {
+  if (!e) return NDG_OK;
+  hipStreamSynchronize(e->stream);
+  for (auto &g : e->graphs) hipGraphExecDestroy(g.second);
+  for (auto &s : e->slots)
+    if (s.block) hipFree(s.block);
+  for (auto &t : e->tables) hipFree(t.second.block);
+  for (auto &fb : e->free_blocks) hipFree(fb.second);
+  for (auto &p : e->ev_pending) {
+    hipEventDestroy(p.first);
+    hipEventDestroy(p.second);
+  }
+  for (auto &ev : e->ev_pool) hipEventDestroy(ev);
+  if (e->d_reps) hipFree(e->d_reps);
+  if (e->d_st) hipFree(e->d_st);
+  if (e->d_tiles) hipFree(e->d_tiles);
+  if (e->d_tabset) hipFree(e->d_tabset);
+  hipStreamDestroy(e->stream);
+  delete e;
+  return NDG_OK;
+}
+
+static int ncell_capacity(const double box[3], double rcut, double skin) {
+  long long c = 1;
+  for (int a = 0; a < 3; ++a) c *= (long long)(1.5 * box[a] / (rcut + skin)) + 1;
+  return (int)std::min<long long>(c, 1 << 26);
+}
+
+extern "C" size_t ndg_footprint(int np, int ntypes, int npairs, int ptsize, int neimax) {
+  Slot s;
+  s.np = np;
+  s.ntypes = ntypes;
+  s.npairs = npairs;
+  s.ptsize = ptsize;
+  s.neimax = neimax;
+  s.tpa = default_tpa(np);
+  s.ncell_cap = np / 8 + 64;
+  return carve(s, NDG_PATH_MIXED, nullptr) + table_bytes(npairs, ptsize);
+}
+
+static int check_host(const HostState &h) {
+  if (h.np <= 0 || h.np > (int)kIndexMask) return fail(NDG_ERR_ARG, "np out of range");
+  if (h.ntypes < 1 || h.ntypes > 8 || h.npairs < 1 || h.npairs > 16)
+    return fail(NDG_ERR_ARG, "ntypes must be 1..8 and npairs 1..16");
+  for (int t : h.typ)
+    if (t < 1 || t > h.ntypes) return fail(NDG_ERR_ARG, "atom type out of range");
+  for (int p : h.pairmat)
+    if (p < 0 || p > h.npairs) return fail(NDG_ERR_ARG, "pair id out of range");
+  // symmetric pairs only (compute0); asymmetric matrices are the reference's compute1
+  for (int a = 0; a < h.ntypes; ++a)
+    for (int b = 0; b < h.ntypes; ++b)
+      if (h.pairmat[a + b * h.ntypes] != h.pairmat[b + a * h.ntypes])
+        return fail(NDG_ERR_ARG, "asymmetric pair matrix is not supported (compute1)");
+  for (int a = 0; a < 3; ++a)
+    if (!(h.box[a] > 0.0)) return fail(NDG_ERR_ARG, "box must be positive");
+  return NDG_OK;
+}
+
+static int fill_host(HostState &h, int np, int ntypes, int npairs, int ptsize, const double *P,
+                     const double *F, const double *rc2, const double *dr2, double ptrmin,
+                     const int *pairmat, const double *pairparams, const double *mass) {
+  const size_t n = (size_t)npairs * (ptsize + 1);
+  h.np = np;
+  h.ntypes = ntypes;
+  h.npairs = npairs;
+  h.ptsize = ptsize;
+  h.P.assign(P, P + n);
+  h.F.assign(F, F + n);
+  h.rc2.assign(rc2, rc2 + npairs);
+  h.dr2.assign(dr2, dr2 + npairs);
+  h.ptrmin = ptrmin;
+  h.pairmat.assign(pairmat, pairmat + ntypes * ntypes);
+  h.pairparams.assign(pairparams, pairparams + 4 * npairs);
+  h.mass.assign(mass, mass + ntypes);
+  h.have_tables = true;
+  return NDG_OK;
+}
+
+// Make slot r resident for a Run: allocate for the list capacity, upload, build lists.
+static int begin_run(ndg_engine *e, int r, HostState &h, const ndg_run_params &rp, int step0,
+                     int neimax_req, bool fresh_state) {
+  Slot &s = e->slots[r];
+  double rcut = 0.0;
+  for (int p = 0; p < h.npairs; ++p) rcut = std::max(rcut, h.pairparams[4 * p]);
+  const double skin = rp.skin < 0.0 ? rcut * 0.2 : rp.skin;
+  double box[3];
+  if (fresh_state) {
+    for (int a = 0; a < 3; ++a) box[a] = h.box[a];
+  } else {
+    RepState S;
+    int rc = get_state_dev(e, r, &S);
+    if (rc) return rc;
+    for (int a = 0; a < 3; ++a) box[a] = S.box[a];
+  }
+  int neimax = neimax_req > 0 ? neimax_req : estimate_neimax(rcut, skin, h.np, box);
+  if (neimax < 1) neimax = 1;
+  const int tpa = e->tpa_override ? e->tpa_override : default_tpa(h.np);
+  const int ncap = ncell_capacity(box, rcut, skin);
+  const bool need_alloc = !s.block || neimax > s.neimax || ncap > s.ncell_cap || tpa != s.tpa;
+  int rc;
+  if (need_alloc) {
+    // keep the state across the reallocation when it is already on the device
+    std::vector<double> pos, vel, acc;
+    if (!fresh_state && s.block) {
+      pos.resize(3 * (size_t)s.np);
+      vel.resize(3 * (size_t)s.np);
+      acc.resize(3 * (size_t)s.np);
+      if ((rc = download_state(e, s, pos.data(), vel.data(), acc.data(), nullptr, nullptr, box))) return rc;
+      h.pos = pos;
+      h.vel = vel;
+      h.acc = acc;
+      for (int a = 0; a < 3; ++a) h.box[a] = box[a];
+      fresh_state = true;
+    }
+    s.np = h.np;
+    s.ntypes = h.ntypes;
+    s.npairs = h.npairs;
+    s.ptsize = h.ptsize;
+    if ((rc = alloc_slot(e, r, neimax, ncap, tpa))) return rc;
+    if ((rc = ensure_capacity(e))) return rc;
+  }
+  if (!s.P.tabF2) {
+    if ((rc = attach_tables(e, s, h))) return rc;
+  }
+  s.P.skin = skin;
+  set_run_params(s, rp, step0);
+  if ((rc = upload_host(e, s, h, fresh_state))) return rc;
+  if ((rc = upload_params(e, r))) return rc;
+  RepState S;
+  if (fresh_state) {
+    memset(&S, 0, sizeof(S));
+    for (int a = 0; a < 3; ++a) S.box[a] = S.box_build[a] = box[a];
+    S.lam1 = S.lam2 = 1.0;
+  } else if ((rc = get_state_dev(e, r, &S))) {
+    return rc;
+  }
+  S.step = step0;
+  S.done = rp.nsteps <= 0 ? 1 : 0;
+  S.err = 0;
+  S.nupdates = 0;
+  S.nrec = 0;
+  S.e0 = 0.0;
+  s.nrec_seen = 0;
+  s.pending_rec.clear();
+  s.err = 0;
+  if ((rc = put_state_dev(e, r, &S))) return rc;
+  return build_slot_now(e, r, skin);
+}
+
+extern "C" int ndg_engine_admit(ndg_engine_t *e, int *slot, int np, int ntypes, int npairs,
+                                int ptsize, int neimax, const double *P, const double *F,
+                                const double *rc2, const double *dr2, double ptrmin,
+                                const int *pairmat, const double *pairparams,
+                                const double *mass, const double *pos, const double *vel,
+                                const int *typ, const double box[3],
+                                const ndg_run_params *rp) {
+  if (!e || !slot || !rp || !pos || !typ || !box) return fail(NDG_ERR_ARG, "ndg_engine_admit: null argument");
+  HostState h;
+  fill_host(h, np, ntypes, npairs, ptsize, P, F, rc2, dr2, ptrmin, pairmat, pairparams, mass);
+  h.pos.assign(pos, pos + 3 * (size_t)np);
+  if (vel) h.vel.assign(vel, vel + 3 * (size_t)np);
+  h.typ.assign(typ, typ + np);
+  for (int a = 0; a < 3; ++a) h.box[a] = box[a];
+  int rc = check_host(h);
+  if (rc) return rc;
+  if (e->path == NDG_PATH_FP64 && e->tpa_override > 1) e->tpa_override = 1;
+  const int r = new_slot(e);
+  rc = begin_run(e, r, h, *rp, 0, neimax, true);
+  if (rc) {
+    Slot &s = e->slots[r];
+    if (s.block) dev_release(e, s.block, s.bytes);
+    if (s.P.tabF2) detach_tables(e, s);
+    s = Slot();
+    e->tiles_dirty = true;
+    return rc;
+  }
+  *slot = r;
+  return NDG_OK;
+}
+
+extern "C" int ndg_engine_step(ndg_engine_t *e, int nsteps, int *ndone) {
+  if (!e) return fail(NDG_ERR_ARG, "null engine");
+  int rc = run_engine(e, nsteps);
+  if (ndone) {
+    int d = 0;
+    for (auto &s : e->slots)
+      if (s.used && (s.finished || s.err)) ++d;
+    *ndone = d;
+  }
+  return rc;
+}
+
+extern "C" int ndg_engine_status(ndg_engine_t *e, int slot, int *steps_done, int *finished,
+                                 ndg_thermo *last, int *err) {
+  if (!e || slot < 0 || slot >= (int)e->slots.size() || !e->slots[slot].used)
+    return fail(NDG_ERR_ARG, "bad slot");
+  Slot &s = e->slots[slot];
+  if (steps_done) *steps_done = s.step - s.run_step0;
+  if (finished) *finished = s.finished ? 1 : 0;
+  if (last) *last = s.last;
+  if (err) *err = s.err;
+  return NDG_OK;
+}
+
+extern "C" int ndg_engine_thermo(ndg_engine_t *e, int slot, ndg_thermo *rec, int max_rec,
+                                 int *nrec) {
+  if (!e || slot < 0 || slot >= (int)e->slots.size() || !e->slots[slot].used)
+    return fail(NDG_ERR_ARG, "bad slot");
+  Slot &s = e->slots[slot];
+  const int n = std::min((int)s.pending_rec.size(), max_rec);
+  for (int k = 0; k < n; ++k) rec[k] = s.pending_rec[k];
+  s.pending_rec.erase(s.pending_rec.begin(), s.pending_rec.begin() + n);
+  if (nrec) *nrec = n;
+  return NDG_OK;
+}
+
+extern "C" int ndg_engine_fetch(ndg_engine_t *e, int slot, double *pos, double *vel,
+                                double *force, double box[3]) {
+  if (!e || slot < 0 || slot >= (int)e->slots.size() || !e->slots[slot].used)
+    return fail(NDG_ERR_ARG, "bad slot");
+  EngineDev E = dev_view(e);
+  launch_apply_pending(E, slot, 1, e->stream);
+  HIPCHK(hipGetLastError());
+  return download_state(e, e->slots[slot], pos, vel, nullptr, force, nullptr, box);
+}
+
+extern "C" int ndg_engine_release(ndg_engine_t *e, int slot) {
+  if (!e || slot < 0 || slot >= (int)e->slots.size() || !e->slots[slot].used)
+    return fail(NDG_ERR_ARG, "bad slot");
+  HIPCHK(hipStreamSynchronize(e->stream));
+  Slot &s = e->slots[slot];
+  dev_release(e, s.block, s.bytes);
+  detach_tables(e, s);
+  s = Slot();
+  RepState z;
+  memset(&z, 0, sizeof(z));
+  z.done = 1;
+  int rc = put_state_dev(e, slot, &z);
+  if (rc) return rc;
+  // an unused slot keeps a zero-size parameter record
+  RepParams zp;
+  memset(&zp, 0, sizeof(zp));
+  HIPCHK(hipMemcpyAsync(e->d_reps + slot, &zp, sizeof(zp), hipMemcpyHostToDevice, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  e->tiles_dirty = true;
+  return NDG_OK;
+}
+
+extern "C" int ndg_engine_config(ndg_engine_t *e, int tpa, int graph_steps) {
+  if (!e || (tpa != 0 && tpa != 1 && tpa != 2 && tpa != 4 && tpa != 8))
+    return fail(NDG_ERR_ARG, "tpa must be 0,1,2,4,8");
+  e->tpa_override = e->path == NDG_PATH_FP64 ? 1 : tpa;
+  e->graph_steps = graph_steps < 0 ? 0 : graph_steps;
+  return NDG_OK;
+}
+
+extern "C" int ndg_engine_timing(ndg_engine_t *e, int enable) {
+  if (!e) return fail(NDG_ERR_ARG, "null engine");
+  e->timing = enable != 0;
+  return NDG_OK;
+}
+
+extern "C" int ndg_engine_kernel_time(ndg_engine_t *e, int reset, double *avg_ms,
+                                      long long *count, double *pairs, double *atoms) {
+  if (!e) return fail(NDG_ERR_ARG, "null engine");
+  HIPCHK(hipStreamSynchronize(e->stream));
+  time_collect(e);
+  if (avg_ms) *avg_ms = e->t_count ? e->t_ms / e->t_count : 0.0;
+  if (count) *count = e->t_count;
+  long long atoms_total = 0;
+  double pairs_total = 0.0;
+  for (auto &s : e->slots)
+    if (s.used) {
+      atoms_total += s.np;
+      pairs_total += (double)s.nbr_sum;
+    }
+  if (pairs) *pairs = pairs_total;
+  if (atoms) *atoms = (double)atoms_total;
+  if (reset) {
+    e->t_ms = 0.0;
+    e->t_count = 0;
+  }
+  return NDG_OK;
+}
+
+extern "C" int ndg_engine_list_stats(ndg_engine_t *e, double *mean_nbr, long long *atoms) {
+  if (!e) return fail(NDG_ERR_ARG, "null engine");
+  long long a = 0;
+  double p = 0;
+  for (auto &s : e->slots)
+    if (s.used) {
+      a += s.np;
+      p += (double)s.nbr_sum;
+    }
+  if (mean_nbr) *mean_nbr = a ? p / a : 0.0;
+  if (atoms) *atoms = a;
+  return NDG_OK;
+}
+
+// ------------------------------------------------------------------ single-system API
+
+extern "C" const char *ndg_last_error(void) { return g_err.c_str(); }
+
+extern "C" int ndg_create(ndg_t **h, int np, int ntypes, int npairs, int ptsize, int neimax,
+                          int flags) {
+  if (!h || np <= 0 || ntypes <= 0 || npairs <= 0 || ptsize < 2)
+    return fail(NDG_ERR_ARG, "ndg_create: bad arguments");
+  const int path = flags & 1;
+  auto sys = std::make_unique<ndg_system>();
+  int rc = engine_new(&sys->e, path, 0);
+  if (rc) return rc;
+  sys->path = path;
+  sys->hs.np = np;
+  sys->hs.ntypes = ntypes;
+  sys->hs.npairs = npairs;
+  sys->hs.ptsize = ptsize;
+  sys->neimax_req = neimax;
+  sys->slot = new_slot(sys->e);
+  Slot &s = sys->e->slots[sys->slot];
+  s.np = np;
+  s.ntypes = ntypes;
+  s.npairs = npairs;
+  s.ptsize = ptsize;
+  *h = sys.release();
+  return NDG_OK;
+}
+
+extern "C" int ndg_destroy(ndg_t *h) {
+  if (!h) return NDG_OK;
+  ndg_engine_destroy(h->e);
+  delete h;
+  return NDG_OK;
+}
+
+extern "C" int ndg_set_tables(ndg_t *h, const double *P, const double *F, const double *rc2,
+                              const double *dr2, double ptrmin, const int *pairmat,
+                              const double *pairparams, const double *mass) {
+  if (!h || !P || !F || !rc2 || !dr2 || !pairmat || !pairparams || !mass)
+    return fail(NDG_ERR_ARG, "ndg_set_tables: null argument");
+  HostState &hs = h->hs;
+  fill_host(hs, hs.np, hs.ntypes, hs.npairs, hs.ptsize, P, F, rc2, dr2, ptrmin, pairmat,
+            pairparams, mass);
+  Slot &s = h->e->slots[h->slot];
+  if (s.P.tabF2) {  // re-attach new tables
+    detach_tables(h->e, s);
+    s.P.tabF2 = nullptr;
+    int rc = attach_tables(h->e, s, hs);
+    if (rc) return rc;
+    h->e->tiles_dirty = true;  // the LDS table set changes
+    if (s.block) {
+      if ((rc = upload_host(h->e, s, hs, false))) return rc;
+      return upload_params(h->e, h->slot);
+    }
+  }
+  return NDG_OK;
+}
+
+extern "C" int ndg_set_state(ndg_t *h, const double *pos, const double *vel, const double *acc,
+                             const int *typ, const double box[3]) {
+  if (!h || !pos || !typ || !box) return fail(NDG_ERR_ARG, "ndg_set_state: null argument");
+  HostState &hs = h->hs;
+  const size_t n3 = 3 * (size_t)hs.np;
+  hs.pos.assign(pos, pos + n3);
+  if (vel) hs.vel.assign(vel, vel + n3); else hs.vel.assign(n3, 0.0);
+  if (acc) hs.acc.assign(acc, acc + n3); else hs.acc.clear();
+  hs.typ.assign(typ, typ + hs.np);
+  for (int a = 0; a < 3; ++a) hs.box[a] = box[a];
+  hs.have_state = true;
+  Slot &s = h->e->slots[h->slot];
+  if (s.block) {
+    // state replaced while resident: upload and invalidate the lists
+    int rc = upload_host(h->e, s, hs, true);
+    if (rc) return rc;
+    RepState S;
+    if ((rc = get_state_dev(h->e, h->slot, &S))) return rc;
+    for (int a = 0; a < 3; ++a) S.box[a] = S.box_build[a] = box[a];
+    S.lam1 = S.lam2 = 1.0;
+    S.parity = 0;
+    if ((rc = put_state_dev(h->e, h->slot, &S))) return rc;
+    return build_slot_now(h->e, h->slot, s.P.skin);
+  }
+  return NDG_OK;
+}
+
+static int ensure_resident(ndg_t *h, double skin) {
+  Slot &s = h->e->slots[h->slot];
+  if (!h->hs.have_tables || !h->hs.have_state)
+    return fail(NDG_ERR_ARG, "tables and state must be set first");
+  int rc = check_host(h->hs);
+  if (rc) return rc;
+  if (s.block) return NDG_OK;
+  ndg_run_params rp;
+  memset(&rp, 0, sizeof(rp));
+  rp.skin = skin;
+  rp.out_steps = 1;
+  rp.dt = 0.0;
+  return begin_run(h->e, h->slot, h->hs, rp, 0, h->neimax_req, true);
+}
+
+extern "C" int ndg_build_lists(ndg_t *h, double skin) {
+  if (!h) return fail(NDG_ERR_ARG, "null handle");
+  Slot &s = h->e->slots[h->slot];
+  if (!s.block) return ensure_resident(h, skin);
+  double rcut = s.P.rcut;
+  if (skin < 0.0) skin = 0.2 * rcut;
+  return build_slot_now(h->e, h->slot, skin);
+}
+
+extern "C" int ndg_get_state(ndg_t *h, double *pos, double *vel, double *acc, double *force,
+                             double *pene, double box[3]) {
+  if (!h) return fail(NDG_ERR_ARG, "null handle");
+  Slot &s = h->e->slots[h->slot];
+  if (!s.block) {
+    const HostState &hs = h->hs;
+    if (pos) std::copy(hs.pos.begin(), hs.pos.end(), pos);
+    if (vel) std::copy(hs.vel.begin(), hs.vel.end(), vel);
+    if (box)
+      for (int a = 0; a < 3; ++a) box[a] = hs.box[a];
+    return NDG_OK;
+  }
+  return download_state(h->e, s, pos, vel, acc, force, pene, box);
+}
+
+extern "C" int ndg_set_box(ndg_t *h, const double box[3]) {
+  if (!h || !box) return fail(NDG_ERR_ARG, "null argument");
+  Slot &s = h->e->slots[h->slot];
+  if (!s.block) {
+    for (int a = 0; a < 3; ++a) h->hs.box[a] = box[a];
+    return NDG_OK;
+  }
+  RepState S;
+  int rc = get_state_dev(h->e, h->slot, &S);
+  if (rc) return rc;
+  for (int a = 0; a < 3; ++a) S.box[a] = box[a];
+  return put_state_dev(h->e, h->slot, &S);
+}
+
+extern "C" int ndg_get_box(ndg_t *h, double box[3]) {
+  if (!h || !box) return fail(NDG_ERR_ARG, "null argument");
+  Slot &s = h->e->slots[h->slot];
+  if (!s.block) {
+    for (int a = 0; a < 3; ++a) box[a] = h->hs.box[a];
+    return NDG_OK;
+  }
+  RepState S;
+  int rc = get_state_dev(h->e, h->slot, &S);
+  if (rc) return rc;
+  for (int a = 0; a < 3; ++a) box[a] = S.box[a];
+  return NDG_OK;
+}
+
+extern "C" int ndg_get_neighbours(ndg_t *h, int *counts, int *nbr, int cap) {
+  if (!h || !counts) return fail(NDG_ERR_ARG, "null argument");
+  ndg_engine *e = h->e;
+  Slot &s = e->slots[h->slot];
+  if (!s.block) return fail(NDG_ERR_ARG, "no lists built");
+  const int np = s.np;
+  std::vector<int> oid(np), nnb(np);
+  const size_t nlist = (size_t)s.ntiles * s.P.list_chunks * kTile * s.tpa;
+  std::vector<unsigned> list(nlist);
+  HIPCHK(hipMemcpyAsync(oid.data(), s.P.oid, 4 * np, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipMemcpyAsync(nnb.data(), s.P.nnb, 4 * np, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipMemcpyAsync(list.data(), s.P.list, 4 * nlist, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  const int tpa = s.tpa, C = s.P.list_chunks;
+  for (int i = 0; i < np; ++i) {
+    const int o = oid[i];
+    counts[o] = nnb[i];
+    if (!nbr) continue;
+    const int lt = i / kTile, a = i % kTile;
+    std::vector<int> row;
+    for (int k = 0; k < nnb[i]; ++k) {
+      const size_t idx = ((size_t)lt * C + k / tpa) * (kTile * tpa) + a * tpa + k % tpa;
+      row.push_back(oid[list[idx] & kIndexMask]);
+    }
+    std::sort(row.begin(), row.end());
+    for (int k = 0; k < (int)row.size() && k < cap; ++k) nbr[(size_t)o * cap + k] = row[k];
+  }
+  return NDG_OK;
+}
+
+static int drop_in_flags(int a, int b, int c) { return F_ACTIVE | a | b | c; }
+
+extern "C" int ndg_integrate1(ndg_t *h, double dt) {
+  if (!h) return fail(NDG_ERR_ARG, "null handle");
+  ndg_engine *e = h->e;
+  Slot &s = e->slots[h->slot];
+  if (!s.block) return fail(NDG_ERR_ARG, "lists not built");
+  s.P.dt = dt;
+  int rc = upload_params(e, h->slot);
+  if (rc) return rc;
+  EngineDev E = dev_view(e);
+  launch_drift(E, e->path, s.tile_base, s.ntiles, drop_in_flags(0, 0, 0), e->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return NDG_OK;
+}
+
+extern "C" int ndg_compute(ndg_t *h, int cv, int cu, double *pe, double virial[6]) {
+  if (!h) return fail(NDG_ERR_ARG, "null handle");
+  ndg_engine *e = h->e;
+  Slot &s = e->slots[h->slot];
+  if (!s.block) return fail(NDG_ERR_ARG, "lists not built");
+  const int xf = drop_in_flags(cv ? F_CV : 0, cu ? F_CU : 0, 0);
+  EngineDev E = dev_view(e);
+  hipEvent_t a = nullptr;
+  time_begin(e, &a);
+  launch_force(E, e->path, s.tpa, (cv || cu), s.tile_base, s.ntiles, xf, e->stream);
+  time_end(e, a);
+  launch_reduce(E, h->slot, 1, RED_FORCE, xf, 0, e->stream);
+  HIPCHK(hipGetLastError());
+  RepState S;
+  int rc = get_state_dev(e, h->slot, &S);
+  if (rc) return rc;
+  // the next integrate2 reads the new force: parity of "latest" flips after the kick
+  if (pe && cu) *pe = S.pe;
+  if (virial && cv)
+    for (int q = 0; q < 6; ++q) virial[q] = S.vir[q];
+  return NDG_OK;
+}
+
+extern "C" int ndg_integrate2(ndg_t *h, double dt, int ck, double *ke) {
+  if (!h) return fail(NDG_ERR_ARG, "null handle");
+  ndg_engine *e = h->e;
+  Slot &s = e->slots[h->slot];
+  if (!s.block) return fail(NDG_ERR_ARG, "lists not built");
+  s.P.dt = dt;
+  int rc = upload_params(e, h->slot);
+  if (rc) return rc;
+  const int xf = drop_in_flags(ck ? F_CK : 0, 0, 0);
+  EngineDev E = dev_view(e);
+  launch_kick(E, e->path, s.tile_base, s.ntiles, xf, e->stream);
+  launch_reduce(E, h->slot, 1, RED_KICK | RED_FLIP, xf, 0, e->stream);
+  HIPCHK(hipGetLastError());
+  RepState S;
+  if ((rc = get_state_dev(e, h->slot, &S))) return rc;
+  if (ke) *ke = S.ke;
+  return NDG_OK;
+}
+
+extern "C" int ndg_scale_velocities(ndg_t *h, double lambda) {
+  if (!h) return fail(NDG_ERR_ARG, "null handle");
+  ndg_engine *e = h->e;
+  Slot &s = e->slots[h->slot];
+  if (!s.block) return fail(NDG_ERR_ARG, "lists not built");
+  RepState S;
+  int rc = get_state_dev(e, h->slot, &S);
+  if (rc) return rc;
+  S.lam1 = lambda;
+  S.lam2 = 1.0;
+  if ((rc = put_state_dev(e, h->slot, &S))) return rc;
+  launch_apply_pending(dev_view(e), h->slot, 1, e->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return NDG_OK;
+}
+
+extern "C" int ndg_check_rebuild(ndg_t *h, double skin, int trigger, int *rebuilt) {
+  if (!h) return fail(NDG_ERR_ARG, "null handle");
+  ndg_engine *e = h->e;
+  Slot &s = e->slots[h->slot];
+  if (!s.block) return fail(NDG_ERR_ARG, "lists not built");
+  if (skin < 0.0) skin = 0.2 * s.P.rcut;
+  s.P.skin = skin;
+  int rc = upload_params(e, h->slot);
+  if (rc) return rc;
+  launch_reduce(dev_view(e), h->slot, 1, RED_CHECK, F_ACTIVE, trigger, e->stream);
+  HIPCHK(hipGetLastError());
+  RepState S;
+  if ((rc = get_state_dev(e, h->slot, &S))) return rc;
+  if (S.err) {
+    const int err = S.err;
+    S.err = 0;
+    put_state_dev(e, h->slot, &S);
+    return fail(err, err_text(err));
+  }
+  if (rebuilt) *rebuilt = S.rebuild;
+  if (S.rebuild) {
+    if (trigger == NDG_TRIGGER_COMPAT) {
+      // the reference keeps MCELL for the whole Run (src/nd.f90:80-83)
+      return build_slot_now(e, h->slot, skin);
+    }
+    return build_slot_now(e, h->slot, skin);
+  }
+  return NDG_OK;
+}
+
+extern "C" int ndg_run_begin(ndg_t *h, const ndg_run_params *rp, int step0) {
+  if (!h || !rp) return fail(NDG_ERR_ARG, "null argument");
+  if (!h->hs.have_tables || !h->hs.have_state)
+    return fail(NDG_ERR_ARG, "tables and state must be set first");
+  int rc = check_host(h->hs);
+  if (rc) return rc;
+  Slot &s = h->e->slots[h->slot];
+  return begin_run(h->e, h->slot, h->hs, *rp, step0, h->neimax_req, s.block == nullptr);
+}
+
+extern "C" int ndg_run_steps(ndg_t *h, int nsteps, ndg_thermo *rec, int max_rec, int *nrec) {
+  if (!h) return fail(NDG_ERR_ARG, "null handle");
+  ndg_engine *e = h->e;
+  int rc = run_engine(e, nsteps);
+  Slot &s = e->slots[h->slot];
+  int n = 0;
+  if (rec) {
+    n = std::min((int)s.pending_rec.size(), max_rec);
+    for (int k = 0; k < n; ++k) rec[k] = s.pending_rec[k];
+    s.pending_rec.erase(s.pending_rec.begin(), s.pending_rec.begin() + n);
+  } else {
+    s.pending_rec.clear();
+  }
+  if (nrec) *nrec = n;
+  return rc;
+}
+
+extern "C" int ndg_run_end(ndg_t *h, int *nupdates, int *step) {
+  if (!h) return fail(NDG_ERR_ARG, "null handle");
+  ndg_engine *e = h->e;
+  launch_apply_pending(dev_view(e), h->slot, 1, e->stream);
+  HIPCHK(hipGetLastError());
+  RepState S;
+  int rc = get_state_dev(e, h->slot, &S);
+  if (rc) return rc;
+  if (nupdates) *nupdates = S.nupdates;
+  if (step) *step = S.step;
+  return NDG_OK;
+}

@@ -1,0 +1,128 @@
+// ndg_internal.h -- device-side data layout of the MD-step engine.
+//
+// One engine holds many independent periodic systems ("replicas"). Each replica owns
+// one device block (carved into the arrays below) so admission and release are a
+// single allocation each; kernels reach a replica's arrays through RepParams.
+// Atoms are stored sorted by cell (re-sorted at every list build) in SoA FP64 for the
+// integrator state and double4 AoS (cell-anchored, A) for the pair kernel's gathers.  The caller's atom
+// order is kept in `oid` and only used at the host boundary.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/ndyn_gpu.h"
+
+namespace ndg {
+
+constexpr int kTile = 128;        // atoms per tile (unit of work of every per-atom kernel)
+constexpr int kCodeShift = 26;    // list entry = j | code << 26, code = image/cell offset
+constexpr unsigned kIndexMask = (1u << kCodeShift) - 1u;
+constexpr int kRing = 1024;       // thermo records kept per replica between host drains
+constexpr int kPartials = 12;     // doubles per tile: ke, pe, vir[6], d1, d2, spare[2]
+
+// step-flag bits (src/nd.f90:138-148 and the ensemble frequencies)
+enum : int { F_OUT = 1, F_CV = 2, F_CU = 4, F_CK = 8, F_CHECK = 16, F_ACTIVE = 32 };
+
+struct Tile {
+  int rep;     // slot index
+  int atom0;   // first local atom of the tile
+  int ltile;   // local tile index within the replica
+  int pad;
+};
+
+// Constant per Run (uploaded on admission / run begin); pointers into the replica block.
+struct RepParams {
+  // sizes
+  int np, ntiles, tpa, neimax;
+  int list_chunks;          // chunks per tile = ceil(neimax / tpa)
+  int ncell_cap;
+  int ntypes, npairs, ptsize;
+  int path;                 // NDG_PATH_*
+  // run control
+  int run_step0, run_nsteps, out_steps;
+  int renei, renei_freq, trigger;
+  int isokinetic, ke_freq, isobaric, p_freq, isoenergetic, e_freq, box_load;
+  int pad0;
+  double dt, skin, rcut, ptrmin;
+  double ke_target, ke_tau, p_target[3], p_tau, e_tau, box_vel[3];
+  // per-type / per-pair data
+  const double *mass;       // [ntypes]
+  const int *pairmat;       // [ntypes*ntypes] column-major, 1-based ids
+  const double *rc_pair;    // [npairs] cutoff radius (PairParams(1,:))
+  const double *rc2d;       // [npairs] PTrc2
+  const float *rc2f;        // [npairs]
+  const float *vdr2;        // [npairs] real(1/PTdr2) (src/nd.f90:254)
+  const double *tabF64;     // [npairs*(ptsize+1)]
+  const double *tabP64;
+  const float2 *tabF2;      // [npairs*(ptsize+1)] {F_k, F_k+1 - F_k} (mixed path)
+  const float2 *tabP2;      // same for the potential
+  int ltab[16];             // offset of each pair's FP32 F table in the engine's LDS set, -1: none
+  // per-atom state (sorted order)
+  double *px, *py, *pz, *vx, *vy, *vz;
+  double *ax, *ay, *az;     // FP64 path acceleration
+  double *fx, *fy, *fz, *pene;  // FP64 path force / per-atom energy
+  double *pbx, *pby, *pbz;  // FP64 path positions at the last build
+  float4 *f32a, *f32b;      // mixed path force (xyz) + per-atom energy (w), two parities
+  double4 *x64a;            // mixed path cell-anchored coordinates (xyz, A) + type bits (w)
+  float4 *x32b;             // FP32 copy of x64a at the last build (displacement check)
+  int4 *anchor;             // integer anchor per axis (image*M + cell) and linear cell (w)
+  int *typ, *oid, *nnb;
+  unsigned *list;
+  // rebuild scratch
+  int *cell_count, *cell_start, *cell_fill;  // cell_start has ncell_cap+1 entries
+  int *key, *slot_src;
+  double *tmpd;             // 9 * np
+  float4 *tmpf;             // 2 * np (both force buffers)
+  int *tmpi;                // 2 * np
+  double *partials;         // ntiles * kPartials
+  ndg_thermo *ring;         // kRing
+};
+
+// Mutable per-replica state, device resident.
+struct RepState {
+  int step;          // last completed step
+  int mcell[3];      // cell / anchor grid (all 1 in brute mode)
+  int brute;         // O(N^2) candidate walk (some mcell < 3)
+  int rebuild;       // 1: rebuild before the next step
+  int err;           // NDG_ERR_* raised by a kernel
+  int nupdates;      // list rebuilds this Run
+  int nrec;          // thermo records written (ring index = nrec % kRing)
+  int parity;        // F buffer holding the forces of the last completed step (mixed path)
+  int fnew;          // 1: a force evaluation wrote buffer parity^1 and no kick has consumed it
+  int done;          // Run finished
+  long long nbr_sum; // sum of list lengths at the last build
+  double box[3];
+  double box_build[3];
+  double e0, pe, ke, vir[6];
+  double lam1, lam2; // pending velocity scales (isokinetic, isoenergetic)
+};
+
+struct EngineDev {
+  RepParams *reps;
+  RepState *st;
+  const Tile *tiles;
+  int ntiles;
+  int nreps;
+  const float *tabset;  // distinct FP32 F tables of all replicas, staged in LDS by the force kernel
+  int tabset_n;         // floats in tabset (0: tables are read from global memory)
+};
+
+constexpr int kLdsTableBytes = 64 * 1024;   // LDS budget of the staged table set
+
+// Flags of the step a replica is about to compute (src/nd.f90:137-151).
+__host__ __device__ inline int step_flags(const RepParams &R, const RepState &S) {
+  if (S.done || S.err) return 0;
+  int s = S.step + 1;
+  int tt = s - R.run_step0;
+  if (tt > R.run_nsteps || tt < 1) return 0;
+  int f = F_ACTIVE;
+  if (s % R.out_steps == 0 || tt == 1) f |= F_OUT | F_CV | F_CU | F_CK;
+  if (R.isokinetic && s % R.ke_freq == 0) f |= F_CK;
+  if (R.isobaric && s % R.p_freq == 0) f |= F_CV;
+  if (R.isoenergetic && s % R.e_freq == 0) f |= F_CU | F_CK;
+  if (R.renei && s % R.renei_freq == 0) f |= F_CHECK;
+  return f;
+}
+
+}  // namespace ndg

@@ -1,0 +1,1256 @@
+// ndg_kernels.hip -- CDNA4 (gfx950) kernels of the MD step.
+//
+// Every per-atom kernel walks 128-atom tiles of one or many replicas with a persistent
+// grid; workgroup b runs on XCD (b % 8) and takes tiles from that XCD's contiguous
+// share, so neighbouring tiles (spatially neighbouring atoms after the cell sort) share
+// an L2.  All reductions use a fixed tree, so results do not depend on how replicas are
+// batched.  Built with -ffp-contract=off: FP64 expressions round in written order (the
+// reference's order); FP32 hot loops use explicit fmaf.
+#include "ndg_internal.h"
+
+#include <math.h>
+
+namespace ndg {
+
+// ------------------------------------------------------------------ helpers
+
+__device__ inline void tile_range(int t0, int nt, int &begin, int &end, int &step) {
+  const int nb = gridDim.x, b = blockIdx.x;
+  if (nb < 8) {
+    begin = t0 + b;
+    end = t0 + nt;
+    step = nb;
+    return;
+  }
+  const int x = b & 7;
+  const int nwx = (nb - x + 7) >> 3;
+  const long long lo = (long long)nt * x / 8, hi = (long long)nt * (x + 1) / 8;
+  begin = t0 + (int)lo + (b >> 3);
+  end = t0 + (int)hi;
+  step = nwx;
+}
+
+__device__ inline double shfl_xor_d(double v, int m) {
+  int2 a = *reinterpret_cast<int2 *>(&v);
+  a.x = __shfl_xor(a.x, m);
+  a.y = __shfl_xor(a.y, m);
+  return *reinterpret_cast<double *>(&a);
+}
+
+// Deterministic block sum of NV doubles per thread; result valid in thread 0.
+template <int NV>
+__device__ inline void block_sum(double (&v)[NV], double *sh) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] += shfl_xor_d(v[q], m);
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) sh[wid * NV + q] = v[q];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < nw; ++w)
+#pragma unroll
+      for (int q = 0; q < NV; ++q) v[q] += sh[w * NV + q];
+  }
+  __syncthreads();
+}
+
+__device__ inline void top2_merge(double &a1, double &a2, double b1, double b2) {
+  double hi = fmax(a1, b1), lo = fmin(a1, b1);
+  a2 = fmax(lo, fmax(a2, b2));
+  a1 = hi;
+}
+
+// Block top-2 (largest, second largest); valid in thread 0.
+__device__ inline void block_top2(double &t1, double &t2, double *sh) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    double b1 = shfl_xor_d(t1, m), b2 = shfl_xor_d(t2, m);
+    top2_merge(t1, t2, b1, b2);
+  }
+  __syncthreads();
+  if (lane == 0) {
+    sh[2 * wid] = t1;
+    sh[2 * wid + 1] = t2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int w = 1; w < nw; ++w) top2_merge(t1, t2, sh[2 * w], sh[2 * w + 1]);
+  __syncthreads();
+}
+
+__device__ inline int active_flags(const EngineDev &E, int rep, int xflags) {
+  if (xflags) return xflags;
+  return step_flags(E.reps[rep], E.st[rep]);
+}
+
+// Cell-anchored coordinate (A) of a scaled position: ((p+1/2) M - c - 1/2) w, |x| <~ w/2.
+// Pair vectors are x_i - x_j + o w with a per-pair integer cell/image offset o.
+__device__ inline double anchored(double p, int c, int m, double w) {
+  return ((p + 0.5) * (double)m - (double)c - 0.5) * w;
+}
+
+__device__ inline double type_bits(int t) { return __longlong_as_double((long long)t); }
+__device__ inline int bits_type(double w) { return (int)__double_as_longlong(w); }
+
+// Reference binning (src/nd.f90:1240-1246): sr = p - anint(p); I = int((sr+0.4999) M).
+__device__ inline void bin_axis(double p, int m, int brute, int &k, int &c) {
+  double n = round(p);
+  if (brute) {
+    k = 0;
+    c = (int)n;
+    return;
+  }
+  double sr = p - n;
+  int kk = (int)((sr + 0.4999) * (double)m);
+  kk = kk < 0 ? 0 : (kk >= m ? m - 1 : kk);
+  k = kk;
+  c = (int)n * m + kk;
+}
+
+// ------------------------------------------------------------------ integrator (mixed)
+
+// integrate1 (src/nd.f90:1029-1046) with the pending thermostat scale of the previous
+// step folded in; writes the FP32 cell-anchored copy and the displacement top-2.
+__global__ __launch_bounds__(kTile) void k_drift_mixed(EngineDev E, int t0, int nt, int xflags) {
+  __shared__ double sh[8];
+  int tb, te, ts;
+  tile_range(t0, nt, tb, te, ts);
+  for (int t = tb; t < te; t += ts) {
+    const Tile T = E.tiles[t];
+    const RepParams &R = E.reps[T.rep];
+    const RepState &S = E.st[T.rep];
+    const int f = active_flags(E, T.rep, xflags);
+    if (!(f & F_ACTIVE)) continue;
+    const int i = T.atom0 + threadIdx.x;
+    double t1 = -1.0, t2 = -1.0;
+    if (i < R.np) {
+      const double dt = R.dt;
+      const double l1 = S.lam1, l2 = S.lam2;
+      double vx = R.vx[i], vy = R.vy[i], vz = R.vz[i];
+      if (l1 != 1.0 || l2 != 1.0) {
+        vx = vx * l1; vy = vy * l1; vz = vz * l1;
+        vx = vx * l2; vy = vy * l2; vz = vz * l2;
+        R.vx[i] = vx; R.vy[i] = vy; R.vz[i] = vz;
+      }
+      const float4 Fo = (S.parity ? R.f32b : R.f32a)[i];
+      const int ty = R.typ[i];
+      const double rm = 1.0 / R.mass[ty - 1];
+      const double ax = (double)Fo.x * rm, ay = (double)Fo.y * rm, az = (double)Fo.z * rm;
+      const double li0 = 1.0 / S.box[0], li1 = 1.0 / S.box[1], li2 = 1.0 / S.box[2];
+      double px = R.px[i] + li0 * (vx * dt + 0.5 * ax * dt * dt);
+      double py = R.py[i] + li1 * (vy * dt + 0.5 * ay * dt * dt);
+      double pz = R.pz[i] + li2 * (vz * dt + 0.5 * az * dt * dt);
+      R.px[i] = px; R.py[i] = py; R.pz[i] = pz;
+      const int4 A = R.anchor[i];
+      const int m0 = S.mcell[0], m1 = S.mcell[1], m2 = S.mcell[2];
+      const double xl = anchored(px, A.x, m0, S.box[0] / m0);
+      const double yl = anchored(py, A.y, m1, S.box[1] / m1);
+      const double zl = anchored(pz, A.z, m2, S.box[2] / m2);
+      R.x64a[i] = make_double4(xl, yl, zl, type_bits(ty));
+      const float4 B = R.x32b[i];
+      const double dx = (double)((float)xl - B.x), dy = (double)((float)yl - B.y),
+                   dz = (double)((float)zl - B.z);
+      if (R.trigger == NDG_TRIGGER_COMPAT) {
+        const double sx = dx * li0, sy = dy * li1, sz = dz * li2;
+        t1 = sx * sx + sy * sy + sz * sz;
+      } else {
+        t1 = dx * dx + dy * dy + dz * dz;
+      }
+    }
+    block_top2(t1, t2, sh);
+    if (threadIdx.x == 0) {
+      R.partials[T.ltile * kPartials + 8] = t1;
+      R.partials[T.ltile * kPartials + 9] = t2;
+    }
+  }
+}
+
+// integrate2 (src/nd.f90:1091-1117): kick with the new force, kinetic energy partial.
+__global__ __launch_bounds__(kTile) void k_kick_mixed(EngineDev E, int t0, int nt, int xflags) {
+  __shared__ double sh[2];
+  int tb, te, ts;
+  tile_range(t0, nt, tb, te, ts);
+  for (int t = tb; t < te; t += ts) {
+    const Tile T = E.tiles[t];
+    const RepParams &R = E.reps[T.rep];
+    const RepState &S = E.st[T.rep];
+    const int f = active_flags(E, T.rep, xflags);
+    if (!(f & F_ACTIVE)) continue;
+    const int i = T.atom0 + threadIdx.x;
+    double kt[1] = {0.0};
+    if (i < R.np) {
+      const float4 Fo = (S.parity ? R.f32b : R.f32a)[i];
+      const float4 Fn = (S.parity ? R.f32a : R.f32b)[i];
+      const int ty = R.typ[i];
+      const double m = R.mass[ty - 1];
+      const double rm = 1.0 / m;
+      const double hdt = 0.5 * R.dt;
+      double vx = R.vx[i] + hdt * ((double)Fn.x * rm + (double)Fo.x * rm);
+      double vy = R.vy[i] + hdt * ((double)Fn.y * rm + (double)Fo.y * rm);
+      double vz = R.vz[i] + hdt * ((double)Fn.z * rm + (double)Fo.z * rm);
+      R.vx[i] = vx; R.vy[i] = vy; R.vz[i] = vz;
+      if (f & F_CK) {
+        double k = vx * vx;
+        k = k + vy * vy;
+        k = k + vz * vz;
+        kt[0] = k * m;
+      }
+    }
+    if (f & F_CK) {
+      block_sum<1>(kt, sh);
+      if (threadIdx.x == 0) R.partials[T.ltile * kPartials + 0] = kt[0];
+    }
+  }
+}
+
+// ------------------------------------------------------------------ force (mixed)
+
+// compute0 (src/nd.f90:262-333) on a full list, mixed precision.  Pair geometry in FP64
+// on cell-anchored coordinates (no rounding of minimum images), so the cutoff test and
+// the single-precision table index rk are the reference's own; table values, the linear
+// interpolation and 1/d2 in FP32; per-atom force, energy and virial sums in FP64 (full
+// rate on CDNA).  The distinct FP32 force tables of all replicas are staged in LDS once
+// per persistent workgroup (two-dword LDS read per pair); otherwise {T_k, T_k+1 - T_k}
+// pairs are read from global memory.  Each lane walks U list entries per iteration with
+// every load issued first and the cutoff as a select.  A workgroup of 512 (1024 for
+// TPA 8) threads runs 512/(128 TPA) tiles at a time; TPA lanes share one atom and split
+// its list; lane sums combine in a fixed order.
+constexpr int kUnroll = 4;
+
+template <int TPA>
+struct ForceShape {
+  static constexpr int kSub = kTile * TPA;                 // threads per tile
+  static constexpr int kThreads = TPA >= 8 ? 1024 : 512;   // threads per workgroup
+  static constexpr int kNSub = kThreads / kSub;            // tiles per pass
+};
+
+template <int TPA, bool FULL, bool LDS_TAB>
+__device__ inline void force_atom(const RepParams &R, int i, int sub, const double4 *shift,
+                                  const double *rc2, const float *vdr2, const int *pm,
+                                  const int *lt, const float *stab, const unsigned *L,
+                                  double &fx, double &fy, double &fz, double (&acc)[7]) {
+  constexpr int U = kUnroll;
+  const double4 xi = R.x64a[i];
+  const int ti = bits_type(xi.w);
+  const int n = R.nnb[i];
+  const int nmine = n > sub ? (n - sub + TPA - 1) / TPA : 0;  // entries of this lane
+  const double4 *X = R.x64a;
+  const float2 *TF = R.tabF2;
+  const float2 *TP = R.tabP2;
+  const bool one = R.npairs == 1;
+  const int nt_ = R.ntypes;
+  const int tab = R.ptsize + 1;
+  const int kmax = R.ptsize;
+  const double ptrmin = R.ptrmin;
+  // absent entries read the atom itself with a zero offset; the select drops them
+  const unsigned pad = (unsigned)i | (21u << kCodeShift);
+  for (int c = 0; c < nmine; c += U) {
+    unsigned e[U];
+    double4 xj[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) e[u] = (c + u < nmine) ? L[(size_t)(c + u) * (kTile * TPA)] : pad;
+#pragma unroll
+    for (int u = 0; u < U; ++u) xj[u] = X[e[u] & kIndexMask];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const double4 sft = shift[e[u] >> kCodeShift];
+      const double dx = (xi.x - xj[u].x) + sft.x;
+      const double dy = (xi.y - xj[u].y) + sft.y;
+      const double dz = (xi.z - xj[u].z) + sft.z;
+      const double d2 = fma(dx, dx, fma(dy, dy, dz * dz));
+      int pid = 0;
+      if (!one) pid = pm[(ti - 1) + (bits_type(xj[u].w) - 1) * nt_] - 1;
+      const bool ok = (c + u < nmine) && !(d2 > rc2[pid]);
+      const float rk = (float)(d2 - ptrmin) * vdr2[pid] + 1.0f;
+      int k = (int)rk;
+      k = k < 1 ? 1 : (k > kmax ? kmax : k);
+      const float w = rk - (float)k;
+      float fv;
+      if (LDS_TAB) {
+        const float *Tl = stab + lt[pid];
+        const float F0 = Tl[k - 1], F1 = Tl[k];
+        fv = fmaf(w, F1 - F0, F0);
+      } else {
+        const float2 tf = TF[pid * tab + k - 1];
+        fv = fmaf(w, tf.y, tf.x);
+      }
+      const double df = ok ? (double)(fv * __builtin_amdgcn_rcpf((float)d2)) : 0.0;
+      fx = fma(dx, df, fx);
+      fy = fma(dy, df, fy);
+      fz = fma(dz, df, fz);
+      if (FULL) {
+        const float2 tp = TP[pid * tab + k - 1];
+        const double pv = ok ? (double)fmaf(w, tp.y, tp.x) : 0.0;
+        const double hdf = 0.5 * df;
+        acc[0] += 0.5 * pv;
+        acc[1] += dx * dx * hdf;
+        acc[2] += dy * dy * hdf;
+        acc[3] += dz * dz * hdf;
+        acc[4] += dx * dy * hdf;
+        acc[5] += dx * dz * hdf;
+        acc[6] += dy * dz * hdf;
+      }
+    }
+  }
+}
+
+template <int TPA, bool FULL>
+__global__ __launch_bounds__(ForceShape<TPA>::kThreads) void k_force_mixed(EngineDev E, int t0,
+                                                                           int nt, int xflags) {
+  using SH = ForceShape<TPA>;
+  constexpr int NSUB = SH::kNSub, SUB = SH::kSub, NW = SH::kThreads / 64;
+  extern __shared__ float sh_tab[];
+  __shared__ double4 sh_shift[NSUB][64];
+  __shared__ double sh_rc2[NSUB][16];
+  __shared__ float sh_vdr2[NSUB][16];
+  __shared__ int sh_pm[NSUB][64];
+  __shared__ int sh_lt[NSUB][16];
+  __shared__ int sh_uselds[NSUB];
+  __shared__ double sh_red[NW][8];
+  for (int q = threadIdx.x; q < E.tabset_n; q += blockDim.x) sh_tab[q] = E.tabset[q];
+  const int sg = threadIdx.x / SUB, lt = threadIdx.x % SUB;
+  const int a = lt / TPA, sub = lt % TPA;
+  const int nblk = (nt + NSUB - 1) / NSUB;
+  int bb, be, bs;
+  tile_range(0, nblk, bb, be, bs);
+  for (int blk = bb; blk < be; blk += bs) {
+    const int t = t0 + blk * NSUB + sg;
+    bool have = t < t0 + nt;
+    Tile T = have ? E.tiles[t] : Tile{0, 0, 0, 0};
+    const RepParams &R = E.reps[T.rep];
+    const RepState &S = E.st[T.rep];
+    if (have) {
+      const int f = active_flags(E, T.rep, xflags);
+      const bool full = (f & (F_CU | F_CV)) != 0;
+      have = (f & F_ACTIVE) && R.tpa == TPA && full == FULL;
+    }
+    __syncthreads();
+    if (have && lt < 64) {
+      const int c = lt;
+      sh_shift[sg][c] = make_double4((double)((c & 3) - 1) * (S.box[0] / S.mcell[0]),
+                                     (double)(((c >> 2) & 3) - 1) * (S.box[1] / S.mcell[1]),
+                                     (double)(((c >> 4) & 3) - 1) * (S.box[2] / S.mcell[2]), 0.0);
+      if (c < R.npairs) {
+        sh_rc2[sg][c] = R.rc2d[c];
+        sh_vdr2[sg][c] = R.vdr2[c];
+        sh_lt[sg][c] = R.ltab[c];
+      }
+      if (c < R.ntypes * R.ntypes) sh_pm[sg][c] = R.pairmat[c];
+      if (c == 0) {
+        int use = E.tabset_n > 0;
+        for (int p = 0; p < R.npairs; ++p) use = use && R.ltab[p] >= 0;
+        sh_uselds[sg] = use;
+      }
+    }
+    __syncthreads();
+    const int i = T.atom0 + a;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};  // e_i, vir xx yy zz xy xz yz (FULL)
+    if (have && i < R.np) {
+      const unsigned *L = R.list + ((size_t)T.ltile * R.list_chunks) * (kTile * TPA) + lt;
+      if (sh_uselds[sg])
+        force_atom<TPA, FULL, true>(R, i, sub, sh_shift[sg], sh_rc2[sg], sh_vdr2[sg], sh_pm[sg],
+                                    sh_lt[sg], sh_tab, L, fx, fy, fz, acc);
+      else
+        force_atom<TPA, FULL, false>(R, i, sub, sh_shift[sg], sh_rc2[sg], sh_vdr2[sg], sh_pm[sg],
+                                     sh_lt[sg], sh_tab, L, fx, fy, fz, acc);
+    }
+    if (TPA > 1) {
+#pragma unroll
+      for (int m = 1; m < TPA; m <<= 1) {
+        fx += shfl_xor_d(fx, m);
+        fy += shfl_xor_d(fy, m);
+        fz += shfl_xor_d(fz, m);
+        if (FULL)
+#pragma unroll
+          for (int q = 0; q < 7; ++q) acc[q] += shfl_xor_d(acc[q], m);
+      }
+    }
+    if (have && i < R.np && sub == 0) {
+      float4 *Fout = S.parity ? R.f32a : R.f32b;
+      Fout[i] = make_float4((float)fx, (float)fy, (float)fz, FULL ? (float)acc[0] : 0.f);
+    }
+    if (FULL) {
+      // fixed-order sum over the tile's waves
+      if (!have || sub != 0 || i >= R.np)
+#pragma unroll
+        for (int q = 0; q < 7; ++q) acc[q] = 0.0;
+#pragma unroll
+      for (int m = 32; m >= 1; m >>= 1)
+#pragma unroll
+        for (int q = 0; q < 7; ++q) acc[q] += shfl_xor_d(acc[q], m);
+      if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < 7; ++q) sh_red[threadIdx.x >> 6][q] = acc[q];
+      __syncthreads();
+      if (have && lt == 0) {
+        const int w0 = threadIdx.x >> 6;
+        for (int q = 0; q < 7; ++q) {
+          double v = 0.0;
+          for (int w = 0; w < SUB / 64; ++w) v += sh_red[w0 + w][q];
+          R.partials[T.ltile * kPartials + 1 + q] = v;
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ FP64 plain path
+
+__global__ __launch_bounds__(kTile) void k_drift_fp64(EngineDev E, int t0, int nt, int xflags) {
+  __shared__ double sh[8];
+  int tb, te, ts;
+  tile_range(t0, nt, tb, te, ts);
+  for (int t = tb; t < te; t += ts) {
+    const Tile T = E.tiles[t];
+    const RepParams &R = E.reps[T.rep];
+    const RepState &S = E.st[T.rep];
+    const int f = active_flags(E, T.rep, xflags);
+    if (!(f & F_ACTIVE)) continue;
+    const int i = T.atom0 + threadIdx.x;
+    double t1 = -1.0, t2 = -1.0;
+    if (i < R.np) {
+      const double dt = R.dt;
+      const double l1 = S.lam1, l2 = S.lam2;
+      double vx = R.vx[i], vy = R.vy[i], vz = R.vz[i];
+      if (l1 != 1.0 || l2 != 1.0) {
+        vx = vx * l1; vy = vy * l1; vz = vz * l1;
+        vx = vx * l2; vy = vy * l2; vz = vz * l2;
+        R.vx[i] = vx; R.vy[i] = vy; R.vz[i] = vz;
+      }
+      const double li0 = 1.0 / S.box[0], li1 = 1.0 / S.box[1], li2 = 1.0 / S.box[2];
+      const double px = R.px[i] + li0 * (vx * dt + 0.5 * R.ax[i] * dt * dt);
+      const double py = R.py[i] + li1 * (vy * dt + 0.5 * R.ay[i] * dt * dt);
+      const double pz = R.pz[i] + li2 * (vz * dt + 0.5 * R.az[i] * dt * dt);
+      R.px[i] = px; R.py[i] = py; R.pz[i] = pz;
+      const double dx = px - R.pbx[i], dy = py - R.pby[i], dz = pz - R.pbz[i];
+      if (R.trigger == NDG_TRIGGER_COMPAT) {
+        t1 = dx * dx;
+        t1 = t1 + dy * dy;
+        t1 = t1 + dz * dz;
+      } else {
+        const double ex = dx * S.box[0], ey = dy * S.box[1], ez = dz * S.box[2];
+        t1 = ex * ex + ey * ey + ez * ez;
+      }
+    }
+    block_top2(t1, t2, sh);
+    if (threadIdx.x == 0) {
+      R.partials[T.ltile * kPartials + 8] = t1;
+      R.partials[T.ltile * kPartials + 9] = t2;
+    }
+  }
+}
+
+// compute0 in FP64 on the full list: the reference's minimum image, d2 order, single-
+// precision table index and FP64 interpolation (src/nd.f90:283-330).  Same walk as the
+// mixed kernel (U entries in flight per lane, cutoff as a select), one atom per lane.
+template <bool FULL>
+__global__ __launch_bounds__(kTile) void k_force_fp64(EngineDev E, int t0, int nt, int xflags) {
+  __shared__ double sh_red[8 * 2];
+  constexpr int U = kUnroll;
+  int tb, te, ts;
+  tile_range(t0, nt, tb, te, ts);
+  for (int t = tb; t < te; t += ts) {
+    const Tile T = E.tiles[t];
+    const RepParams &R = E.reps[T.rep];
+    const RepState &S = E.st[T.rep];
+    const int f = active_flags(E, T.rep, xflags);
+    if (!(f & F_ACTIVE)) continue;
+    const bool full = (f & (F_CU | F_CV)) != 0;
+    if (full != FULL) continue;
+    const int i = T.atom0 + threadIdx.x;
+    const int tab = R.ptsize + 1;
+    const int kmax = R.ptsize;
+    const double b0 = S.box[0], b1 = S.box[1], b2 = S.box[2];
+    double fx = 0, fy = 0, fz = 0;
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (i < R.np) {
+      const double xi = R.px[i], yi = R.py[i], zi = R.pz[i];
+      const int ti = R.typ[i];
+      const int n = R.nnb[i];
+      const unsigned *L = R.list + ((size_t)T.ltile * R.list_chunks) * kTile + threadIdx.x;
+      for (int c = 0; c < n; c += U) {
+        int j[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) j[u] = (c + u < n) ? (int)(L[(size_t)(c + u) * kTile] & kIndexMask) : i;
+        double qx[U], qy[U], qz[U];
+        int tj[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          qx[u] = R.px[j[u]];
+          qy[u] = R.py[j[u]];
+          qz[u] = R.pz[j[u]];
+          tj[u] = R.typ[j[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int pid = R.pairmat[(ti - 1) + (tj[u] - 1) * R.ntypes] - 1;
+          double s0 = xi - qx[u], s1 = yi - qy[u], s2 = zi - qz[u];
+          s0 = s0 - round(s0);
+          s1 = s1 - round(s1);
+          s2 = s2 - round(s2);
+          const double r0 = b0 * s0, r1 = b1 * s1, r2 = b2 * s2;
+          double d2 = r0 * r0;
+          d2 = d2 + r1 * r1;
+          d2 = d2 + r2 * r2;
+          const bool ok = (c + u < n) && !(d2 > R.rc2d[pid]);
+          const float rk = (float)(d2 - R.ptrmin) * R.vdr2[pid] + 1.0f;
+          int k = (int)rk;
+          k = k < 1 ? 1 : (k > kmax ? kmax : k);
+          const double w = (double)rk - (double)k;
+          const double *Ft = R.tabF64 + (size_t)pid * tab;
+          double df = w * Ft[k] + (1.0 - w) * Ft[k - 1];
+          df = ok ? df / d2 : 0.0;
+          fx = fx + r0 * df;
+          fy = fy + r1 * df;
+          fz = fz + r2 * df;
+          if (FULL) {
+            const double *Pt = R.tabP64 + (size_t)pid * tab;
+            const double dp = ok ? w * Pt[k] + (1.0 - w) * Pt[k - 1] : 0.0;
+            acc[0] += 0.5 * dp;
+            acc[1] += 0.5 * (r0 * r0 * df);
+            acc[2] += 0.5 * (r1 * r1 * df);
+            acc[3] += 0.5 * (r2 * r2 * df);
+            acc[4] += 0.5 * (r0 * r1 * df);
+            acc[5] += 0.5 * (r0 * r2 * df);
+            acc[6] += 0.5 * (r1 * r2 * df);
+          }
+        }
+      }
+      R.fx[i] = fx;
+      R.fy[i] = fy;
+      R.fz[i] = fz;
+      if (FULL) R.pene[i] = acc[0];
+    }
+    if (FULL) {
+      for (int q = 0; q < 7; q += 2) {
+        double pr[2] = {acc[q], q + 1 < 7 ? acc[q + 1] : 0.0};
+        block_sum<2>(pr, sh_red);
+        if (threadIdx.x == 0) {
+          R.partials[T.ltile * kPartials + 1 + q] = pr[0];
+          if (q + 1 < 7) R.partials[T.ltile * kPartials + 2 + q] = pr[1];
+        }
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(kTile) void k_kick_fp64(EngineDev E, int t0, int nt, int xflags) {
+  __shared__ double sh[2];
+  int tb, te, ts;
+  tile_range(t0, nt, tb, te, ts);
+  for (int t = tb; t < te; t += ts) {
+    const Tile T = E.tiles[t];
+    const RepParams &R = E.reps[T.rep];
+    const int f = active_flags(E, T.rep, xflags);
+    if (!(f & F_ACTIVE)) continue;
+    const int i = T.atom0 + threadIdx.x;
+    double kt[1] = {0.0};
+    if (i < R.np) {
+      const int ty = R.typ[i];
+      const double m = R.mass[ty - 1];
+      const double rm = 1.0 / m;
+      const double hdt = 0.5 * R.dt;
+      const double gx = R.fx[i] * rm, gy = R.fy[i] * rm, gz = R.fz[i] * rm;
+      const double vx = R.vx[i] + hdt * (gx + R.ax[i]);
+      const double vy = R.vy[i] + hdt * (gy + R.ay[i]);
+      const double vz = R.vz[i] + hdt * (gz + R.az[i]);
+      R.vx[i] = vx; R.vy[i] = vy; R.vz[i] = vz;
+      R.ax[i] = gx; R.ay[i] = gy; R.az[i] = gz;
+      if (f & F_CK) {
+        double k = vx * vx;
+        k = k + vy * vy;
+        k = k + vz * vz;
+        kt[0] = k * m;
+      }
+    }
+    if (f & F_CK) {
+      block_sum<1>(kt, sh);
+      if (threadIdx.x == 0) R.partials[T.ltile * kPartials + 0] = kt[0];
+    }
+  }
+}
+
+// ------------------------------------------------------------------ reduction + ensembles
+
+enum : int { RED_FORCE = 1, RED_KICK = 2, RED_FLIP = 4, RED_STEP = 8, RED_CHECK = 16 };
+
+__device__ inline void regrid(const RepParams &R, RepState &S) {
+  int m[3];
+  for (int a = 0; a < 3; ++a) m[a] = (int)(S.box[a] / (R.rcut + R.skin));
+  while ((long long)m[0] * m[1] * m[2] > R.ncell_cap) {
+    int a = (m[0] >= m[1] && m[0] >= m[2]) ? 0 : (m[1] >= m[2] ? 1 : 2);
+    m[a] -= 1;
+  }
+  S.brute = (m[0] < 3 || m[1] < 3 || m[2] < 3) ? 1 : 0;
+  for (int a = 0; a < 3; ++a) S.mcell[a] = S.brute ? 1 : m[a];
+}
+
+// Rebuild criterion of CheckListUpdate (src/nd.f90:1488-1496) from the top-2 metric.
+__device__ inline int rebuild_test(const RepParams &R, RepState &S, int trigger, double d1,
+                                   double d2) {
+  if (d1 < 0) return 0;
+  if (d2 < 0) d2 = 0;
+  if (trigger == NDG_TRIGGER_COMPAT) {
+    const double dsum = d1 + d2;
+    if (dsum > R.skin * R.skin) {
+      if (dsum > 1.0) {
+        S.err = NDG_ERR_OUTSIDE;
+        return 0;
+      }
+      return 1;
+    }
+    return 0;
+  }
+  const double bmin = fmin(S.box[0], fmin(S.box[1], S.box[2]));
+  if (sqrt(d1) > 0.5 * bmin) {
+    S.err = NDG_ERR_OUTSIDE;
+    return 0;
+  }
+  double strain = 0.0;
+  for (int a = 0; a < 3; ++a) strain = fmax(strain, fabs(S.box[a] / S.box_build[a] - 1.0));
+  return (sqrt(d1) + sqrt(d2) + (R.rcut + R.skin) * strain > R.skin) ? 1 : 0;
+}
+
+// End of a step for one replica (src/nd.f90:164-178): e0, thermo record, mkIsoKE /
+// mkIsoP / mkIsoE, BoxVel, and the list-update decision for the next step.
+__device__ void step_epilogue(const RepParams &R, RepState &S, int f, double t1, double t2) {
+  const int s = S.step + 1;
+  const int tt = s - R.run_step0;
+  if (tt == 1) S.e0 = S.pe + S.ke;
+  if (f & F_OUT) {
+    ndg_thermo &rec = R.ring[S.nrec % kRing];
+    rec.step = s;
+    rec.pad_ = 0;
+    rec.pe = S.pe;
+    rec.ke = S.ke;
+    rec.eerr = (S.pe + S.ke - S.e0) / S.e0;
+    rec.pressure = (S.vir[0] + S.vir[1] + S.vir[2]) / (S.box[0] * S.box[1] * S.box[2] * 3.0);
+    rec.volume = S.box[0] * S.box[1] * S.box[2];
+    for (int q = 0; q < 6; ++q) rec.virial[q] = S.vir[q];
+    for (int q = 0; q < 3; ++q) rec.box[q] = S.box[q];
+    S.nrec += 1;
+  }
+  const double dt = R.dt;
+  double lam1 = 1.0, lam2 = 1.0;
+  if (R.isokinetic && s % R.ke_freq == 0) {
+    lam1 = 1.0 + dt / R.ke_tau * (R.ke_target * (double)R.np / S.ke - 1.0);
+    lam1 = sqrt(lam1);
+  }
+  if (R.isobaric && s % R.p_freq == 0) {
+    const double vol = S.box[0] * S.box[1] * S.box[2];
+    double pr[6], lm[6];
+    for (int q = 0; q < 6; ++q) pr[q] = S.vir[q] / vol;
+    for (int q = 0; q < 3; ++q) lm[q] = 1.0 - dt / R.p_tau * (R.p_target[q] - pr[q]);
+    lm[3] = dt / R.p_tau * pr[3];
+    lm[4] = dt / R.p_tau * pr[4];
+    lm[5] = dt / R.p_tau * pr[5];
+    S.box[0] = S.box[0] * lm[0] + S.box[1] * lm[5] + S.box[2] * lm[4];
+    S.box[1] = S.box[0] * lm[5] + S.box[1] * lm[1] + S.box[2] * lm[3];
+    S.box[2] = S.box[0] * lm[4] + S.box[1] * lm[3] + S.box[2] * lm[2];
+  }
+  if (R.isoenergetic && s % R.e_freq == 0)
+    lam2 = 1.0 - dt / R.e_tau * (S.e0 / (S.pe + S.ke) - 1.0);
+  if (R.box_load)
+    for (int q = 0; q < 3; ++q) S.box[q] = S.box[q] + R.box_vel[q] * dt;
+  S.lam1 = lam1;
+  S.lam2 = lam2;
+  S.parity ^= 1;
+  S.fnew = 0;
+  S.step = s;
+  S.rebuild = 0;
+  if (tt >= R.run_nsteps) {
+    S.done = 1;
+    return;
+  }
+  if (R.renei && (s + 1) % R.renei_freq == 0) {
+    if (rebuild_test(R, S, R.trigger, t1, t2)) {
+      S.nupdates += 1;
+      if (R.trigger == NDG_TRIGGER_SKIN) regrid(R, S);
+      S.rebuild = 1;
+    }
+  }
+}
+
+// One workgroup per replica: fixed-order sums of the tile partials, then (by mode) the
+// force / kick results, the drop-in rebuild check, or the full step epilogue.
+__global__ __launch_bounds__(256) void k_reduce(EngineDev E, int r0, int nr, int mode,
+                                                int xflags, int check_trigger) {
+  __shared__ double sh[8 * 8];
+  for (int r = r0 + blockIdx.x; r < r0 + nr; r += gridDim.x) {
+    const RepParams &R = E.reps[r];
+    RepState &S = E.st[r];
+    const int f = xflags ? xflags : step_flags(R, S);
+    if (!(f & F_ACTIVE)) continue;
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double t1 = -1.0, t2 = -1.0;
+    for (int lt = threadIdx.x; lt < R.ntiles; lt += blockDim.x) {
+      const double *p = R.partials + lt * kPartials;
+      if (f & F_CK) v[0] += p[0];
+      if (f & (F_CU | F_CV))
+        for (int q = 1; q < 8; ++q) v[q] += p[q];
+      top2_merge(t1, t2, p[8], p[9]);
+    }
+    block_sum<8>(v, sh);
+    block_top2(t1, t2, sh);
+    if (threadIdx.x == 0) {
+      if (mode & RED_FORCE) {
+        S.fnew = 1;
+        if (f & F_CU) S.pe = v[1];
+        if (f & F_CV)
+          for (int q = 0; q < 6; ++q) S.vir[q] = v[2 + q];
+      }
+      if (mode & RED_KICK) S.ke = (f & F_CK) ? 0.5 * v[0] : 0.0;
+      if (mode & RED_FLIP) {
+        S.parity ^= 1;
+        S.fnew = 0;
+      }
+      if (mode & RED_CHECK) S.rebuild = rebuild_test(R, S, check_trigger, t1, t2);
+      if (mode & RED_STEP) step_epilogue(R, S, f, t1, t2);
+    }
+    __syncthreads();
+  }
+}
+
+// Applies a pending velocity scale (end of a Run: the last step's mkIsoKE/mkIsoE).
+__global__ __launch_bounds__(256) void k_apply_pending(EngineDev E, int r0, int nr) {
+  for (int r = r0 + blockIdx.x; r < r0 + nr; r += gridDim.x) {
+    const RepParams &R = E.reps[r];
+    RepState &S = E.st[r];
+    const double l1 = S.lam1, l2 = S.lam2;
+    if (l1 == 1.0 && l2 == 1.0) continue;
+    for (int i = threadIdx.x; i < R.np; i += blockDim.x) {
+      double vx = R.vx[i], vy = R.vy[i], vz = R.vz[i];
+      vx = vx * l1; vy = vy * l1; vz = vz * l1;
+      vx = vx * l2; vy = vy * l2; vz = vz * l2;
+      R.vx[i] = vx; R.vy[i] = vy; R.vz[i] = vz;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      S.lam1 = 1.0;
+      S.lam2 = 1.0;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ list rebuild
+// Counting sort of atoms by reference cell (canonical order: cell, then caller index),
+// then a full Verlet list from the 27 surrounding cells in a fixed stencil order.
+
+__device__ inline void permute_atom(const RepParams &R, int src, int dst, int parity) {
+  R.tmpd[dst] = R.px[src];
+  R.tmpd[R.np + dst] = R.py[src];
+  R.tmpd[2 * R.np + dst] = R.pz[src];
+  R.tmpd[3 * R.np + dst] = R.vx[src];
+  R.tmpd[4 * R.np + dst] = R.vy[src];
+  R.tmpd[5 * R.np + dst] = R.vz[src];
+  if (R.path == NDG_PATH_FP64) {
+    R.tmpd[6 * R.np + dst] = R.ax[src];
+    R.tmpd[7 * R.np + dst] = R.ay[src];
+    R.tmpd[8 * R.np + dst] = R.az[src];
+  } else {
+    R.tmpf[dst] = R.f32a[src];
+    R.tmpf[R.np + dst] = R.f32b[src];
+  }
+  R.tmpi[dst] = R.typ[src];
+  R.tmpi[R.np + dst] = R.oid[src];
+}
+
+__device__ inline void place_atom(const RepParams &R, const RepState &S, int p) {
+  const double px = R.tmpd[p], py = R.tmpd[R.np + p], pz = R.tmpd[2 * R.np + p];
+  R.px[p] = px; R.py[p] = py; R.pz[p] = pz;
+  R.vx[p] = R.tmpd[3 * R.np + p];
+  R.vy[p] = R.tmpd[4 * R.np + p];
+  R.vz[p] = R.tmpd[5 * R.np + p];
+  const int ty = R.tmpi[p];
+  R.typ[p] = ty;
+  R.oid[p] = R.tmpi[R.np + p];
+  if (R.path == NDG_PATH_FP64) {
+    R.ax[p] = R.tmpd[6 * R.np + p];
+    R.ay[p] = R.tmpd[7 * R.np + p];
+    R.az[p] = R.tmpd[8 * R.np + p];
+    R.pbx[p] = px; R.pby[p] = py; R.pbz[p] = pz;
+  } else {
+    R.f32a[p] = R.tmpf[p];
+    R.f32b[p] = R.tmpf[R.np + p];
+  }
+  int kx, ky, kz, cxa, cya, cza;
+  bin_axis(px, S.mcell[0], S.brute, kx, cxa);
+  bin_axis(py, S.mcell[1], S.brute, ky, cya);
+  bin_axis(pz, S.mcell[2], S.brute, kz, cza);
+  const int cell = kx + ky * S.mcell[0] + kz * S.mcell[0] * S.mcell[1];
+  R.anchor[p] = make_int4(cxa, cya, cza, cell);
+  const double4 x = make_double4(anchored(px, cxa, S.mcell[0], S.box[0] / S.mcell[0]),
+                                 anchored(py, cya, S.mcell[1], S.box[1] / S.mcell[1]),
+                                 anchored(pz, cza, S.mcell[2], S.box[2] / S.mcell[2]),
+                                 type_bits(ty));
+  R.x64a[p] = x;
+  R.x32b[p] = make_float4((float)x.x, (float)x.y, (float)x.z, 0.f);
+}
+
+__device__ inline int cell_of(const RepParams &R, const RepState &S, int i) {
+  int kx, ky, kz, c;
+  bin_axis(R.px[i], S.mcell[0], S.brute, kx, c);
+  bin_axis(R.py[i], S.mcell[1], S.brute, ky, c);
+  bin_axis(R.pz[i], S.mcell[2], S.brute, kz, c);
+  return kx + ky * S.mcell[0] + kz * S.mcell[0] * S.mcell[1];
+}
+
+// Final order of one cell's slots [lo, hi): ascending caller index.  Cells of up to 64
+// atoms are sorted inside one wave (bitonic network over shuffles); larger cells use a
+// per-slot rank count.  The atoms are then copied to their final slots in tmp.
+__device__ inline void sort_cell_wave(const RepParams &R, int lo, int hi, int parity) {
+  const int lane = threadIdx.x & 63;
+  const int cnt = hi - lo;
+  if (cnt <= 64) {
+    int u = lane < cnt ? R.slot_src[lo + lane] : -1;
+    int key = lane < cnt ? R.oid[u] : 0x7fffffff;
+    for (int k = 2; k <= 64; k <<= 1)
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        const int okey = __shfl_xor(key, j), ou = __shfl_xor(u, j);
+        const bool up = (lane & k) == 0, lower = (lane & j) == 0;
+        const bool take = lower ? (up ? okey < key : okey > key) : (up ? okey > key : okey < key);
+        if (take) {
+          key = okey;
+          u = ou;
+        }
+      }
+    if (lane < cnt) permute_atom(R, u, lo + lane, parity);
+  } else {
+    for (int s = lo + lane; s < hi; s += 64) {
+      const int i = R.slot_src[s];
+      const int me = R.oid[i];
+      int rank = 0;
+      for (int v = lo; v < hi; ++v) rank += R.oid[R.slot_src[v]] < me;
+      permute_atom(R, i, lo + rank, parity);
+    }
+  }
+}
+
+// Whole sort of one small replica inside one workgroup (replicas up to kSmallSort atoms).
+constexpr int kSmallSort = 65536;
+
+__global__ __launch_bounds__(1024) void k_rb_small(EngineDev E, int r0, int nr) {
+  __shared__ int sh_scan[1024];
+  for (int r = r0 + blockIdx.x; r < r0 + nr; r += gridDim.x) {
+    const RepParams &R = E.reps[r];
+    RepState &S = E.st[r];
+    if (!S.rebuild || S.err || R.np > kSmallSort) continue;
+    const int np = R.np, nb = blockDim.x, tid = threadIdx.x;
+    const int ncell = S.mcell[0] * S.mcell[1] * S.mcell[2];
+    for (int c = tid; c < ncell; c += nb) { R.cell_count[c] = 0; R.cell_fill[c] = 0; }
+    __syncthreads();
+    for (int i = tid; i < np; i += nb) {
+      const int c = cell_of(R, S, i);
+      R.key[i] = c;
+      atomicAdd(&R.cell_count[c], 1);
+    }
+    __syncthreads();
+    // exclusive scan: thread t owns a contiguous chunk of cells
+    const int per = (ncell + nb - 1) / nb;
+    const int c0 = tid * per, c1 = min(ncell, c0 + per);
+    int loc = 0;
+    for (int c = c0; c < c1; ++c) loc += R.cell_count[c];
+    sh_scan[tid] = loc;
+    __syncthreads();
+    for (int off = 1; off < nb; off <<= 1) {
+      int v = tid >= off ? sh_scan[tid - off] : 0;
+      __syncthreads();
+      sh_scan[tid] += v;
+      __syncthreads();
+    }
+    int run = sh_scan[tid] - loc;
+    for (int c = c0; c < c1; ++c) { R.cell_start[c] = run; run += R.cell_count[c]; }
+    if (tid == nb - 1) R.cell_start[ncell] = sh_scan[nb - 1];
+    __syncthreads();
+    for (int i = tid; i < np; i += nb) {
+      const int c = R.key[i];
+      R.slot_src[R.cell_start[c] + atomicAdd(&R.cell_fill[c], 1)] = i;
+    }
+    __syncthreads();
+    const int parity = S.parity;
+    const int nwave = nb >> 6, wid = tid >> 6;
+    for (int c = wid; c < ncell; c += nwave) {
+      const int lo = R.cell_start[c], hi = R.cell_start[c + 1];
+      if (hi > lo) sort_cell_wave(R, lo, hi, parity);
+    }
+    __syncthreads();
+    for (int p = tid; p < np; p += nb) place_atom(R, S, p);
+    __syncthreads();
+    if (tid == 0) {
+      S.nbr_sum = 0;
+      for (int a = 0; a < 3; ++a) S.box_build[a] = S.box[a];
+    }
+  }
+}
+
+// Large replicas: the same sort in five grid-wide phases.
+__global__ __launch_bounds__(kTile) void k_rb_bin(EngineDev E, int t0, int nt) {
+  int tb, te, ts;
+  tile_range(t0, nt, tb, te, ts);
+  for (int t = tb; t < te; t += ts) {
+    const Tile T = E.tiles[t];
+    const RepParams &R = E.reps[T.rep];
+    const RepState &S = E.st[T.rep];
+    if (!S.rebuild || S.err || R.np <= kSmallSort) continue;
+    const int i = T.atom0 + threadIdx.x;
+    if (i >= R.np) continue;
+    const int c = cell_of(R, S, i);
+    R.key[i] = c;
+    atomicAdd(&R.cell_count[c], 1);
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_rb_scan(EngineDev E, int r0, int nr) {
+  __shared__ int sh_scan[1024];
+  for (int r = r0 + blockIdx.x; r < r0 + nr; r += gridDim.x) {
+    const RepParams &R = E.reps[r];
+    RepState &S = E.st[r];
+    if (!S.rebuild || S.err || R.np <= kSmallSort) continue;
+    const int nb = blockDim.x, tid = threadIdx.x;
+    const int ncell = S.mcell[0] * S.mcell[1] * S.mcell[2];
+    const int per = (ncell + nb - 1) / nb;
+    const int c0 = tid * per, c1 = min(ncell, c0 + per);
+    int loc = 0;
+    for (int c = c0; c < c1; ++c) loc += R.cell_count[c];
+    sh_scan[tid] = loc;
+    __syncthreads();
+    for (int off = 1; off < nb; off <<= 1) {
+      int v = tid >= off ? sh_scan[tid - off] : 0;
+      __syncthreads();
+      sh_scan[tid] += v;
+      __syncthreads();
+    }
+    int run = sh_scan[tid] - loc;
+    for (int c = c0; c < c1; ++c) {
+      R.cell_start[c] = run;
+      run += R.cell_count[c];
+      R.cell_count[c] = 0;
+      R.cell_fill[c] = 0;
+    }
+    if (tid == nb - 1) R.cell_start[ncell] = sh_scan[nb - 1];
+    if (tid == 0) {
+      S.nbr_sum = 0;
+      for (int a = 0; a < 3; ++a) S.box_build[a] = S.box[a];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kTile) void k_rb_scatter(EngineDev E, int t0, int nt) {
+  int tb, te, ts;
+  tile_range(t0, nt, tb, te, ts);
+  for (int t = tb; t < te; t += ts) {
+    const Tile T = E.tiles[t];
+    const RepParams &R = E.reps[T.rep];
+    const RepState &S = E.st[T.rep];
+    if (!S.rebuild || S.err || R.np <= kSmallSort) continue;
+    const int i = T.atom0 + threadIdx.x;
+    if (i >= R.np) continue;
+    const int c = R.key[i];
+    R.slot_src[R.cell_start[c] + atomicAdd(&R.cell_fill[c], 1)] = i;
+  }
+}
+
+// Each tile sorts the cells whose first slot lies in it (one wave per cell).
+__global__ __launch_bounds__(kTile) void k_rb_rank(EngineDev E, int t0, int nt) {
+  int tb, te, ts;
+  tile_range(t0, nt, tb, te, ts);
+  for (int t = tb; t < te; t += ts) {
+    const Tile T = E.tiles[t];
+    const RepParams &R = E.reps[T.rep];
+    const RepState &S = E.st[T.rep];
+    if (!S.rebuild || S.err || R.np <= kSmallSort) continue;
+    const int ncell = S.mcell[0] * S.mcell[1] * S.mcell[2];
+    int c = R.key[R.slot_src[T.atom0]];
+    if (R.cell_start[c] < T.atom0) ++c;
+    const int wid = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    for (c += wid; c < ncell && R.cell_start[c] < T.atom0 + kTile; c += nwave) {
+      const int lo = R.cell_start[c], hi = R.cell_start[c + 1];
+      if (hi > lo) sort_cell_wave(R, lo, hi, S.parity);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kTile) void k_rb_place(EngineDev E, int t0, int nt) {
+  int tb, te, ts;
+  tile_range(t0, nt, tb, te, ts);
+  for (int t = tb; t < te; t += ts) {
+    const Tile T = E.tiles[t];
+    const RepParams &R = E.reps[T.rep];
+    const RepState &S = E.st[T.rep];
+    if (!S.rebuild || S.err || R.np <= kSmallSort) continue;
+    const int p = T.atom0 + threadIdx.x;
+    if (p >= R.np) continue;
+    place_atom(R, S, p);
+  }
+}
+
+// Full Verlet list (both directions) of each atom: the reference's pair test
+// (src/nd.f90:1358-1377,1404-1420) over the 27 surrounding cells (or all atoms in brute
+// mode).  A workgroup takes the cells whose first slot lies in its tile, stages each
+// cell's candidates through LDS in canonical order (stencil cell (dz,dy,dx)
+// lexicographic, then slot), and each wave walks one home atom at a time, 64 candidates
+// per step, appending accepted ones in order (ballot prefix).  Entries carry the
+// cell/image offset code used by the mixed force kernel.
+constexpr int kStage = 1024;
+constexpr int kListThreads = 256;
+
+struct StageBuf {
+  double x[kStage], y[kStage], z[kStage];
+  int t[kStage], u[kStage], q[kStage];
+  int ax[kStage], ay[kStage], az[kStage];
+};
+
+// home atoms [h0, h1); candidate ranges rlo[q], prefix rpref[q] (shared memory)
+__device__ void build_group(const RepParams &R, RepState &S, StageBuf &sb, int *hn,
+                            const int *rlo, const int *rpref, int nrange, int same_q,
+                            int h0, int h1, long long &mine) {
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, nw = blockDim.x >> 6;
+  const int nh = h1 - h0;
+  for (int h = tid; h < nh; h += blockDim.x) hn[h] = 0;
+  const int mx = S.mcell[0], my = S.mcell[1], mz = S.mcell[2];
+  const double b0 = S.box[0], b1 = S.box[1], b2 = S.box[2];
+  const int nt_ = R.ntypes, tpa = R.tpa;
+  const int total = rpref[nrange];
+  for (int base = 0; base < total; base += kStage) {
+    const int cnt = min(kStage, total - base);
+    __syncthreads();
+    for (int g = tid; g < cnt; g += blockDim.x) {
+      const int gg = base + g;
+      int q = 0;
+      while (rpref[q + 1] <= gg) ++q;
+      const int u = rlo[q] + (gg - rpref[q]);
+      sb.x[g] = R.px[u];
+      sb.y[g] = R.py[u];
+      sb.z[g] = R.pz[u];
+      sb.t[g] = R.typ[u];
+      sb.u[g] = u;
+      sb.q[g] = q;
+      const int4 A = R.anchor[u];
+      sb.ax[g] = A.x;
+      sb.ay[g] = A.y;
+      sb.az[g] = A.z;
+    }
+    __syncthreads();
+    for (int h = wid; h < nh; h += nw) {
+      const int i = h0 + h;
+      const double xi = R.px[i], yi = R.py[i], zi = R.pz[i];
+      const int ti = R.typ[i];
+      const int4 Ai = R.anchor[i];
+      unsigned *L = R.list + ((size_t)(i / kTile) * R.list_chunks) * (kTile * tpa) + (i % kTile) * tpa;
+      int n = hn[h];
+      bool over = n < 0;
+      for (int s0 = 0; s0 < cnt && !over; s0 += 64) {
+        const int s2 = s0 + lane;
+        bool acc = false;
+        unsigned ent = 0;
+        bool bad = false;
+        if (s2 < cnt) {
+          const int u = sb.u[s2];
+          const int tj = sb.t[s2];
+          const int pij = R.pairmat[(ti - 1) + (tj - 1) * nt_];
+          const int pji = R.pairmat[(tj - 1) + (ti - 1) * nt_];
+          const int pm = pij > pji ? pij : pji;
+          if (u != i && pm > 0) {
+            const double dpx = xi - sb.x[s2], dpy = yi - sb.y[s2], dpz = zi - sb.z[s2];
+            const double nx = round(dpx), ny = round(dpy), nz = round(dpz);
+            const double r0 = b0 * (dpx - nx), r1 = b1 * (dpy - ny), r2 = b2 * (dpz - nz);
+            double d2 = r0 * r0;
+            d2 = d2 + r1 * r1;
+            d2 = d2 + r2 * r2;
+            const double rcs = R.skin + R.rc_pair[pm - 1];
+            const bool excl = sb.q[s2] == same_q && d2 < R.ptrmin;
+            if (!excl && !(d2 > rcs * rcs)) {
+              acc = true;
+              const int ox = (Ai.x - sb.ax[s2]) - (int)nx * mx;
+              const int oy = (Ai.y - sb.ay[s2]) - (int)ny * my;
+              const int oz = (Ai.z - sb.az[s2]) - (int)nz * mz;
+              // offsets beyond one cell cannot occur for mcell >= 3 or brute grids
+              bad = ox < -1 || ox > 1 || oy < -1 || oy > 1 || oz < -1 || oz > 1;
+              const unsigned code = (unsigned)((ox + 1) | ((oy + 1) << 2) | ((oz + 1) << 4));
+              ent = (unsigned)u | (code << kCodeShift);
+            }
+          }
+        }
+        const unsigned long long bal = __ballot(acc);
+        const unsigned long long badb = __ballot(bad);
+        const int nacc = __popcll(bal);
+        if (badb || n + nacc > R.neimax) {
+          over = true;
+          break;
+        }
+        if (acc) {
+          const int k = n + __popcll(bal & ((1ull << lane) - 1ull));
+          L[(size_t)(k / tpa) * (kTile * tpa) + k % tpa] = ent;
+        }
+        n += nacc;
+      }
+      if (lane == 0) hn[h] = over ? -1 : n;
+    }
+  }
+  __syncthreads();
+  for (int h = tid; h < nh; h += blockDim.x) {
+    const int n = hn[h];
+    if (n < 0) {
+      atomicMax(&S.err, NDG_ERR_NEIMAX);
+      R.nnb[h0 + h] = 0;
+    } else {
+      R.nnb[h0 + h] = n;
+      mine += n;
+    }
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kListThreads) void k_rb_list(EngineDev E, int t0, int nt) {
+  __shared__ StageBuf sb;
+  __shared__ int s_hn[kTile];
+  __shared__ int s_lo[27], s_cnt[27], s_pref[28];
+  __shared__ unsigned long long s_sum;
+  int tb, te, ts;
+  tile_range(t0, nt, tb, te, ts);
+  for (int t = tb; t < te; t += ts) {
+    const Tile T = E.tiles[t];
+    const RepParams &R = E.reps[T.rep];
+    RepState &S = E.st[T.rep];
+    if (!S.rebuild || S.err) continue;
+    const int tid = threadIdx.x;
+    long long mine = 0;
+    __syncthreads();
+    if (tid == 0) s_sum = 0;
+    if (S.brute) {
+      if (tid == 0) {
+        s_lo[0] = 0;
+        s_pref[0] = 0;
+        s_pref[1] = R.np;
+      }
+      __syncthreads();
+      build_group(R, S, sb, s_hn, s_lo, s_pref, 1, -1, T.atom0, min(T.atom0 + kTile, R.np), mine);
+    } else {
+      const int mx = S.mcell[0], my = S.mcell[1], mz = S.mcell[2];
+      const int ncell = mx * my * mz;
+      int c = R.anchor[T.atom0].w;
+      if (R.cell_start[c] < T.atom0) ++c;
+      for (; c < ncell && R.cell_start[c] < T.atom0 + kTile; ++c) {
+        const int lo = R.cell_start[c], hi = R.cell_start[c + 1];
+        if (hi <= lo) continue;
+        __syncthreads();
+        if (tid < 27) {
+          const int cx = c % mx, cy = (c / mx) % my, cz = c / (mx * my);
+          const int dz = tid / 9 - 1, dy = (tid / 3) % 3 - 1, dx = tid % 3 - 1;
+          int ux = cx + dx, uy = cy + dy, uz = cz + dz;
+          ux += ux < 0 ? mx : (ux >= mx ? -mx : 0);
+          uy += uy < 0 ? my : (uy >= my ? -my : 0);
+          uz += uz < 0 ? mz : (uz >= mz ? -mz : 0);
+          const int cu = ux + uy * mx + uz * mx * my;
+          s_lo[tid] = R.cell_start[cu];
+          s_cnt[tid] = R.cell_start[cu + 1] - R.cell_start[cu];
+        }
+        __syncthreads();
+        if (tid == 0) {
+          s_pref[0] = 0;
+          for (int q = 0; q < 27; ++q) s_pref[q + 1] = s_pref[q] + s_cnt[q];
+        }
+        __syncthreads();
+        for (int h0 = lo; h0 < hi; h0 += kTile)
+          build_group(R, S, sb, s_hn, s_lo, s_pref, 27, 13, h0, min(h0 + kTile, hi), mine);
+      }
+    }
+    __syncthreads();
+    atomicAdd(&s_sum, (unsigned long long)mine);
+    __syncthreads();
+    if (tid == 0 && s_sum) atomicAdd((unsigned long long *)&S.nbr_sum, s_sum);
+  }
+}
+
+// ------------------------------------------------------------------ launchers
+
+static int grid_for(int nt, int per_cu) {
+  int g = 256 * per_cu;
+  if (nt < g) g = nt;
+  if (g < 1) g = 1;
+  return g;
+}
+
+void launch_drift(const EngineDev &E, int path, int t0, int nt, int xflags, hipStream_t s) {
+  if (nt <= 0) return;
+  if (path == NDG_PATH_FP64)
+    hipLaunchKernelGGL(k_drift_fp64, dim3(grid_for(nt, 8)), dim3(kTile), 0, s, E, t0, nt, xflags);
+  else
+    hipLaunchKernelGGL(k_drift_mixed, dim3(grid_for(nt, 8)), dim3(kTile), 0, s, E, t0, nt, xflags);
+}
+
+void launch_kick(const EngineDev &E, int path, int t0, int nt, int xflags, hipStream_t s) {
+  if (nt <= 0) return;
+  if (path == NDG_PATH_FP64)
+    hipLaunchKernelGGL(k_kick_fp64, dim3(grid_for(nt, 8)), dim3(kTile), 0, s, E, t0, nt, xflags);
+  else
+    hipLaunchKernelGGL(k_kick_mixed, dim3(grid_for(nt, 8)), dim3(kTile), 0, s, E, t0, nt, xflags);
+}
+
+template <int TPA>
+static void launch_force_tpa(const EngineDev &E, bool full, int t0, int nt, int xflags,
+                             hipStream_t s) {
+  using SH = ForceShape<TPA>;
+  const int nblk = (nt + SH::kNSub - 1) / SH::kNSub;
+  const int g = grid_for(nblk, 2);
+  const size_t lds = sizeof(float) * (size_t)E.tabset_n;
+  if (full)
+    hipLaunchKernelGGL((k_force_mixed<TPA, true>), dim3(g), dim3(SH::kThreads), lds, s, E, t0, nt, xflags);
+  else
+    hipLaunchKernelGGL((k_force_mixed<TPA, false>), dim3(g), dim3(SH::kThreads), lds, s, E, t0, nt, xflags);
+}
+
+void launch_force(const EngineDev &E, int path, int tpa, bool full, int t0, int nt, int xflags,
+                  hipStream_t s) {
+  if (nt <= 0) return;
+  if (path == NDG_PATH_FP64) {
+    if (full)
+      hipLaunchKernelGGL(k_force_fp64<true>, dim3(grid_for(nt, 8)), dim3(kTile), 0, s, E, t0, nt, xflags);
+    else
+      hipLaunchKernelGGL(k_force_fp64<false>, dim3(grid_for(nt, 8)), dim3(kTile), 0, s, E, t0, nt, xflags);
+    return;
+  }
+  switch (tpa) {
+    case 1: launch_force_tpa<1>(E, full, t0, nt, xflags, s); break;
+    case 2: launch_force_tpa<2>(E, full, t0, nt, xflags, s); break;
+    case 4: launch_force_tpa<4>(E, full, t0, nt, xflags, s); break;
+    default: launch_force_tpa<8>(E, full, t0, nt, xflags, s); break;
+  }
+}
+
+void launch_reduce(const EngineDev &E, int r0, int nr, int mode, int xflags, int trigger,
+                   hipStream_t s) {
+  if (nr <= 0) return;
+  hipLaunchKernelGGL(k_reduce, dim3(nr < 1024 ? nr : 1024), dim3(256), 0, s, E, r0, nr, mode,
+                     xflags, trigger);
+}
+
+void launch_apply_pending(const EngineDev &E, int r0, int nr, hipStream_t s) {
+  if (nr <= 0) return;
+  hipLaunchKernelGGL(k_apply_pending, dim3(nr < 1024 ? nr : 1024), dim3(256), 0, s, E, r0, nr);
+}
+
+void launch_rebuild(const EngineDev &E, int t0, int nt, int r0, int nr, bool any_large,
+                    hipStream_t s) {
+  if (nt <= 0) return;
+  const int gr = nr < 1024 ? nr : 1024;
+  hipLaunchKernelGGL(k_rb_small, dim3(gr), dim3(1024), 0, s, E, r0, nr);
+  if (any_large) {
+    const int g = grid_for(nt, 8);
+    hipLaunchKernelGGL(k_rb_bin, dim3(g), dim3(kTile), 0, s, E, t0, nt);
+    hipLaunchKernelGGL(k_rb_scan, dim3(gr), dim3(1024), 0, s, E, r0, nr);
+    hipLaunchKernelGGL(k_rb_scatter, dim3(g), dim3(kTile), 0, s, E, t0, nt);
+    hipLaunchKernelGGL(k_rb_rank, dim3(g), dim3(kTile), 0, s, E, t0, nt);
+    hipLaunchKernelGGL(k_rb_place, dim3(g), dim3(kTile), 0, s, E, t0, nt);
+  }
+  hipLaunchKernelGGL(k_rb_list, dim3(grid_for(nt, 4)), dim3(kListThreads), 0, s, E, t0, nt);
+}
+
+}  // namespace ndg

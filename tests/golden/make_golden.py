@@ -13,9 +13,7 @@ the reference with one OpenMP thread (its force sum order then equals a serial s
 from __future__ import annotations
 
 import os
-import subprocess
 import sys
-import tempfile
 
 import numpy as np
 
@@ -23,54 +21,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)
 sys.path.insert(0, ROOT)
 from ndyn_omp_amd import workloads as W  # noqa: E402
 
-HARNESS = os.path.join(ROOT, "oracle", "_ref", "ndyn_ref_harness")
+from oracle import refrun  # noqa: E402
+
+HARNESS = refrun.HARNESS
 OUT = os.path.dirname(os.path.abspath(__file__))
-
-
-def write_input(path, mode, s, nsteps=0, outsteps=1, step0=0, neimax=0, skin=-1.0, dt=0.0,
-                renei=True, renei_freq=1, iso_ke=None, iso_p=None, iso_e=None, acc=None):
-    iv = np.array([mode, s.np, s.ntypes, s.npairs, s.ptsize, neimax, nsteps, outsteps, step0,
-                   int(renei), renei_freq,
-                   1 if iso_ke else 0, iso_ke[2] if iso_ke else 1,
-                   1 if iso_p else 0, iso_p[2] if iso_p else 1,
-                   1 if iso_e else 0, iso_e[1] if iso_e else 1, 1], dtype="<i4")
-    dv = np.zeros(13, dtype="<f8")
-    dv[0:3] = s.box
-    dv[3] = dt
-    dv[4] = skin
-    dv[5] = s.ptrmin
-    if iso_ke:
-        dv[6], dv[7] = iso_ke[0], iso_ke[1]
-    if iso_p:
-        dv[8:11] = iso_p[0]
-        dv[11] = iso_p[1]
-    if iso_e:
-        dv[12] = iso_e[0]
-    with open(path, "wb") as f:
-        f.write(iv.tobytes())
-        f.write(dv.tobytes())
-        f.write(np.asarray(s.masses, "<f8").tobytes())
-        f.write(np.asarray(s.pairmat.T, "<i4").tobytes())          # column-major
-        f.write(np.asarray(s.kinds, "<i4").tobytes())
-        f.write(np.asarray(s.params, "<f8").tobytes())             # (npairs,4) -> 4 x npairs col-major
-        f.write(np.asarray(s.typ, "<i4").tobytes())
-        f.write(np.asarray(s.pos, "<f8").tobytes())                # (np,3) -> 3 x np col-major
-        f.write(np.asarray(s.vel, "<f8").tobytes())
-        a = np.zeros_like(s.pos) if acc is None else acc
-        f.write(np.asarray(a, "<f8").tobytes())
-
-
-def run_harness(mode, s, **kw):
-    with tempfile.TemporaryDirectory() as d:
-        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
-        write_input(fin, mode, s, **kw)
-        env = dict(os.environ, OMP_NUM_THREADS="1")
-        p = subprocess.run([HARNESS, fin, fout], cwd=d, env=env, capture_output=True,
-                           text=True, timeout=3600)
-        if p.returncode != 0:
-            raise RuntimeError(p.stdout + p.stderr)
-        raw = open(fout, "rb").read()
-        return raw, p.stdout
+run_harness = refrun.run_harness
 
 
 def sys_arrays(s):

@@ -1,0 +1,273 @@
+"""Parity of the HIP engine (through the C ABI) with the oracle and the reference goldens.
+
+Bars (stated per test): neighbour sets bit-exact; FP64 path forces/energy/virial within
+1e-12 relative; mixed path RMS relative force error <= 1e-5, energy and virial <= 1e-6
+relative; trajectories as stated per test.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import golden, system_from_golden
+from ndyn_omp_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+G = pytest.importorskip("ndyn_omp_amd.engine")
+
+PATHS = [("fp64", 0), ("mixed", 1)]
+
+
+def rms_rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def full_sets_from_golden(g, n):
+    counts = g["counts"]
+    ent = g["entries"]
+    ii = np.repeat(np.arange(n), counts)
+    a = np.concatenate([ii, ent])
+    b = np.concatenate([ent, ii])
+    order = np.lexsort((b, a))
+    a, b = a[order], b[order]
+    starts = np.searchsorted(a, np.arange(n + 1))
+    return [b[starts[i]:starts[i + 1]] for i in range(n)]
+
+
+@pytest.mark.parametrize("path_name,path", PATHS)
+@pytest.mark.parametrize("fname", ["list_cu7.npz", "list_cu10.npz", "list_alloy5.npz",
+                                   "list_ar6.npz"])
+def test_neighbour_sets_bit_exact(fname, path_name, path):
+    """The device full list equals the reference half list (golden) as sorted index sets."""
+    g = golden(fname)
+    s = system_from_golden(g)
+    print(f"case {fname} {path_name}", flush=True)
+    sim = G.Simulation(s, path=path, neimax=int(g["neimax"]))
+    sim.build_lists(float(g["skin"]))
+    counts, nbr = sim.neighbours()
+    ref = full_sets_from_golden(g, s.np)
+    for i in range(s.np):
+        assert counts[i] == len(ref[i]), i
+        np.testing.assert_array_equal(nbr[i, :counts[i]], ref[i])
+    sim.close()
+
+
+@pytest.mark.parametrize("fname", ["force_cu10.npz", "force_ar6.npz", "force_alloy5.npz"])
+def test_forces_fp64_path(fname, oracle):
+    """FP64 path vs the compiled reference (golden): forces, PE, virial within 1e-12."""
+    g = golden(fname)
+    s = system_from_golden(g)
+    print(f"case {fname} fp64", flush=True)
+    sim = G.Simulation(s, path=G.PATH_FP64)
+    sim.build_lists(float(g["skin"]))
+    pe, vir = sim.compute(True, True)
+    st = sim.state()
+    assert rms_rel(st["force"], g["force"]) < 1e-12
+    assert np.abs(st["force"] - g["force"]).max() < 1e-11 * np.abs(g["force"]).max()
+    assert pe == pytest.approx(float(g["pe"]), rel=1e-12)
+    np.testing.assert_allclose(vir, g["virial"], rtol=1e-12, atol=1e-12 * np.abs(g["virial"]).max())
+    # per-atom energy (not in the reference): sums to PE, equals the oracle's 1/2 sum_j V_ij
+    tables = oracle.pair_tables(s.kinds, s.params, s.ptrmin, s.ptsize)
+    _, vnl, nm = oracle.half_list(s, float(g["skin"]))
+    _, _, _, pene = oracle.compute0(s, vnl, nm, tables)
+    np.testing.assert_allclose(st["pene"], pene, rtol=1e-12, atol=1e-14)
+    sim.close()
+
+
+@pytest.mark.parametrize("fname", ["force_cu10.npz", "force_ar6.npz", "force_alloy5.npz"])
+def test_forces_mixed_path(fname, oracle):
+    """Mixed path vs the FP64 reference: RMS relative force error <= 1e-5; PE and virial
+    trace <= 1e-6 relative; per-atom energy within 1e-6 eV of the oracle."""
+    g = golden(fname)
+    s = system_from_golden(g)
+    print(f"case {fname} mixed", flush=True)
+    sim = G.Simulation(s, path=G.PATH_MIXED)
+    sim.build_lists(float(g["skin"]))
+    pe, vir = sim.compute(True, True)
+    st = sim.state()
+    err = rms_rel(st["force"], g["force"])
+    print(f"  rms rel force err {err:.3e}")
+    assert err <= 1e-5
+    assert pe == pytest.approx(float(g["pe"]), rel=1e-6)
+    assert vir[:3].sum() == pytest.approx(float(g["virial"][:3].sum()), rel=1e-6)
+    tables = oracle.pair_tables(s.kinds, s.params, s.ptrmin, s.ptsize)
+    _, vnl, nm = oracle.half_list(s, float(g["skin"]))
+    _, _, _, pene = oracle.compute0(s, vnl, nm, tables)
+    assert np.abs(st["pene"] - pene).max() < 1e-6
+    sim.close()
+
+
+def _golden_run_kwargs(g):
+    kw = {}
+    if "iso_ke" in g.files:
+        ke = g["iso_ke"]
+        kw["isokinetic"] = (float(ke[0]), float(ke[1]), int(ke[2]))
+    if "iso_p" in g.files:
+        p = g["iso_p"]
+        kw["isobaric"] = ((p[0], p[1], p[2]), float(p[3]), int(p[4]))
+    if "iso_e" in g.files:
+        e = g["iso_e"]
+        kw["isoenergetic"] = (float(e[0]), int(e[1]))
+    return kw
+
+
+@pytest.mark.parametrize("fname", ["run_cu10_nve.npz", "run_ar6_nvt.npz", "run_cu7_npt.npz",
+                                   "run_alloy5_nve.npz"])
+def test_dynamics_fp64_path_matches_reference(fname):
+    """'dynamics' on the device (FP64 path) vs the compiled reference: final positions
+    within 1e-10 (scaled), velocities 1e-8 relative, box 1e-12, every thermo line."""
+    g = golden(fname)
+    s = system_from_golden(g)
+    kw = _golden_run_kwargs(g)
+    print(f"case {fname} fp64 run", flush=True)
+    sim = G.Simulation(s, path=G.PATH_FP64)
+    rp = G.run_params(int(g["nsteps"]), float(g["dt"]), out_steps=int(g["outsteps"]),
+                      skin=float(g["skin"]), **kw)
+    recs, nup, step = sim.run(rp)
+    st = sim.state()
+    assert step == int(g["out_step"]) and nup == int(g["out_nupdates"])
+    np.testing.assert_allclose(st["box"], g["out_box"], rtol=1e-12)
+    np.testing.assert_allclose(st["pos"], g["out_pos"], rtol=0, atol=1e-10)
+    np.testing.assert_allclose(st["vel"], g["out_vel"], rtol=1e-8, atol=1e-10)
+    t = g["thermo"]
+    assert len(recs) == t.shape[0]
+    for row, rec in zip(t, recs):
+        assert rec["step"] == int(row[0])
+        for k, col in (("pe", 1), ("ke", 2), ("volume", 5)):
+            assert rec[k] == pytest.approx(row[col], rel=1e-5, abs=1e-9)
+    sim.close()
+
+
+@pytest.mark.parametrize("fname", ["run_cu10_nve.npz", "run_ar6_nvt.npz", "run_cu7_npt.npz",
+                                   "run_alloy5_nve.npz"])
+def test_dynamics_mixed_path_tracks_reference(fname):
+    """Mixed path 'dynamics' vs the reference: after the run positions agree to 1e-5 A
+    (max) and 1e-6 A (RMS) -- FP32 forces, so trajectories separate slowly -- thermo PE/KE
+    within 1e-5 relative, box within 1e-8 relative."""
+    g = golden(fname)
+    s = system_from_golden(g)
+    kw = _golden_run_kwargs(g)
+    print(f"case {fname} mixed run", flush=True)
+    sim = G.Simulation(s, path=G.PATH_MIXED)
+    rp = G.run_params(int(g["nsteps"]), float(g["dt"]), out_steps=int(g["outsteps"]),
+                      skin=float(g["skin"]), **kw)
+    recs, nup, step = sim.run(rp)
+    st = sim.state()
+    np.testing.assert_allclose(st["box"], g["out_box"], rtol=1e-8)
+    dpos = (st["pos"] - g["out_pos"]) * g["out_box"]
+    assert np.abs(dpos).max() < 1e-5
+    assert math.sqrt((dpos ** 2).mean()) < 1e-6
+    t = g["thermo"]
+    for row, rec in zip(t, recs):
+        assert rec["pe"] == pytest.approx(row[1], rel=1e-5)
+        assert rec["ke"] == pytest.approx(row[2], rel=1e-5, abs=1e-9)
+    sim.close()
+
+
+@pytest.mark.parametrize("path_name,path", PATHS)
+def test_testT_barostat_log(path_name, path):
+    """examples/testT run 1 (perfect lattice at rest, Isobaric 0 0 0 0.1 10, ReNeighbor F,
+    dt 0.1): examples/testT/out:38-40 -> PE -13581.1 V 49836.0 at step 1, PE -12798.9
+    V 56637.3 at step 100, PE -12798.8 V 56637.5 at step 200."""
+    s = W.cu_morse(10, a=3.68, temp_k=0.0, params=(6.0, 0.3429, 1.359, 2.866), ptsize=1000,
+                   fp32_like_dump=True)
+    sim = G.Simulation(s, path=path)
+    recs, _, _ = sim.run(G.run_params(200, 0.1, out_steps=100, skin=0.2, renei=False,
+                                      isobaric=((0.0, 0.0, 0.0), 0.1, 10)))
+    got = [(r["step"], float(f"{r['pe']:.6g}"), float(f"{r['volume']:.6g}")) for r in recs]
+    assert got == [(1, -13581.1, 49836.0), (100, -12798.9, 56637.3), (200, -12798.8, 56637.5)]
+    sim.close()
+
+
+@pytest.mark.parametrize("path_name,path", PATHS)
+def test_drop_in_step_sequence(path_name, path, oracle):
+    """The reference's per-routine sequence (integrate1, compute0, integrate2, mkIsoKE)
+    driven through the fine-grained entry points matches the oracle step by step."""
+    s = W.cu_morse(7, temp_k=500.0, seed=21)
+    sim = G.Simulation(s, path=path)
+    sim.build_lists(0.5)
+    dt = W.DT_1FS
+    tables = oracle.pair_tables(s.kinds, s.params, s.ptrmin, s.ptsize)
+    pos, vel, acc = s.pos.copy(), s.vel.copy(), np.zeros_like(s.pos)
+    disp = np.zeros_like(pos)
+    _, vnl, nm = oracle.half_list(s, 0.5)
+    L = oracle.lib()
+    tol_f = 1e-12 if path == 0 else 1e-5
+    for it in range(5):
+        sim.integrate1(dt)
+        L.ndo_integrate1(s.np, pos, vel, acc, disp, np.ascontiguousarray(s.box), dt)
+        pe, vir = sim.compute(True, True)
+        s2 = s.copy()
+        s2.pos = pos
+        f, pe_r, vir_r, _ = oracle.compute0(s2, vnl, nm, tables)
+        ke = sim.integrate2(dt, True)
+        import ctypes
+        ke_r = ctypes.c_double(0)
+        L.ndo_integrate2(s.np, vel, acc, f, s.typ.astype(np.int32), np.ascontiguousarray(s.masses),
+                         dt, 1, ctypes.byref(ke_r))
+        assert pe == pytest.approx(pe_r, rel=1e-12 if path == 0 else 1e-6)
+        assert ke == pytest.approx(ke_r.value, rel=1e-12 if path == 0 else 1e-7)
+        st = sim.state()
+        assert rms_rel(st["force"], f) < tol_f
+        lam = math.sqrt(1.0 + 0.1 * (1.5 * W.KB_EV * 300.0 * s.np / ke - 1.0))
+        sim.scale_velocities(lam)
+        vel *= math.sqrt(1.0 + 0.1 * (1.5 * W.KB_EV * 300.0 * s.np / ke_r.value - 1.0))
+    st = sim.state()
+    vtol = 1e-12 if path == 0 else 1e-6 * np.abs(vel).max()   # FP32 forces: absolute bar
+    np.testing.assert_allclose(st["vel"], vel, rtol=1e-9 if path == 0 else 0.0, atol=vtol)
+    assert not sim.check_rebuild(0.5)
+    sim.close()
+
+
+def test_neimax_overflow_error():
+    """NeiMax overflow maps to the reference's 'too many neighbours in list' (code 1)."""
+    s = W.cu_morse(7, temp_k=0.0)
+    sim = G.Simulation(s, path=G.PATH_MIXED, neimax=20)
+    with pytest.raises(G.NdgError) as ei:
+        sim.build_lists(0.5)
+    assert ei.value.code == G.ERR_NEIMAX
+    sim.close()
+
+
+def test_particle_outside_box_error():
+    """A displacement larger than the box maps to 'Particle outside of box' (code 2) with
+    the reference trigger (forced from the host: huge velocities, kernels exit normally)."""
+    s = W.cu_morse(7, temp_k=0.0)
+    s.vel = np.full_like(s.pos, 40.0)
+    sim = G.Simulation(s, path=G.PATH_FP64)
+    with pytest.raises(G.NdgError) as ei:
+        sim.run(G.run_params(3, 1.0, skin=0.5, trigger=G.TRIGGER_COMPAT))
+    assert ei.value.code == G.ERR_OUTSIDE
+    sim.close()
+
+
+def test_skin_trigger_rebuilds_match_oracle(oracle):
+    """SKIN trigger: the device rebuilds at the same steps as the oracle (FP64 path) and
+    the forces after the run equal a fresh-list evaluation."""
+    s = W.cu_morse(7, temp_k=1500.0, seed=9)
+    sim = G.Simulation(s, path=G.PATH_FP64)
+    recs, nup, _ = sim.run(G.run_params(300, 2 * W.DT_1FS, out_steps=100, skin=0.3,
+                                        trigger=G.TRIGGER_SKIN))
+    st_ref, thermo_ref, err = oracle.run(s, 300, 2 * W.DT_1FS, skin=0.3, out_steps=100,
+                                         trigger=oracle.TRIGGER_SKIN)
+    assert err == 0 and nup == st_ref["nupdates"] and nup > 0
+    st = sim.state()
+    np.testing.assert_allclose(st["pos"], st_ref["pos"], rtol=0, atol=1e-9)
+    np.testing.assert_allclose(st["force"], st_ref["force"], rtol=1e-7, atol=1e-9)
+    sim.close()
+
+
+def test_small_box_brute_force_list(oracle):
+    """MCELL < 3 (the reference would double count, App. quirk): the device uses the
+    minimum-image O(N^2) walk; neighbour sets equal the brute-force oracle list."""
+    s = W.cu_morse(5, temp_k=300.0, seed=5, jitter_a=0.05)   # 500 atoms, box 18.1 A
+    _, vnl, _ = oracle.half_list(s, 0.5, brute=True)
+    ref = oracle.neighbour_sets_from_half(vnl)
+    for path in (G.PATH_FP64, G.PATH_MIXED):
+        sim = G.Simulation(s, path=path)
+        sim.build_lists(0.5)
+        counts, nbr = sim.neighbours()
+        for i in range(s.np):
+            np.testing.assert_array_equal(nbr[i, :counts[i]], ref[i])
+        sim.close()
