@@ -64,6 +64,45 @@ def workload_systems(name):
     raise SystemExit(f"unknown workload {name}")
 
 
+def serve_bench(args):
+    """Serving workload: a bounded sample of the job stream (sizes and step counts scaled
+    down so the run takes about a minute), Poisson arrivals at a rate set for ~80% of the
+    given capacity, continuous batching with state resident on the device."""
+    from ndyn_omp_amd import serve as S
+    probe = W.serving_jobs(args.jobs, seed=12349, size_scale=args.size_scale,
+                           step_scale=args.step_scale, rate_hz=1.0)
+    work = np.mean([s.np * n for _, s, n, _, _ in probe])
+    rate = 0.8 * args.capacity / work
+    stream = W.serving_jobs(args.jobs, seed=12349, size_scale=args.size_scale,
+                            step_scale=args.step_scale, rate_hz=rate)
+    jobs = S.jobs_from_stream(stream)
+    log(f"bench: serving {len(jobs)} jobs, mean work {work:.3e} atom-steps, rate {rate:.2f}/s")
+    srv = S.Server(tick_steps=args.tick_steps, graph_steps=args.graph_steps)
+    # warm the kernels (first launches compile/load code objects)
+    warm = S.jobs_from_stream(W.serving_jobs(2, seed=1, size_scale=0.01, step_scale=0.001))
+    srv.run(warm, realtime=False)
+    st = srv.run(jobs, realtime=True, log=log)
+    srv.close()
+    desc = (f"serving: {args.jobs} jobs from the config-4 generator (Morse Cu / Buckingham "
+            f"alloy, NVE/NVT 100-1200 K), sizes log-U[1e3,2e5]x{args.size_scale}, steps "
+            f"log-U[1e3,5e4]x{args.step_scale}, Poisson arrivals {rate:.2f}/s (~80% of "
+            f"{args.capacity:.2e} atom-steps/s), tick {args.tick_steps} steps")
+    out = {
+        "metric": METRIC, "value": round(st["atom_steps_per_s"], 1), "unit": "atom-steps/s",
+        "n_gpus": 1, "steps": args.jobs, "warmup": 2, "ms_per_step": None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 pair math / f64 integration",
+        "data": "synthetic (seeded job stream)",
+        "config": {"workload": f"c4: {desc}", "jobs": st["jobs"],
+                   "p50_latency_s": round(st["p50_latency_s"], 4),
+                   "p99_latency_s": round(st["p99_latency_s"], 4),
+                   "wall_s": round(st["wall_s"], 3), "busy_s": round(st["busy_s"], 3),
+                   "atom_steps_per_busy_s": round(st["atom_steps_per_busy_s"], 1)},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def make_rp(cfg, nsteps):
     return G.run_params(nsteps, cfg["dt"], out_steps=nsteps, skin=cfg["skin"],
                         trigger=G.TRIGGER_SKIN, isokinetic=cfg.get("iso_ke"),
@@ -81,6 +120,9 @@ def run_engine(systems, cfg, path, warmup, steps, timing_steps, graph_steps=0, t
     t1 = time.perf_counter()
     res = dict(atom_steps_per_s=atoms * steps / (t1 - t0), ms_per_step=1e3 * (t1 - t0) / steps,
                atoms=atoms)
+    cnt = [eng.counters(sl) for sl in slots]
+    res["rebuilds_per_step"] = sum(c["nupdates"] for c in cnt) / float(warmup + steps)
+    res["rebuilt_atoms_per_step"] = sum(c["nupdates"] * s.np for c, s in zip(cnt, systems)) / float(warmup + steps)
     if timing_steps:
         eng.timing(True)
         eng.kernel_time(reset=True)
@@ -161,12 +203,20 @@ def main():
     ap.add_argument("--cpu-budget-s", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--jobs", type=int, default=200)
+    ap.add_argument("--size-scale", type=float, default=0.25)
+    ap.add_argument("--step-scale", type=float, default=0.02)
+    ap.add_argument("--capacity", type=float, default=7e8)
+    ap.add_argument("--tick-steps", type=int, default=50)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is "
                                    "designed (DESIGN) but not built, --gpus must be 1"}))
         sys.exit(2)
     G.load_library()
+    if args.workload == "c4":
+        serve_bench(args)
+        return
     systems, desc, cfg = workload_systems(args.workload)
     atoms = sum(s.np for s in systems)
     log(f"bench: workload {args.workload}: {len(systems)} systems, {atoms} atoms")
@@ -174,7 +224,9 @@ def main():
     eng, slots, mixed = run_engine(systems, cfg, G.PATH_MIXED, args.warmup, args.steps,
                                    args.timing_steps, args.graph_steps, args.tpa)
     log(f"bench: mixed {mixed['atom_steps_per_s']:.4e} atom-steps/s, "
-        f"{mixed['ms_per_step']:.3f} ms/step, force {mixed.get('force_ms', 0):.4f} ms")
+        f"{mixed['ms_per_step']:.3f} ms/step, force {mixed.get('force_ms', 0):.4f} ms, "
+        f"rebuilds/step {mixed['rebuilds_per_step']:.2f} "
+        f"({mixed['rebuilt_atoms_per_step']:.0f} atoms/step)")
     ferr = force_error_mixed_vs_fp64(eng, slots, systems)
     log(f"bench: mixed vs fp64 rms rel force err {ferr:.3e}")
     eng.close()
@@ -221,6 +273,8 @@ def main():
                 mixed["atom_steps_per_s"] / plain["atom_steps_per_s"], 3),
             "force_rms_rel_err_vs_fp64": ferr,
             "graph_steps": args.graph_steps,
+            "list_rebuilds_per_step": round(mixed["rebuilds_per_step"], 3),
+            "rebuilt_atoms_per_step": round(mixed["rebuilt_atoms_per_step"], 1),
         },
         "roofline": roof,
         "cpu_baseline": cpu,

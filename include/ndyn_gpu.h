@@ -40,7 +40,8 @@ enum {
 /* precision / path selection (ndg_create flags, ndg_engine_create precision) */
 enum {
   NDG_PATH_FP64 = 0,   /* plain path: FP64 positions, tables, accumulation; one system per launch */
-  NDG_PATH_MIXED = 1   /* FP32 pair math on cell-anchored FP32 coordinates, FP64 integration */
+  NDG_PATH_MIXED = 1   /* FP64 cell-anchored pair geometry (reference rk), FP32 tables and
+                          interpolation, FP64 sums and integration; batched launches */
 };
 
 /* list-rebuild trigger (ndg_run_params.trigger) */
@@ -192,6 +193,8 @@ int ndg_engine_config(ndg_engine_t *e, int tpa, int graph_steps);
 int ndg_engine_kernel_time(ndg_engine_t *e, int reset, double *avg_ms, long long *count,
                            double *pairs_per_launch, double *atoms_per_launch);
 int ndg_engine_timing(ndg_engine_t *e, int enable);
+/* Per-slot counters: list rebuilds this Run, last completed step, mean list length. */
+int ndg_engine_counters(ndg_engine_t *e, int slot, int *nupdates, int *step, double *mean_nbr);
 /* Mean full-list length over admitted systems at their last build. */
 int ndg_engine_list_stats(ndg_engine_t *e, double *mean_nbr, long long *atoms);
 

@@ -28,7 +28,7 @@ void launch_apply_pending(const EngineDev &E, int r0, int nr, hipStream_t s);
 void launch_rebuild(const EngineDev &E, int t0, int nt, int r0, int nr, bool any_large,
                     hipStream_t s);
 enum : int { RED_FORCE = 1, RED_KICK = 2, RED_FLIP = 4, RED_STEP = 8, RED_CHECK = 16 };
-constexpr int kSmallSortHost = 65536;
+constexpr int kSmallSortHost = 8192;
 }  // namespace ndg
 
 using namespace ndg;
@@ -519,16 +519,16 @@ static int attach_tables(ndg_engine *e, Slot &s, const HostState &h) {
     tb.P64 = (double *)(b + align_up(n * 8, 256));
     tb.F2 = (float2 *)(b + 2 * align_up(n * 8, 256));
     tb.P2 = (float2 *)(b + 3 * align_up(n * 8, 256));
-    // FP32 interpolation tables {T_k, T_k+1 - T_k} (difference formed in FP64)
+    // FP32 interpolation tables {T_k, T_k+1}: one 8-byte load per lookup; the kernel forms
+    // T_k+1 - T_k in FP32 exactly as from the LDS copy, so both give identical forces
     std::vector<float2> f2(n), p2(n);
     const int tab = h.ptsize + 1;
     for (int p = 0; p < h.npairs; ++p)
       for (int k = 0; k < tab; ++k) {
         const size_t q = (size_t)p * tab + k;
-        const double dF = k + 1 < tab ? h.F[q + 1] - h.F[q] : 0.0;
-        const double dP = k + 1 < tab ? h.P[q + 1] - h.P[q] : 0.0;
-        f2[q] = make_float2((float)h.F[q], (float)dF);
-        p2[q] = make_float2((float)h.P[q], (float)dP);
+        const size_t q1 = k + 1 < tab ? q + 1 : q;
+        f2[q] = make_float2((float)h.F[q], (float)h.F[q1]);
+        p2[q] = make_float2((float)h.P[q], (float)h.P[q1]);
       }
     HIPCHK(hipMemcpyAsync(tb.F64, h.F.data(), n * 8, hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(tb.P64, h.P.data(), n * 8, hipMemcpyHostToDevice, e->stream));
@@ -1344,6 +1344,19 @@ extern "C" int ndg_engine_kernel_time(ndg_engine_t *e, int reset, double *avg_ms
     e->t_ms = 0.0;
     e->t_count = 0;
   }
+  return NDG_OK;
+}
+
+extern "C" int ndg_engine_counters(ndg_engine_t *e, int slot, int *nupdates, int *step,
+                                   double *mean_nbr) {
+  if (!e || slot < 0 || slot >= (int)e->slots.size() || !e->slots[slot].used)
+    return fail(NDG_ERR_ARG, "bad slot");
+  RepState S;
+  int rc = get_state_dev(e, slot, &S);
+  if (rc) return rc;
+  if (nupdates) *nupdates = S.nupdates;
+  if (step) *step = S.step;
+  if (mean_nbr) *mean_nbr = e->slots[slot].np ? (double)S.nbr_sum / e->slots[slot].np : 0.0;
   return NDG_OK;
 }
 

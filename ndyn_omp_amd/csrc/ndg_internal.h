@@ -55,7 +55,7 @@ struct RepParams {
   const float *vdr2;        // [npairs] real(1/PTdr2) (src/nd.f90:254)
   const double *tabF64;     // [npairs*(ptsize+1)]
   const double *tabP64;
-  const float2 *tabF2;      // [npairs*(ptsize+1)] {F_k, F_k+1 - F_k} (mixed path)
+  const float2 *tabF2;      // [npairs*(ptsize+1)] {F_k, F_k+1} (mixed path)
   const float2 *tabP2;      // same for the potential
   int ltab[16];             // offset of each pair's FP32 F table in the engine's LDS set, -1: none
   // per-atom state (sorted order)

@@ -83,6 +83,13 @@ __device__ inline void block_top2(double &t1, double &t2, double *sh) {
   __syncthreads();
 }
 
+// Orders one wave's LDS writes before its later LDS reads (no block barrier).
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ inline int active_flags(const EngineDev &E, int rep, int xflags) {
   if (xflags) return xflags;
   return step_flags(E.reps[rep], E.st[rep]);
@@ -215,8 +222,8 @@ __global__ __launch_bounds__(kTile) void k_kick_mixed(EngineDev E, int t0, int n
 // the single-precision table index rk are the reference's own; table values, the linear
 // interpolation and 1/d2 in FP32; per-atom force, energy and virial sums in FP64 (full
 // rate on CDNA).  The distinct FP32 force tables of all replicas are staged in LDS once
-// per persistent workgroup (two-dword LDS read per pair); otherwise {T_k, T_k+1 - T_k}
-// pairs are read from global memory.  Each lane walks U list entries per iteration with
+// per persistent workgroup (two-dword LDS read per pair); otherwise {T_k, T_k+1} pairs
+// are read from global memory (same arithmetic, same result).  Each lane walks U list entries per iteration with
 // every load issued first and the cutoff as a select.  A workgroup of 512 (1024 for
 // TPA 8) threads runs 512/(128 TPA) tiles at a time; TPA lanes share one atom and split
 // its list; lane sums combine in a fixed order.
@@ -277,7 +284,7 @@ __device__ inline void force_atom(const RepParams &R, int i, int sub, const doub
         fv = fmaf(w, F1 - F0, F0);
       } else {
         const float2 tf = TF[pid * tab + k - 1];
-        fv = fmaf(w, tf.y, tf.x);
+        fv = fmaf(w, tf.y - tf.x, tf.x);
       }
       const double df = ok ? (double)(fv * __builtin_amdgcn_rcpf((float)d2)) : 0.0;
       fx = fma(dx, df, fx);
@@ -285,7 +292,7 @@ __device__ inline void force_atom(const RepParams &R, int i, int sub, const doub
       fz = fma(dz, df, fz);
       if (FULL) {
         const float2 tp = TP[pid * tab + k - 1];
-        const double pv = ok ? (double)fmaf(w, tp.y, tp.x) : 0.0;
+        const double pv = ok ? (double)fmaf(w, tp.y - tp.x, tp.x) : 0.0;
         const double hdf = 0.5 * df;
         acc[0] += 0.5 * pv;
         acc[1] += dx * dx * hdf;
@@ -802,9 +809,13 @@ __device__ inline int cell_of(const RepParams &R, const RepState &S, int i) {
 }
 
 // Final order of one cell's slots [lo, hi): ascending caller index.  Cells of up to 64
-// atoms are sorted inside one wave (bitonic network over shuffles); larger cells use a
-// per-slot rank count.  The atoms are then copied to their final slots in tmp.
-__device__ inline void sort_cell_wave(const RepParams &R, int lo, int hi, int parity) {
+// atoms are sorted inside one wave (bitonic network over shuffles); cells up to
+// kCellLds atoms by rank counts over a per-wave LDS copy; larger cells by rank counts over
+// global memory.  The atoms are then copied to their final slots in tmp.
+constexpr int kCellLds = 256;
+
+__device__ inline void sort_cell_wave(const RepParams &R, int lo, int hi, int parity, int *wsrc,
+                                      int *wkey) {
   const int lane = threadIdx.x & 63;
   const int cnt = hi - lo;
   if (cnt <= 64) {
@@ -821,22 +832,37 @@ __device__ inline void sort_cell_wave(const RepParams &R, int lo, int hi, int pa
         }
       }
     if (lane < cnt) permute_atom(R, u, lo + lane, parity);
+  } else if (cnt <= kCellLds) {
+    for (int v = lane; v < cnt; v += 64) {
+      const int u = R.slot_src[lo + v];
+      wsrc[v] = u;
+      wkey[v] = R.oid[u];
+    }
+    wave_sync();
+    for (int v = lane; v < cnt; v += 64) {
+      const int me = wkey[v];
+      int rank = 0;
+      for (int w = 0; w < cnt; ++w) rank += wkey[w] < me;
+      permute_atom(R, wsrc[v], lo + rank, parity);
+    }
+    wave_sync();
   } else {
-    for (int s = lo + lane; s < hi; s += 64) {
-      const int i = R.slot_src[s];
+    for (int v = lo + lane; v < hi; v += 64) {
+      const int i = R.slot_src[v];
       const int me = R.oid[i];
       int rank = 0;
-      for (int v = lo; v < hi; ++v) rank += R.oid[R.slot_src[v]] < me;
+      for (int w = lo; w < hi; ++w) rank += R.oid[R.slot_src[w]] < me;
       permute_atom(R, i, lo + rank, parity);
     }
   }
 }
 
 // Whole sort of one small replica inside one workgroup (replicas up to kSmallSort atoms).
-constexpr int kSmallSort = 65536;
+constexpr int kSmallSort = 8192;
 
 __global__ __launch_bounds__(1024) void k_rb_small(EngineDev E, int r0, int nr) {
   __shared__ int sh_scan[1024];
+  __shared__ int sh_sort[16][2 * kCellLds];
   for (int r = r0 + blockIdx.x; r < r0 + nr; r += gridDim.x) {
     const RepParams &R = E.reps[r];
     RepState &S = E.st[r];
@@ -874,10 +900,15 @@ __global__ __launch_bounds__(1024) void k_rb_small(EngineDev E, int r0, int nr) 
     }
     __syncthreads();
     const int parity = S.parity;
-    const int nwave = nb >> 6, wid = tid >> 6;
-    for (int c = wid; c < ncell; c += nwave) {
-      const int lo = R.cell_start[c], hi = R.cell_start[c + 1];
-      if (hi > lo) sort_cell_wave(R, lo, hi, parity);
+    if (S.brute) {
+      // one cell holding every atom: the final slot of an atom is its caller index
+      for (int i = tid; i < np; i += nb) permute_atom(R, i, R.oid[i], parity);
+    } else {
+      const int nwave = nb >> 6, wid = tid >> 6;
+      for (int c = wid; c < ncell; c += nwave) {
+        const int lo = R.cell_start[c], hi = R.cell_start[c + 1];
+        if (hi > lo) sort_cell_wave(R, lo, hi, parity, sh_sort[wid], sh_sort[wid] + kCellLds);
+      }
     }
     __syncthreads();
     for (int p = tid; p < np; p += nb) place_atom(R, S, p);
@@ -959,6 +990,7 @@ __global__ __launch_bounds__(kTile) void k_rb_scatter(EngineDev E, int t0, int n
 
 // Each tile sorts the cells whose first slot lies in it (one wave per cell).
 __global__ __launch_bounds__(kTile) void k_rb_rank(EngineDev E, int t0, int nt) {
+  __shared__ int sh_sort[kTile / 64][2 * kCellLds];
   int tb, te, ts;
   tile_range(t0, nt, tb, te, ts);
   for (int t = tb; t < te; t += ts) {
@@ -966,13 +998,18 @@ __global__ __launch_bounds__(kTile) void k_rb_rank(EngineDev E, int t0, int nt) 
     const RepParams &R = E.reps[T.rep];
     const RepState &S = E.st[T.rep];
     if (!S.rebuild || S.err || R.np <= kSmallSort) continue;
+    if (S.brute) {
+      const int i = T.atom0 + threadIdx.x;
+      if (i < R.np) permute_atom(R, i, R.oid[i], S.parity);
+      continue;
+    }
     const int ncell = S.mcell[0] * S.mcell[1] * S.mcell[2];
     int c = R.key[R.slot_src[T.atom0]];
     if (R.cell_start[c] < T.atom0) ++c;
     const int wid = threadIdx.x >> 6, nwave = blockDim.x >> 6;
     for (c += wid; c < ncell && R.cell_start[c] < T.atom0 + kTile; c += nwave) {
       const int lo = R.cell_start[c], hi = R.cell_start[c + 1];
-      if (hi > lo) sort_cell_wave(R, lo, hi, S.parity);
+      if (hi > lo) sort_cell_wave(R, lo, hi, S.parity, sh_sort[wid], sh_sort[wid] + kCellLds);
     }
   }
 }
@@ -993,179 +1030,168 @@ __global__ __launch_bounds__(kTile) void k_rb_place(EngineDev E, int t0, int nt)
 
 // Full Verlet list (both directions) of each atom: the reference's pair test
 // (src/nd.f90:1358-1377,1404-1420) over the 27 surrounding cells (or all atoms in brute
-// mode).  A workgroup takes the cells whose first slot lies in its tile, stages each
-// cell's candidates through LDS in canonical order (stencil cell (dz,dy,dx)
-// lexicographic, then slot), and each wave walks one home atom at a time, 64 candidates
-// per step, appending accepted ones in order (ballot prefix).  Entries carry the
-// cell/image offset code used by the mixed force kernel.
-constexpr int kStage = 1024;
-constexpr int kListThreads = 256;
+// mode).  Work unit: one wave per kListAtoms consecutive home atoms (so even a small
+// replica spreads over many waves).  The wave walks the candidates in canonical order
+// (stencil cell (dz,dy,dx) lexicographic, then slot), 64 per step with coalesced loads,
+// and appends accepted ones in order (ballot prefix).  Entries carry the cell/image
+// offset code used by the mixed force kernel.
+constexpr int kListAtoms = 4;
+constexpr int kListTasks = kTile / kListAtoms;   // wave tasks per tile
 
-struct StageBuf {
-  double x[kStage], y[kStage], z[kStage];
-  int t[kStage], u[kStage], q[kStage];
-  int ax[kStage], ay[kStage], az[kStage];
-};
-
-// home atoms [h0, h1); candidate ranges rlo[q], prefix rpref[q] (shared memory)
-__device__ void build_group(const RepParams &R, RepState &S, StageBuf &sb, int *hn,
-                            const int *rlo, const int *rpref, int nrange, int same_q,
-                            int h0, int h1, long long &mine) {
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, nw = blockDim.x >> 6;
-  const int nh = h1 - h0;
-  for (int h = tid; h < nh; h += blockDim.x) hn[h] = 0;
-  const int mx = S.mcell[0], my = S.mcell[1], mz = S.mcell[2];
-  const double b0 = S.box[0], b1 = S.box[1], b2 = S.box[2];
-  const int nt_ = R.ntypes, tpa = R.tpa;
-  const int total = rpref[nrange];
-  for (int base = 0; base < total; base += kStage) {
-    const int cnt = min(kStage, total - base);
-    __syncthreads();
-    for (int g = tid; g < cnt; g += blockDim.x) {
-      const int gg = base + g;
-      int q = 0;
-      while (rpref[q + 1] <= gg) ++q;
-      const int u = rlo[q] + (gg - rpref[q]);
-      sb.x[g] = R.px[u];
-      sb.y[g] = R.py[u];
-      sb.z[g] = R.pz[u];
-      sb.t[g] = R.typ[u];
-      sb.u[g] = u;
-      sb.q[g] = q;
-      const int4 A = R.anchor[u];
-      sb.ax[g] = A.x;
-      sb.ay[g] = A.y;
-      sb.az[g] = A.z;
-    }
-    __syncthreads();
-    for (int h = wid; h < nh; h += nw) {
-      const int i = h0 + h;
-      const double xi = R.px[i], yi = R.py[i], zi = R.pz[i];
-      const int ti = R.typ[i];
-      const int4 Ai = R.anchor[i];
-      unsigned *L = R.list + ((size_t)(i / kTile) * R.list_chunks) * (kTile * tpa) + (i % kTile) * tpa;
-      int n = hn[h];
-      bool over = n < 0;
-      for (int s0 = 0; s0 < cnt && !over; s0 += 64) {
-        const int s2 = s0 + lane;
-        bool acc = false;
-        unsigned ent = 0;
-        bool bad = false;
-        if (s2 < cnt) {
-          const int u = sb.u[s2];
-          const int tj = sb.t[s2];
-          const int pij = R.pairmat[(ti - 1) + (tj - 1) * nt_];
-          const int pji = R.pairmat[(tj - 1) + (ti - 1) * nt_];
-          const int pm = pij > pji ? pij : pji;
-          if (u != i && pm > 0) {
-            const double dpx = xi - sb.x[s2], dpy = yi - sb.y[s2], dpz = zi - sb.z[s2];
-            const double nx = round(dpx), ny = round(dpy), nz = round(dpz);
-            const double r0 = b0 * (dpx - nx), r1 = b1 * (dpy - ny), r2 = b2 * (dpz - nz);
-            double d2 = r0 * r0;
-            d2 = d2 + r1 * r1;
-            d2 = d2 + r2 * r2;
-            const double rcs = R.skin + R.rc_pair[pm - 1];
-            const bool excl = sb.q[s2] == same_q && d2 < R.ptrmin;
-            if (!excl && !(d2 > rcs * rcs)) {
-              acc = true;
-              const int ox = (Ai.x - sb.ax[s2]) - (int)nx * mx;
-              const int oy = (Ai.y - sb.ay[s2]) - (int)ny * my;
-              const int oz = (Ai.z - sb.az[s2]) - (int)nz * mz;
-              // offsets beyond one cell cannot occur for mcell >= 3 or brute grids
-              bad = ox < -1 || ox > 1 || oy < -1 || oy > 1 || oz < -1 || oz > 1;
-              const unsigned code = (unsigned)((ox + 1) | ((oy + 1) << 2) | ((oz + 1) << 4));
-              ent = (unsigned)u | (code << kCodeShift);
-            }
-          }
-        }
-        const unsigned long long bal = __ballot(acc);
-        const unsigned long long badb = __ballot(bad);
-        const int nacc = __popcll(bal);
-        if (badb || n + nacc > R.neimax) {
-          over = true;
-          break;
-        }
-        if (acc) {
-          const int k = n + __popcll(bal & ((1ull << lane) - 1ull));
-          L[(size_t)(k / tpa) * (kTile * tpa) + k % tpa] = ent;
-        }
-        n += nacc;
-      }
-      if (lane == 0) hn[h] = over ? -1 : n;
-    }
-  }
-  __syncthreads();
-  for (int h = tid; h < nh; h += blockDim.x) {
-    const int n = hn[h];
-    if (n < 0) {
-      atomicMax(&S.err, NDG_ERR_NEIMAX);
-      R.nnb[h0 + h] = 0;
-    } else {
-      R.nnb[h0 + h] = n;
-      mine += n;
-    }
-  }
-  __syncthreads();
-}
-
-__global__ __launch_bounds__(kListThreads) void k_rb_list(EngineDev E, int t0, int nt) {
-  __shared__ StageBuf sb;
-  __shared__ int s_hn[kTile];
+__global__ __launch_bounds__(64) void k_rb_list(EngineDev E, int t0, int nt) {
   __shared__ int s_lo[27], s_cnt[27], s_pref[28];
-  __shared__ unsigned long long s_sum;
-  int tb, te, ts;
-  tile_range(t0, nt, tb, te, ts);
-  for (int t = tb; t < te; t += ts) {
-    const Tile T = E.tiles[t];
+  __shared__ double s_cut2[64];
+  const int lane = threadIdx.x;
+  // round-robin tasks (not XCD-contiguous): a rebuilding replica spreads over the chip
+  const int te = nt * kListTasks, ts = gridDim.x;
+  int cur_rep = -1;
+  for (int task = blockIdx.x; task < te; task += ts) {
+    const Tile T = E.tiles[t0 + task / kListTasks];
     const RepParams &R = E.reps[T.rep];
     RepState &S = E.st[T.rep];
     if (!S.rebuild || S.err) continue;
-    const int tid = threadIdx.x;
-    long long mine = 0;
-    __syncthreads();
-    if (tid == 0) s_sum = 0;
-    if (S.brute) {
-      if (tid == 0) {
-        s_lo[0] = 0;
-        s_pref[0] = 0;
-        s_pref[1] = R.np;
-      }
-      __syncthreads();
-      build_group(R, S, sb, s_hn, s_lo, s_pref, 1, -1, T.atom0, min(T.atom0 + kTile, R.np), mine);
-    } else {
-      const int mx = S.mcell[0], my = S.mcell[1], mz = S.mcell[2];
-      const int ncell = mx * my * mz;
-      int c = R.anchor[T.atom0].w;
-      if (R.cell_start[c] < T.atom0) ++c;
-      for (; c < ncell && R.cell_start[c] < T.atom0 + kTile; ++c) {
-        const int lo = R.cell_start[c], hi = R.cell_start[c + 1];
-        if (hi <= lo) continue;
-        __syncthreads();
-        if (tid < 27) {
-          const int cx = c % mx, cy = (c / mx) % my, cz = c / (mx * my);
-          const int dz = tid / 9 - 1, dy = (tid / 3) % 3 - 1, dx = tid % 3 - 1;
-          int ux = cx + dx, uy = cy + dy, uz = cz + dz;
-          ux += ux < 0 ? mx : (ux >= mx ? -mx : 0);
-          uy += uy < 0 ? my : (uy >= my ? -my : 0);
-          uz += uz < 0 ? mz : (uz >= mz ? -mz : 0);
-          const int cu = ux + uy * mx + uz * mx * my;
-          s_lo[tid] = R.cell_start[cu];
-          s_cnt[tid] = R.cell_start[cu + 1] - R.cell_start[cu];
+    const int h0 = T.atom0 + (task % kListTasks) * kListAtoms;
+    if (h0 >= R.np) continue;
+    const int h1 = min(h0 + kListAtoms, R.np);
+    if (T.rep != cur_rep) {
+      // the reference's per-pair list cutoff (src/nd.f90:1361,1375-1377), per type pair
+      const int ti = lane / 8 + 1, tj = lane % 8 + 1;
+      double c2 = -1.0;
+      if (ti <= R.ntypes && tj <= R.ntypes) {
+        const int pij = R.pairmat[(ti - 1) + (tj - 1) * R.ntypes];
+        const int pji = R.pairmat[(tj - 1) + (ti - 1) * R.ntypes];
+        const int pm = pij > pji ? pij : pji;
+        if (pm > 0) {
+          const double rcs = R.skin + R.rc_pair[pm - 1];
+          c2 = rcs * rcs;
         }
-        __syncthreads();
-        if (tid == 0) {
-          s_pref[0] = 0;
-          for (int q = 0; q < 27; ++q) s_pref[q + 1] = s_pref[q] + s_cnt[q];
-        }
-        __syncthreads();
-        for (int h0 = lo; h0 < hi; h0 += kTile)
-          build_group(R, S, sb, s_hn, s_lo, s_pref, 27, 13, h0, min(h0 + kTile, hi), mine);
       }
+      s_cut2[lane] = c2;
+      cur_rep = T.rep;
     }
-    __syncthreads();
-    atomicAdd(&s_sum, (unsigned long long)mine);
-    __syncthreads();
-    if (tid == 0 && s_sum) atomicAdd((unsigned long long *)&S.nbr_sum, s_sum);
+    const int mx = S.mcell[0], my = S.mcell[1], mz = S.mcell[2];
+    const double b0 = S.box[0], b1 = S.box[1], b2 = S.box[2];
+    const int tpa = R.tpa;
+    const bool brute = S.brute != 0;
+    long long mine = 0;
+    int cell_loaded = -1;
+    for (int i = h0; i < h1; ++i) {
+      const double xi = R.px[i], yi = R.py[i], zi = R.pz[i];
+      const int ti = R.typ[i];
+      const int4 Ai = R.anchor[i];
+      int nrange;
+      if (brute) {
+        nrange = 1;
+        if (cell_loaded != 0 && lane == 0) {
+          s_lo[0] = 0;
+          s_cnt[0] = R.np;
+          s_pref[0] = 0;
+          s_pref[1] = R.np;
+        }
+        cell_loaded = 0;
+      } else {
+        nrange = 27;
+        if (Ai.w != cell_loaded) {
+          if (lane < 27) {
+            const int c = Ai.w;
+            const int cx = c % mx, cy = (c / mx) % my, cz = c / (mx * my);
+            const int dz = lane / 9 - 1, dy = (lane / 3) % 3 - 1, dx = lane % 3 - 1;
+            int ux = cx + dx, uy = cy + dy, uz = cz + dz;
+            ux += ux < 0 ? mx : (ux >= mx ? -mx : 0);
+            uy += uy < 0 ? my : (uy >= my ? -my : 0);
+            uz += uz < 0 ? mz : (uz >= mz ? -mz : 0);
+            const int cu = ux + uy * mx + uz * mx * my;
+            s_lo[lane] = R.cell_start[cu];
+            s_cnt[lane] = R.cell_start[cu + 1] - R.cell_start[cu];
+          }
+          __syncthreads();
+          if (lane == 0) {
+            s_pref[0] = 0;
+            for (int q = 0; q < 27; ++q) s_pref[q + 1] = s_pref[q] + s_cnt[q];
+          }
+          cell_loaded = Ai.w;
+        }
+      }
+      __syncthreads();  // one wave: orders the LDS writes above before the reads below
+      unsigned *L = R.list + ((size_t)(i / kTile) * R.list_chunks) * (kTile * tpa) + (i % kTile) * tpa;
+      int n = 0;
+      bool over = false;
+      // the 27 cells (or all atoms) as one candidate stream in canonical order; two
+      // batches of 64: every load issued before either is tested
+      const int total = s_pref[nrange];
+      for (int g0 = 0; g0 < total && !over; g0 += 128) {
+        int u[2], tj[2], qq[2];
+        double qx[2], qy[2], qz[2];
+        bool in[2];
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          const int g = g0 + 64 * b + lane;
+          in[b] = g < total;
+          int q = 0;
+          if (!brute) {
+            // last q with s_pref[q] <= g (binary search over 28 prefix entries)
+            int lo_ = 0, hi_ = 27;
+            while (lo_ < hi_) {
+              const int mid = (lo_ + hi_ + 1) >> 1;
+              if (s_pref[mid] <= g) lo_ = mid; else hi_ = mid - 1;
+            }
+            q = lo_ > 26 ? 26 : lo_;
+          }
+          qq[b] = q;
+          u[b] = in[b] ? s_lo[q] + (g - s_pref[q]) : i;
+          qx[b] = R.px[u[b]];
+          qy[b] = R.py[u[b]];
+          qz[b] = R.pz[u[b]];
+          tj[b] = R.typ[u[b]];
+        }
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          bool acc = false, bad = false;
+          unsigned ent = 0;
+          if (in[b] && u[b] != i) {
+            const double c2 = s_cut2[(ti - 1) * 8 + tj[b] - 1];
+            if (c2 >= 0.0) {
+              const double dpx = xi - qx[b], dpy = yi - qy[b], dpz = zi - qz[b];
+              const double nx = round(dpx), ny = round(dpy), nz = round(dpz);
+              const double r0 = b0 * (dpx - nx), r1 = b1 * (dpy - ny), r2 = b2 * (dpz - nz);
+              double d2 = r0 * r0;
+              d2 = d2 + r1 * r1;
+              d2 = d2 + r2 * r2;
+              const bool excl = !brute && qq[b] == 13 && d2 < R.ptrmin;
+              if (!excl && !(d2 > c2)) {
+                acc = true;
+                const int4 Au = R.anchor[u[b]];
+                const int ox = (Ai.x - Au.x) - (int)nx * mx;
+                const int oy = (Ai.y - Au.y) - (int)ny * my;
+                const int oz = (Ai.z - Au.z) - (int)nz * mz;
+                // offsets beyond one cell cannot occur for mcell >= 3 or brute grids
+                bad = ox < -1 || ox > 1 || oy < -1 || oy > 1 || oz < -1 || oz > 1;
+                const unsigned code = (unsigned)((ox + 1) | ((oy + 1) << 2) | ((oz + 1) << 4));
+                ent = (unsigned)u[b] | (code << kCodeShift);
+              }
+            }
+          }
+          const unsigned long long bal = __ballot(acc);
+          const int nacc = __popcll(bal);
+          if (__ballot(bad) || n + nacc > R.neimax) {
+            over = true;
+            break;
+          }
+          if (acc) {
+            const int k = n + __popcll(bal & ((1ull << lane) - 1ull));
+            L[(size_t)(k / tpa) * (kTile * tpa) + k % tpa] = ent;
+          }
+          n += nacc;
+        }
+      }
+      if (lane == 0) {
+        if (over) atomicMax(&S.err, NDG_ERR_NEIMAX);
+        R.nnb[i] = over ? 0 : n;
+      }
+      mine += over ? 0 : n;
+      __syncthreads();
+    }
+    if (lane == 0 && mine) atomicAdd((unsigned long long *)&S.nbr_sum, (unsigned long long)mine);
   }
 }
 
@@ -1250,7 +1276,7 @@ void launch_rebuild(const EngineDev &E, int t0, int nt, int r0, int nr, bool any
     hipLaunchKernelGGL(k_rb_rank, dim3(g), dim3(kTile), 0, s, E, t0, nt);
     hipLaunchKernelGGL(k_rb_place, dim3(g), dim3(kTile), 0, s, E, t0, nt);
   }
-  hipLaunchKernelGGL(k_rb_list, dim3(grid_for(nt, 4)), dim3(kListThreads), 0, s, E, t0, nt);
+  hipLaunchKernelGGL(k_rb_list, dim3(grid_for(nt * kListTasks, 32)), dim3(64), 0, s, E, t0, nt);
 }
 
 }  // namespace ndg

@@ -103,6 +103,7 @@ def load_library():
         "ndg_engine_kernel_time": [V, C.c_int, P(C.c_double), P(C.c_longlong), P(C.c_double),
                                    P(C.c_double)],
         "ndg_engine_list_stats": [V, P(C.c_double), P(C.c_longlong)],
+        "ndg_engine_counters": [V, C.c_int, P(C.c_int), P(C.c_int), P(C.c_double)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -355,6 +356,12 @@ class BatchEngine:
         _check(load_library().ndg_engine_kernel_time(self.e, int(reset), C.byref(ms), C.byref(cnt),
                                                      C.byref(pairs), C.byref(atoms)))
         return dict(avg_ms=ms.value, count=cnt.value, list_entries=pairs.value, atoms=atoms.value)
+
+    def counters(self, slot):
+        nup, step, mean = C.c_int(0), C.c_int(0), C.c_double(0)
+        _check(load_library().ndg_engine_counters(self.e, int(slot), C.byref(nup), C.byref(step),
+                                                  C.byref(mean)))
+        return dict(nupdates=nup.value, step=step.value, mean_nbr=mean.value)
 
     def list_stats(self):
         m, a = C.c_double(0), C.c_longlong(0)

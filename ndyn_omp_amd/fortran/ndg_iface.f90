@@ -1,0 +1,178 @@
+! ndg_iface.f90 -- Fortran binding of the C ABI in include/ndyn_gpu.h (iso_c_binding).
+!
+! This is the interface a Fortran MD driver (Ndyn's src/nd.f90 'dynamics' or the
+! driver in ndyn_gpu.f90) uses in place of compute0 / integrate1 / integrate2 /
+! buildCellList / CheckListUpdate / mkIsoKE.  Arrays are passed in the reference's own
+! layouts: pos(3,np), vel(3,np), typ(np), PairMat(Ntypes,Ntypes), tables (PTsize+1,NPairs).
+module ndg_iface
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: NDG_OK = 0, NDG_ERR_NEIMAX = 1, NDG_ERR_OUTSIDE = 2, &
+                               NDG_ERR_HIP = 3, NDG_ERR_ARG = 4, NDG_ERR_NOMEM = 5
+  integer(c_int), parameter :: NDG_PATH_FP64 = 0, NDG_PATH_MIXED = 1
+  integer(c_int), parameter :: NDG_TRIGGER_COMPAT = 0, NDG_TRIGGER_SKIN = 1
+  integer(c_int), parameter :: NDG_MORSE = 1, NDG_BUCK = 2, NDG_LJ = 3, NDG_MORSE_PHYS = 4
+
+  type, bind(C) :: ndg_run_params
+    integer(c_int) :: nsteps
+    real(c_double) :: dt
+    integer(c_int) :: out_steps
+    real(c_double) :: skin
+    integer(c_int) :: renei, renei_freq
+    integer(c_int) :: trigger
+    integer(c_int) :: isokinetic
+    real(c_double) :: ke_target, ke_tau
+    integer(c_int) :: ke_freq
+    integer(c_int) :: isobaric
+    real(c_double) :: p_target(3), p_tau
+    integer(c_int) :: p_freq
+    integer(c_int) :: isoenergetic
+    real(c_double) :: e_tau
+    integer(c_int) :: e_freq
+    integer(c_int) :: box_load
+    real(c_double) :: box_vel(3)
+  end type ndg_run_params
+
+  type, bind(C) :: ndg_thermo
+    integer(c_int) :: step
+    integer(c_int) :: pad_
+    real(c_double) :: pe, ke, eerr, pressure, volume
+    real(c_double) :: virial(6)
+    real(c_double) :: box(3)
+  end type ndg_thermo
+
+  interface
+    integer(c_int) function ndg_pair_table(kind, params, ptrmin, ptsize, P, F, rc2, dr2) &
+        bind(C, name='ndg_pair_table')
+      import :: c_int, c_double
+      integer(c_int), value :: kind, ptsize
+      real(c_double), intent(in) :: params(4)
+      real(c_double), value :: ptrmin
+      real(c_double), intent(out) :: P(*), F(*), rc2, dr2
+    end function
+
+    integer(c_int) function ndg_create(h, np, ntypes, npairs, ptsize, neimax, flags) &
+        bind(C, name='ndg_create')
+      import :: c_int, c_ptr
+      type(c_ptr), intent(out) :: h
+      integer(c_int), value :: np, ntypes, npairs, ptsize, neimax, flags
+    end function
+
+    integer(c_int) function ndg_set_tables(h, P, F, rc2, dr2, ptrmin, pairmat, pairparams, &
+        mass) bind(C, name='ndg_set_tables')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: P(*), F(*), rc2(*), dr2(*), pairparams(*), mass(*)
+      real(c_double), value :: ptrmin
+      integer(c_int), intent(in) :: pairmat(*)
+    end function
+
+    integer(c_int) function ndg_set_state(h, pos, vel, acc, typ, box) &
+        bind(C, name='ndg_set_state')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: pos(*), vel(*), acc(*), box(3)
+      integer(c_int), intent(in) :: typ(*)
+    end function
+
+    integer(c_int) function ndg_get_state(h, pos, vel, acc, force, pene, box) &
+        bind(C, name='ndg_get_state')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: pos(*), vel(*), acc(*), force(*), pene(*), box(3)
+    end function
+
+    integer(c_int) function ndg_build_lists(h, skin) bind(C, name='ndg_build_lists')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), value :: skin
+    end function
+
+    integer(c_int) function ndg_integrate1(h, dt) bind(C, name='ndg_integrate1')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), value :: dt
+    end function
+
+    integer(c_int) function ndg_compute(h, cv, cu, pe, virial) bind(C, name='ndg_compute')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: cv, cu
+      real(c_double), intent(out) :: pe, virial(6)
+    end function
+
+    integer(c_int) function ndg_integrate2(h, dt, ck, ke) bind(C, name='ndg_integrate2')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), value :: dt
+      integer(c_int), value :: ck
+      real(c_double), intent(out) :: ke
+    end function
+
+    integer(c_int) function ndg_scale_velocities(h, lambda) bind(C, name='ndg_scale_velocities')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), value :: lambda
+    end function
+
+    integer(c_int) function ndg_check_rebuild(h, skin, trigger, rebuilt) &
+        bind(C, name='ndg_check_rebuild')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), value :: skin
+      integer(c_int), value :: trigger
+      integer(c_int), intent(out) :: rebuilt
+    end function
+
+    integer(c_int) function ndg_run_begin(h, rp, step0) bind(C, name='ndg_run_begin')
+      import :: c_int, c_ptr, ndg_run_params
+      type(c_ptr), value :: h
+      type(ndg_run_params), intent(in) :: rp
+      integer(c_int), value :: step0
+    end function
+
+    integer(c_int) function ndg_run_steps(h, nsteps, rec, max_rec, nrec) &
+        bind(C, name='ndg_run_steps')
+      import :: c_int, c_ptr, ndg_thermo
+      type(c_ptr), value :: h
+      integer(c_int), value :: nsteps, max_rec
+      type(ndg_thermo), intent(out) :: rec(*)
+      integer(c_int), intent(out) :: nrec
+    end function
+
+    integer(c_int) function ndg_run_end(h, nupdates, step) bind(C, name='ndg_run_end')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), intent(out) :: nupdates, step
+    end function
+
+    integer(c_int) function ndg_destroy(h) bind(C, name='ndg_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+
+    type(c_ptr) function ndg_last_error() bind(C, name='ndg_last_error')
+      import :: c_ptr
+    end function
+  end interface
+
+contains
+
+  ! message of the last failing call as a Fortran string
+  function ndg_error_text() result(msg)
+    character(len=256) :: msg
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: cs(:)
+    integer :: k
+    msg = ' '
+    p = ndg_last_error()
+    if (.not. c_associated(p)) return
+    call c_f_pointer(p, cs, [256])
+    do k = 1, 256
+      if (cs(k) == c_null_char) exit
+      msg(k:k) = cs(k)
+    enddo
+  end function ndg_error_text
+
+end module ndg_iface

@@ -1,0 +1,550 @@
+! ndyn_gpu.f90 -- Ndyn deck driver over the MI355X engine (C ABI via iso_c_binding).
+!
+! Reads the same input deck on stdin as the reference (src/read_input.f90:60-206: START,
+! Types, Pairs, Model, BoxSize, NTotal, Seed, Velocity, Skin, NeiMax, ReNeighbor,
+! Isokinetic/Isobaric/Isoenergetic and their Aniso* forms, BoxVel/StopBoxVel, OutFreq,
+! ConfOutFreq, StepNumber, WritePotentialTable, Exec, Print, Run, END), builds the same
+! potential tables (src/potentials.f90:2-56, through ndg_pair_table), and writes the same
+! thermo lines (src/read_input.f90:433-446) and trajectory files out.xyz / out.dump
+! (src/read_input.f90:394-431).  The MD step runs on the GPU ('dynamics',
+! src/nd.f90:60-199, through ndg_run_begin / ndg_run_steps / ndg_run_end).
+! Two directives are added: 'Precision mixed|fp64' selects the engine path and
+! 'ListTrigger compat|skin' the neighbour-list rebuild criterion.
+! Group/Hold/Disp/ChangeType/InitPos and asymmetric or spring pairs (compute1-3) are not
+! part of this engine; they are reported and ignored.
+program ndyn_gpu
+  use iso_c_binding
+  use ndg_iface
+  implicit none
+
+  ! system
+  integer :: np = 1000, ntypes = 0, npairs = 0, ptsize = 1000
+  real(8) :: ptrmin = 0.25d0, box(3) = 10.d0
+  integer, allocatable :: typ(:), pairmat(:,:), pairkind(:)
+  real(8), allocatable :: pos(:,:), vel(:,:), acc(:,:), pene(:), frc(:,:)
+  real(8), allocatable :: mass(:), pairparams(:,:), ptab(:,:), ftab(:,:), rc2(:), dr2(:)
+  character(len=8), allocatable :: typename(:), pairname(:)
+  ! control (defaults of src/read_input.f90:18-41)
+  real(8) :: dt = 0.0001d0, skin = -1.d0
+  integer :: step = 0, step_num = 400, outsteps = 40, confouts = 0, neimax = 0
+  logical :: renei = .true., isokin = .false., isobar = .false., isoen = .false., boxload = .false.
+  integer :: renei_freq = 1, kefreq = 1, pfreq = 1, efreq = 1
+  real(8) :: ketarget = 0.d0, keparm = 0.d0, ptarget(3) = 0.d0, pparm = 0.d0, eparm = 0.d0
+  real(8) :: boxvel(3) = 0.d0
+  character(len=10) :: outtype = 'NA'
+  logical :: potouts = .false.
+  integer :: path = NDG_PATH_MIXED, trigger = NDG_TRIGGER_COMPAT
+  ! engine handle
+  type(c_ptr) :: h = c_null_ptr
+  logical :: tables_dirty = .true., state_dirty = .true., have_model = .false.
+  ! parsing
+  character(len=256) :: line, w(16)
+  integer :: nw, ios, i, j, ln
+  real(8) :: rtmp
+
+  write(6,'(a)') ''
+  write(6,'(a)') '#  N D Y N  (MI355X engine)'
+  write(6,'(a)') ''
+  write(6,'(A)') "#  type 'START' to begin specifying directives"
+  call flush(6)
+
+  do   ! wait for START (src/read_input.f90:48-57)
+    read(5,'(A)',iostat=ios) line
+    if (ios /= 0) call die('#  read_input: ERROR: premature end of input file. No END directive found')
+    call split(line, w, nw)
+    if (nw < 1) cycle
+    if (trim(w(1)) == 'START') exit
+    if (any(trim(w(1)) == [character(len=5) :: 'END', 'DONE', 'EXIT', 'quit', 'abort'])) stop
+  enddo
+
+  ln = 0
+  do
+    read(5,'(A)',iostat=ios) line
+    if (ios /= 0) call die('#  read_input: ERROR: premature end of input file. No END directive found')
+    ln = ln + 1
+    call strip_comment(line)
+    call split(line, w, nw)
+    if (nw < 1) cycle
+    select case (trim(w(1)))
+    case ('help')
+      call help
+    case ('BoxSize')
+      do i = 1, 3
+        read(w(1+i),*) box(i)
+      enddo
+      state_dirty = .true.
+    case ('Model')
+      call read_model(trim(w(2)), merge(trim(w(3)), 'xyz ', nw >= 3))
+    case ('Seed')
+      call set_seed(w(2))
+    case ('NTotal')
+      read(w(2),*) np
+      call random_model
+    case ('Velocity')
+      read(w(2),*) rtmp
+      call velocities(rtmp)
+    case ('ReNeibour', 'ReNeighbor')
+      read(w(2),*) renei
+      if (nw >= 3) read(w(3),*) renei_freq
+    case ('Skin')
+      read(w(2),*) skin
+    case ('NeiMax')
+      read(w(2),*) neimax
+    case ('Types')
+      read(w(2),*) ntypes
+      if (allocated(mass)) deallocate(mass, typename)
+      allocate(mass(ntypes), typename(ntypes))
+      do i = 1, ntypes
+        read(5,'(A)',iostat=ios) line
+        if (ios /= 0) call die('#  read_input: ERROR: premature end of input file. No END directive found')
+        call split(line, w, nw)
+        typename(i) = trim(w(1))
+        read(w(2),*) mass(i)
+      enddo
+      tables_dirty = .true.
+    case ('WritePotentialTable')
+      potouts = .true.
+    case ('Pairs')
+      read(w(2),*) npairs
+      if (nw >= 3) read(w(3),*) ptrmin
+      if (nw >= 4) read(w(4),*) ptsize
+      call read_pairs
+    case ('Isokinetic')
+      isokin = .true.
+      read(w(2),*) ketarget
+      read(w(3),*) keparm
+      if (nw >= 4) read(w(4),*) kefreq
+    case ('Anisokinetic')
+      isokin = .false.
+    case ('Isoenergetic')
+      isoen = .true.
+      read(w(2),*) eparm
+      if (nw >= 3) read(w(3),*) efreq
+    case ('Anisoenergetic')
+      isoen = .false.
+    case ('Isobaric')
+      isobar = .true.
+      do i = 1, 3
+        read(w(1+i),*) ptarget(i)
+      enddo
+      read(w(5),*) pparm
+      if (nw >= 6) read(w(6),*) pfreq
+    case ('Anisobaric')
+      isobar = .false.
+    case ('BoxVel')
+      boxload = .true.
+      do i = 1, 3
+        read(w(1+i),*) boxvel(i)
+      enddo
+    case ('StopBoxVel')
+      boxload = .false.
+    case ('Exec')
+      call execute_command_line(trim(adjustl(line(5:))))
+    case ('Print')
+      write(6,'(A)') '# '//trim(adjustl(line(6:)))
+    case ('OutFreq')
+      read(w(2),*) outsteps
+    case ('StepNumber')
+      read(w(2),*) step
+    case ('ConfOutFreq')
+      read(w(2),*) confouts
+      if (nw >= 3) outtype = trim(w(3))
+    case ('Precision')
+      if (trim(w(2)) == 'fp64') then
+        path = NDG_PATH_FP64
+      else
+        path = NDG_PATH_MIXED
+      endif
+    case ('ListTrigger')
+      if (trim(w(2)) == 'skin') then
+        trigger = NDG_TRIGGER_SKIN
+      else
+        trigger = NDG_TRIGGER_COMPAT
+      endif
+    case ('Run')
+      read(w(2),*) step_num
+      if (nw >= 3) read(w(3),*) dt
+      call dynamics
+    case ('Group', 'ClearGroups', 'Hold', 'ClearHolds', 'ChangeType', 'Disp', 'InitPos', &
+          'GroupBufferSize', 'HoldBufferSize')
+      write(0,'(A)') '#  read_input: WARNING: directive not supported by this engine: '//trim(w(1))
+    case ('END', 'DONE', 'end', 'done')
+      exit
+    case ('EXIT', 'exit', 'quit', 'abort')
+      stop
+    case default
+      write(0,'(A,i4)') '#  read_input: ERROR: Unknown directive: '//trim(w(1))//' line:', ln
+      call flush(0)
+    end select
+  enddo
+  if (c_associated(h)) i = ndg_destroy(h)
+  write(6,'(a)') '#  Ndyn:  Normal end of execution.'
+
+contains
+
+  subroutine die(msg)
+    character(*), intent(in) :: msg
+    write(0,'(A)') msg
+    call flush(0)
+    stop 1
+  end subroutine die
+
+  subroutine check(ierr)
+    integer(c_int), intent(in) :: ierr
+    select case (ierr)
+    case (NDG_OK)
+      return
+    case (NDG_ERR_NEIMAX)
+      call die('too many neighbours in list')
+    case (NDG_ERR_OUTSIDE)
+      call die('Particle outside of box')
+    case default
+      call die('engine error: '//trim(ndg_error_text()))
+    end select
+  end subroutine check
+
+  subroutine strip_comment(s)
+    character(*), intent(inout) :: s
+    integer :: k
+    k = index(s, '#')
+    if (k > 0) s(k:) = ' '
+  end subroutine strip_comment
+
+  subroutine split(s, words, n)
+    character(*), intent(in) :: s
+    character(len=256), intent(out) :: words(:)
+    integer, intent(out) :: n
+    integer :: k, b
+    logical :: inw
+    n = 0
+    inw = .false.
+    words = ' '
+    b = 1
+    do k = 1, len_trim(s) + 1
+      if (k <= len_trim(s)) then
+        if (s(k:k) /= ' ' .and. s(k:k) /= char(9)) then
+          if (.not. inw) then
+            inw = .true.
+            b = k
+          endif
+          cycle
+        endif
+      endif
+      if (inw) then
+        inw = .false.
+        if (n < size(words)) then
+          n = n + 1
+          words(n) = s(b:k-1)
+        endif
+      endif
+    enddo
+  end subroutine split
+
+  integer function type_id(name)
+    character(*), intent(in) :: name
+    integer :: k
+    type_id = 0
+    do k = 1, ntypes
+      if (trim(name) == trim(typename(k))) type_id = k
+    enddo
+  end function type_id
+
+  subroutine alloc_atoms
+    if (allocated(pos)) deallocate(pos, vel, acc, typ, pene, frc)
+    allocate(pos(3,np), vel(3,np), acc(3,np), typ(np), pene(np), frc(3,np))
+    pos = 0.d0; vel = 0.d0; acc = 0.d0; typ = 1; pene = 0.d0; frc = 0.d0
+    have_model = .true.
+    state_dirty = .true.
+  end subroutine alloc_atoms
+
+  ! read_model (src/read_input.f90:277-392): xyz (real coordinates) or LAMMPS dump
+  ! (scaled xs in [0,1), single precision as the reference reads them)
+  subroutine read_model(fname, kind)
+    character(*), intent(in) :: fname, kind
+    integer :: u, k, id, ty, e
+    real(4) :: lm, lx, x(3)
+    character(len=8) :: ctyp
+    open(newunit=u, file=fname, action='read', status='old', iostat=e)
+    if (e /= 0) call die('read_model: ERROR: opening model file: '//fname)
+    write(6,'(A)') '# reading model input file:'//fname//' format: '//kind
+    select case (kind)
+    case ('xyz', 'XYZ')
+      read(u,*) np
+      read(u,*)
+      call alloc_atoms
+      do k = 1, np
+        read(u,*,iostat=e) ctyp, pos(1:3,k)
+        if (e /= 0) call die('read_model: ERROR: bad model line')
+        typ(k) = type_id(ctyp)
+        if (typ(k) == 0) call die('ERROR: no atom type defined for atom')
+        pos(1:3,k) = pos(1:3,k) / box(1:3)
+      enddo
+    case ('dump', 'trj')
+      read(u,*); read(u,*); read(u,*)
+      read(u,*) np
+      read(u,*)
+      do k = 1, 3
+        read(u,*) lm, lx
+        box(k) = dble(lx - lm)
+      enddo
+      read(u,*)
+      call alloc_atoms
+      do k = 1, np
+        read(u,*,iostat=e) id, ty, x
+        if (e /= 0) call die('read_model: ERROR: bad model line')
+        typ(id) = ty
+        pos(1:3,id) = dble(x(1:3) - 0.5)
+      enddo
+    case default
+      call die('ERROR, input Model Type: '//kind//' not a valid input model format')
+    end select
+    close(u)
+    write(6,'(A,i8,A)') '#  read_model: successfully read ', np, ' model nodes.'
+  end subroutine read_model
+
+  ! randomPositions (src/nd.f90:934-983): uniform scaled positions in [-0.5,0.5)
+  subroutine random_model
+    call alloc_atoms
+    call random_number(pos)
+    pos = pos - 0.5d0
+    write(6,'(a,i8)') '#  Number of particles: ', np
+  end subroutine random_model
+
+  subroutine set_seed(word)
+    character(*), intent(in) :: word
+    integer :: n, sd
+    integer, allocatable :: seeds(:)
+    read(word,*) sd
+    call random_seed(size=n)
+    allocate(seeds(n))
+    seeds = sd
+    call random_seed(put=seeds)
+  end subroutine set_seed
+
+  ! normalVelocityDistribution (src/nd.f90:1500-1624): v = N(0,1) sqrt(2 kt / (3 m)) per
+  ! component (Box-Muller here), then the mean velocity is removed
+  subroutine velocities(kt)
+    real(8), intent(in) :: kt
+    real(8) :: u1, u2, z(3), vsum(3)
+    integer :: k, a
+    if (.not. have_model) call die('ERROR: Velocity needs a model')
+    do k = 1, np
+      do a = 1, 3
+        call random_number(u1)
+        call random_number(u2)
+        u1 = max(u1, 1.d-300)
+        z(a) = sqrt(-2.d0*log(u1)) * cos(2.d0*3.141592653589793d0*u2)
+      enddo
+      vel(1:3,k) = z * sqrt(2.d0*kt/(3.d0*mass(typ(k))))
+    enddo
+    do a = 1, 3
+      vsum(a) = sum(vel(a,:)) / dble(np)
+      vel(a,:) = vel(a,:) - vsum(a)
+    enddo
+    state_dirty = .true.
+  end subroutine velocities
+
+  ! Pairs directive (src/read_input.f90:110-146) and buildPairTables
+  subroutine read_pairs
+    integer :: k, p1, p2, q, e
+    real(8) :: rcut
+    if (ntypes == 0) call die('ERROR: must first define particle Types before Pairs')
+    if (allocated(pairmat)) deallocate(pairmat, pairkind, pairname, pairparams, ptab, ftab, rc2, dr2)
+    allocate(pairmat(ntypes,ntypes), pairkind(npairs), pairname(npairs), pairparams(4,npairs), &
+             ptab(ptsize+1,npairs), ftab(ptsize+1,npairs), rc2(npairs), dr2(npairs))
+    pairmat = 0
+    pairparams = 0.d0
+    pairparams(1,:) = 3.141592653589793d0 / 2.d0
+    do k = 1, npairs
+      read(5,'(A)',iostat=e) line
+      if (e /= 0) call die('#  read_input: ERROR: premature end of input file. No END directive found')
+      call strip_comment(line)
+      call split(line, w, nw)
+      p1 = type_id(w(1))
+      p2 = type_id(w(2))
+      if (p1 == 0 .or. p2 == 0) call die('ERROR: unknown type in Pairs')
+      pairmat(p1,p2) = k
+      if (trim(w(3)) == 'sym') then
+        pairmat(p2,p1) = k
+      else
+        call die('ERROR: asymmetric pairs (compute1) are not supported by this engine')
+      endif
+      pairname(k) = trim(w(4))
+      select case (trim(w(4)))
+      case ('morse', 'Morse')
+        pairkind(k) = NDG_MORSE
+      case ('morse_phys')
+        pairkind(k) = NDG_MORSE_PHYS
+      case ('buck', 'Buck')
+        pairkind(k) = NDG_BUCK
+      case ('LJ')
+        pairkind(k) = NDG_LJ
+      case default
+        call die('ERROR: No potential available for Potential Type: '//trim(w(4)))
+      end select
+      do q = 5, nw
+        read(w(q),*) pairparams(q-4,k)
+      enddo
+    enddo
+    write(6,'(A)') '# pair potential type matrix'
+    rcut = 0.d0
+    do k = 1, npairs
+      call check(ndg_pair_table(int(pairkind(k), c_int), pairparams(:,k), ptrmin, &
+                                int(ptsize, c_int), ptab(:,k), ftab(:,k), rc2(k), dr2(k)))
+      write(6,'(a,i3,2f12.6)') '# pair,rcut2,dr2', k, rc2(k), dr2(k)
+      rcut = max(rcut, pairparams(1,k))
+    enddo
+    write(6,'(A)') '# Built potential lists'
+    write(6,'(A,g11.4)') '# Global maximum cutoff radius: ', rcut
+    if (potouts) call write_tables
+    tables_dirty = .true.
+    call flush(6)
+  end subroutine read_pairs
+
+  subroutine write_tables
+    integer :: u1, u2, k
+    open(newunit=u1, file='pottab.dat')
+    open(newunit=u2, file='forcetab.dat')
+    do k = 1, ptsize
+      write(u1,*) sqrt(real(k)/real(ptsize)), ptab(k,1:npairs)
+      write(u2,*) sqrt(real(k)/real(ptsize)), ftab(k,1:npairs)
+    enddo
+    close(u1)
+    close(u2)
+  end subroutine write_tables
+
+  ! write_model (src/read_input.f90:394-431)
+  subroutine write_model(u)
+    integer, intent(in) :: u
+    integer :: k
+    select case (trim(outtype))
+    case ('XYZ', 'xyz')
+      write(u,'(i8)') np
+      write(u,'(A)')
+      do k = 1, np
+        write(u,'(A4,x,7EN14.4)') typename(typ(k)), pos(1:3,k), vel(1:3,k), pene(k)
+      enddo
+    case ('dump', 'trj')
+      write(u,'(A)') 'ITEM: TIMESTEP'
+      write(u,'(i7)') step
+      write(u,'(A)') 'ITEM: NUMBER OF ATOMS'
+      write(u,'(i7)') np
+      write(u,'(A)') 'ITEM: BOX BOUNDS pp pp pp'
+      write(u,'(2f12.4)') 0.0, box(1)
+      write(u,'(2f12.4)') 0.0, box(2)
+      write(u,'(2f12.4)') 0.0, box(3)
+      write(u,'(A)') 'ITEM: ATOMS id type xs ys zs'
+      do k = 1, np
+        write(u,'(i7,x,i2,3f9.5)') k, typ(k), real(pos(1:3,k)) + 0.0
+      enddo
+    case default
+      call die('ERROR: model out type: '//trim(outtype)//' not supported')
+    end select
+    call flush(u)
+  end subroutine write_model
+
+  subroutine pull_state
+    call check(ndg_get_state(h, pos, vel, acc, frc, pene, box))
+  end subroutine pull_state
+
+  ! dynamics (src/nd.f90:60-199) on the device
+  subroutine dynamics
+    type(ndg_run_params) :: rp
+    type(ndg_thermo), allocatable :: rec(:)
+    integer(c_int) :: nrec, nup, stepdev
+    integer :: u, left, chunk, k, cur, mcell(3), ne
+    real(8) :: rcut, sk, t0, t1
+    if (.not. have_model) call random_model
+    if (npairs == 0) call die('ERROR: no Pairs defined')
+    if (.not. c_associated(h)) then
+      call check(ndg_create(h, int(np, c_int), int(ntypes, c_int), int(npairs, c_int), &
+                            int(ptsize, c_int), int(neimax, c_int), int(path, c_int)))
+      tables_dirty = .true.
+      state_dirty = .true.
+    endif
+    if (tables_dirty) then
+      call check(ndg_set_tables(h, ptab, ftab, rc2, dr2, ptrmin, pairmat, pairparams, mass))
+      tables_dirty = .false.
+    endif
+    if (state_dirty) then
+      call check(ndg_set_state(h, pos, vel, acc, typ, box))
+      state_dirty = .false.
+    endif
+    rcut = maxval(pairparams(1,:))
+    sk = skin
+    if (sk < 0.d0) sk = rcut * 0.2d0
+    mcell = int(box / (rcut + sk))
+    ne = neimax
+    if (ne == 0) ne = int(4./3.*3.141592653589793d0*((rcut+sk)*1.1)**3*np/(box(1)*box(2)*box(3)))
+    write(6,'(A,3i4)') '# Number of link cell: ', mcell
+    write(6,'(A,i4)') '# Estimating max number of neighbours: ', ne
+    if (confouts == 0) confouts = step_num
+    if (trim(outtype) == 'NA') outtype = 'xyz'
+    write(6,'(a)') ''
+    write(6,'(a)') '#  Computing initial forces and energies.'
+    write(6,'(a,i8)') '#  Number of time steps: ', step_num
+    write(6,'(a,g14.6)') '#  Time step increment: ', dt
+    write(6,'(a,g14.6)') '#  Cutoff radius: ', rcut
+    write(6,'(a,g14.6)') '#  Cutoff skin: ', sk
+    write(6,'(a)') '# Model Output format: '//trim(outtype)
+    write(6,'(a)') ''
+    write(6,'(a)') '#  Step   Potential     Kinetic        EnergyError    pressure       volume'
+    call flush(6)
+    rp%nsteps = step_num
+    rp%dt = dt
+    rp%out_steps = outsteps
+    rp%skin = skin
+    rp%renei = merge(1, 0, renei)
+    rp%renei_freq = renei_freq
+    rp%trigger = trigger
+    rp%isokinetic = merge(1, 0, isokin); rp%ke_target = ketarget; rp%ke_tau = keparm
+    rp%ke_freq = kefreq
+    rp%isobaric = merge(1, 0, isobar); rp%p_target = ptarget; rp%p_tau = pparm; rp%p_freq = pfreq
+    rp%isoenergetic = merge(1, 0, isoen); rp%e_tau = eparm; rp%e_freq = efreq
+    rp%box_load = merge(1, 0, boxload); rp%box_vel = boxvel
+    call cpu_time(t0)
+    open(newunit=u, file='out.'//trim(outtype), position='append')
+    call write_model(u)
+    call check(ndg_run_begin(h, rp, int(step, c_int)))
+    left = step_num
+    cur = step
+    do while (left > 0)
+      chunk = min(left, confouts - mod(cur, confouts))
+      allocate(rec(chunk / max(1, outsteps) + 2))
+      call check(ndg_run_steps(h, int(chunk, c_int), rec, int(size(rec), c_int), nrec))
+      do k = 1, nrec
+        write(6, '(x,i8,x,g14.6,x,g14.6,x,g14.6,x,g14.6,x,g14.6)') rec(k)%step, rec(k)%pe, &
+          rec(k)%ke, rec(k)%eerr, rec(k)%pressure, rec(k)%volume
+      enddo
+      call flush(6)
+      deallocate(rec)
+      cur = cur + chunk
+      left = left - chunk
+      step = cur
+      if (mod(cur, confouts) == 0) then
+        call pull_state
+        call write_model(u)
+      endif
+    enddo
+    call check(ndg_run_end(h, nup, stepdev))
+    step = stepdev
+    call pull_state
+    close(u)
+    call cpu_time(t1)
+    write(6, '(a,i8)') '# Number of neighbour list updates:', nup
+    write(6, '(a)') ''
+    write(6, '(a)') '#  Elapsed time for main dynamics:'
+    write(6, '(a,g14.6,a)') '#  ', t1 - t0, ' seconds'
+    call flush(6)
+  end subroutine dynamics
+
+  subroutine help
+    write(6,'(A)') 'Directives (reference grammar): BoxSize Model Seed NTotal Velocity ReNeighbor'
+    write(6,'(A)') ' Skin NeiMax Types Pairs WritePotentialTable Isokinetic Anisokinetic'
+    write(6,'(A)') ' Isoenergetic Anisoenergetic Isobaric Anisobaric BoxVel StopBoxVel Exec Print'
+    write(6,'(A)') ' OutFreq StepNumber ConfOutFreq Run END; engine: Precision mixed|fp64,'
+    write(6,'(A)') ' ListTrigger compat|skin.  Pair kinds: morse morse_phys buck LJ (sym only).'
+    call flush(6)
+  end subroutine help
+
+end program ndyn_gpu

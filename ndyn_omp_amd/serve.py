@@ -1,0 +1,138 @@
+"""Continuous-batching MD job server on one GPU.
+
+Jobs (structure, potential, ensemble, step count) arrive over time.  The server admits a
+queued job as soon as its HBM footprint fits the engine's budget, keeps every admitted
+job's state (positions, velocities, neighbour lists, thermostat state) resident on the
+device, and advances all resident jobs together: each tick is one batched launch
+sequence of `tick_steps` MD steps over every resident job, each job at its own step
+count.  Finished jobs are fetched and released between ticks, which frees memory for the
+queue.  Latency = completion - arrival; throughput = atom-steps of finished work / wall
+time.  Each job's result does not depend on what else is resident (per-replica kernels,
+fixed-order reductions).
+"""
+from __future__ import annotations
+
+import dataclasses
+import time
+from collections import deque
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import engine as G
+from . import workloads as W
+
+
+@dataclasses.dataclass
+class Job:
+    jid: int
+    system: W.System
+    nsteps: int
+    ensemble: str = "nve"          # "nve" | "nvt"
+    temp_k: float = 300.0
+    arrival: float = 0.0           # seconds after the server start
+    skin: float = 0.5
+    # filled by the server
+    slot: Optional[int] = None
+    admitted: Optional[float] = None
+    finished: Optional[float] = None
+    result: Optional[dict] = None
+    thermo: Optional[list] = None
+
+    @property
+    def atom_steps(self) -> int:
+        return self.system.np * self.nsteps
+
+    def run_params(self):
+        iso = None
+        if self.ensemble == "nvt":
+            # Berendsen velocity scaling at temp_k, tau = 100 fs (Isokinetic KE/atom tau freq)
+            iso = (1.5 * W.KB_EV * self.temp_k, 100 * W.DT_1FS, 1)
+        return G.run_params(self.nsteps, W.DT_1FS, out_steps=max(1, self.nsteps), skin=self.skin,
+                            trigger=G.TRIGGER_SKIN, isokinetic=iso)
+
+
+class Server:
+    def __init__(self, hbm_budget: int = 0, tick_steps: int = 50, path: int = G.PATH_MIXED,
+                 graph_steps: int = 0):
+        self.eng = G.BatchEngine(path=path, hbm_budget=hbm_budget, graph_steps=graph_steps)
+        self.tick_steps = tick_steps
+        self.queue: deque = deque()
+        self.active: Dict[int, Job] = {}
+        self.done: List[Job] = []
+        self.t0 = time.perf_counter()
+
+    def now(self) -> float:
+        return time.perf_counter() - self.t0
+
+    def submit(self, job: Job):
+        self.queue.append(job)
+
+    def _admit(self):
+        """Admit queued jobs in arrival order while their footprint fits."""
+        while self.queue:
+            job = self.queue[0]
+            slot = self.eng.admit(job.system, job.run_params(), raise_on_full=False)
+            if slot is None:
+                if not self.active:
+                    raise RuntimeError(f"job {job.jid} ({job.system.np} atoms) exceeds the HBM budget")
+                return
+            self.queue.popleft()
+            job.slot = slot
+            job.admitted = self.now()
+            self.active[slot] = job
+
+    def _retire(self):
+        for slot in list(self.active):
+            st = self.eng.status(slot)
+            if st["err"]:
+                raise G.NdgError(st["err"], f"job {self.active[slot].jid} failed")
+            if st["finished"]:
+                job = self.active.pop(slot)
+                job.thermo = self.eng.thermo(slot)
+                job.result = self.eng.fetch(slot, job.system.np)
+                job.finished = self.now()
+                self.eng.release(slot)
+                self.done.append(job)
+
+    def run(self, jobs: List[Job], realtime: bool = True, log=None) -> dict:
+        """Serve `jobs` (sorted by arrival).  realtime=True honours arrival times on the
+        wall clock; False admits each job as soon as it is submitted."""
+        pending = deque(sorted(jobs, key=lambda j: j.arrival))
+        self.done = []
+        self.t0 = time.perf_counter()
+        busy = 0.0
+        ticks = 0
+        while pending or self.queue or self.active:
+            t = self.now()
+            while pending and (not realtime or pending[0].arrival <= t):
+                self.submit(pending.popleft())
+            self._admit()
+            if not self.active:
+                if pending:
+                    time.sleep(max(0.0, min(0.05, pending[0].arrival - self.now())))
+                continue
+            tb = time.perf_counter()
+            self.eng.step(self.tick_steps)
+            busy += time.perf_counter() - tb
+            ticks += 1
+            self._retire()
+            if log and ticks % 20 == 0:
+                log(f"serve: t={self.now():.1f}s done={len(self.done)} active={len(self.active)} "
+                    f"queued={len(self.queue)} pending={len(pending)}")
+        wall = self.now()
+        lat = np.array([j.finished - j.arrival for j in self.done])
+        atom_steps = float(sum(j.atom_steps for j in self.done))
+        return dict(jobs=len(self.done), wall_s=wall, busy_s=busy, ticks=ticks,
+                    atom_steps=atom_steps, atom_steps_per_s=atom_steps / wall,
+                    atom_steps_per_busy_s=atom_steps / busy if busy > 0 else 0.0,
+                    p50_latency_s=float(np.percentile(lat, 50)) if len(lat) else None,
+                    p99_latency_s=float(np.percentile(lat, 99)) if len(lat) else None)
+
+    def close(self):
+        self.eng.close()
+
+
+def jobs_from_stream(stream) -> List[Job]:
+    """Jobs from workloads.serving_jobs tuples (arrival, System, nsteps, ensemble, T)."""
+    return [Job(k, s, n, ens, t, arrival=a) for k, (a, s, n, ens, t) in enumerate(stream)]

@@ -1,0 +1,145 @@
+"""The Fortran deck driver (ndyn_omp_amd/bin/ndyn_gpu) over the C ABI.
+
+CPU: it parses the reference's deck grammar and writes the reference's potential tables
+(WritePotentialTable, src/potentials.f90:44-54) equal to the oracle's.
+GPU: the reference's example decks reproduce the thermo lines of their committed logs
+(examples/MorseTest/out, examples/testT/out) through the driver.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from ndyn_omp_amd import workloads as W
+
+DRIVER = os.path.join(ROOT, "ndyn_omp_amd", "bin", "ndyn_gpu")
+needs_driver = pytest.mark.skipif(not os.path.exists(DRIVER), reason="driver not built")
+
+
+def run_deck(deck, cwd, timeout=300):
+    p = subprocess.run([DRIVER], input=deck, cwd=cwd, capture_output=True, text=True,
+                       timeout=timeout)
+    return p
+
+
+def thermo(stdout):
+    rows = []
+    for line in stdout.splitlines():
+        parts = line.split()
+        if len(parts) == 6 and not line.lstrip().startswith("#"):
+            try:
+                rows.append([float(x) for x in parts])
+            except ValueError:
+                pass
+    return rows
+
+
+def write_fcc_dump(path, n=10, a=3.68):
+    """The perfect FCC lattice of examples/MorseTest/model.dump (id order, xs in [0,1))."""
+    pos, box = W.fcc(n, n, n, a)
+    with open(path, "w") as f:
+        f.write("ITEM: TIMESTEP\n0\nITEM: NUMBER OF ATOMS\n%d\nITEM: BOX BOUNDS pp pp pp\n" % len(pos))
+        for L in box:
+            f.write(f"0 {L:g}\n")
+        f.write("ITEM: ATOMS id type xs ys zs\n")
+        for k, p in enumerate(pos + 0.5):
+            f.write(f"{k + 1} 1 {p[0]:.10g} {p[1]:.10g} {p[2]:.10g}\n")
+
+
+@needs_driver
+def test_driver_writes_reference_tables(tmp_path, oracle):
+    deck = """START
+Types 1
+ Cu 63.546
+WritePotentialTable
+Pairs 1
+ Cu Cu sym morse 6.0 0.3429 1.359 2.866
+END
+"""
+    p = run_deck(deck, tmp_path)
+    assert p.returncode == 0, p.stderr
+    assert "# pair,rcut2,dr2  1   36.000000    0.035786" in p.stdout   # examples/MorseTest/out:20
+    got = np.loadtxt(tmp_path / "forcetab.dat")
+    P, F, _, _ = oracle.pair_tables([1], (6.0, 0.3429, 1.359, 2.866), 0.25, 1000)
+    # rows i = 1..PTsize of the Fortran table are 0-based rows 0..PTsize-1
+    np.testing.assert_allclose(got[:, 1], F[0][0:1000], rtol=1e-6)
+    np.testing.assert_allclose(got[:, 0], np.sqrt(np.arange(1, 1001) / 1000.0), rtol=1e-6)
+    pot = np.loadtxt(tmp_path / "pottab.dat")
+    np.testing.assert_allclose(pot[:, 1], P[0][0:1000], rtol=1e-6)
+
+
+@needs_driver
+def test_driver_rejects_unknown_potential(tmp_path):
+    deck = "START\nTypes 1\n Cu 63.546\nPairs 1\n Cu Cu sym sine 1.0\nEND\n"
+    p = run_deck(deck, tmp_path)
+    assert p.returncode != 0
+    assert "No potential available for Potential Type" in p.stderr
+
+
+@pytest.mark.gpu
+@needs_driver
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_driver_morsetest_deck(tmp_path, precision):
+    """examples/MorseTest/input.sh (Run shortened to 200 steps): thermo lines of
+    examples/MorseTest/out:38-43 -- PE -13581.1, pressure 0.358006E-01, volume 49836.0 at
+    steps 1, 40, ..., 200; the dump trajectory written at the start and the end."""
+    write_fcc_dump(tmp_path / "model.dump")
+    deck = f"""START
+Model 	model.dump dump
+NeiMax  200
+Types 1
+ Cu 63.546
+Pairs 1
+ Cu Cu sym morse 6.0 0.3429 1.359 2.866
+Precision {precision}
+OutFreq 40
+ConfOutFreq 0 dump
+Run 200 0.0001
+END
+"""
+    print(f"case morsetest {precision}", flush=True)
+    p = run_deck(deck, tmp_path)
+    assert p.returncode == 0, p.stderr
+    rows = thermo(p.stdout)
+    assert [int(r[0]) for r in rows] == [1, 40, 80, 120, 160, 200]
+    for r in rows:
+        assert float(f"{r[1]:.6g}") == -13581.1
+        assert float(f"{r[4]:.6g}") == 0.0358006
+        assert float(f"{r[5]:.6g}") == 49836.0
+    assert "# Number of neighbour list updates:       0" in p.stdout
+    dump = (tmp_path / "out.dump").read_text().splitlines()
+    assert dump.count("ITEM: TIMESTEP") == 2 and dump[1].strip() == "0" and "200" in dump[dump.index("ITEM: TIMESTEP", 1) + 1]
+
+
+@pytest.mark.gpu
+@needs_driver
+def test_driver_testT_npt_deck(tmp_path):
+    """examples/testT/input.sh run 1 (Run shortened to 200 steps): examples/testT/out:38-40
+    -> step 1 PE -13581.1 V 49836.0; step 100 PE -12798.9 V 56637.3; step 200 PE -12798.8
+    V 56637.5; ConfOutFreq 100 writes a dump frame every 100 steps."""
+    write_fcc_dump(tmp_path / "model.dump")
+    deck = """START
+Model 	model.dump dump
+Seed	1234567890
+Skin 	0.2
+Types 1
+ Cu 63.546 0.0
+Pairs 1
+ Cu Cu sym morse 6.0  0.3429 1.359 2.866
+OutFreq 	100
+ReNeighbor 	F
+ConfOutFreq 	100 dump
+Isobaric 	0.0 0.0 0.0  0.1 10
+Run 		200 0.1
+Anisobaric
+END
+"""
+    p = run_deck(deck, tmp_path)
+    assert p.returncode == 0, p.stderr
+    rows = thermo(p.stdout)
+    got = [(int(r[0]), float(f"{r[1]:.6g}"), float(f"{r[5]:.6g}")) for r in rows]
+    assert got == [(1, -13581.1, 49836.0), (100, -12798.9, 56637.3), (200, -12798.8, 56637.5)]
+    dump = (tmp_path / "out.dump").read_text()
+    assert dump.count("ITEM: TIMESTEP") == 3

@@ -1,0 +1,67 @@
+"""Serving mode: jobs served alone and mixed with others give the same result.
+
+Four jobs of different size, potential, ensemble and length are served (a) each alone
+and (b) together through one server whose HBM budget cannot hold all of them at once, with
+an odd tick length so jobs finish mid-tick and queued jobs are admitted while others are
+on the GPU.  Bar: final positions, velocities and the last thermo record identical bit
+for bit (per-replica kernels and fixed-order reductions make a job independent of its
+neighbours in the batch).
+"""
+import numpy as np
+import pytest
+
+from ndyn_omp_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+G = pytest.importorskip("ndyn_omp_amd.engine")
+S = pytest.importorskip("ndyn_omp_amd.serve")
+
+
+def make_jobs():
+    specs = [
+        (W.cu_morse(6, temp_k=700.0, seed=31), 37, "nvt", 700.0),       # small box: brute list
+        (W.buck_alloy((5, 5, 5), temp_k=900.0, seed=32), 53, "nve", 900.0),
+        (W.cu_morse(8, temp_k=300.0, seed=33), 80, "nve", 300.0),
+        (W.cu_morse(7, temp_k=1100.0, seed=34), 45, "nvt", 1100.0),
+    ]
+    return [S.Job(k, s, n, ens, t, arrival=0.0) for k, (s, n, ens, t) in enumerate(specs)]
+
+
+def test_served_alone_equals_served_in_mix():
+    alone = {}
+    for job in make_jobs():
+        print(f"case alone job {job.jid} np={job.system.np}", flush=True)
+        srv = S.Server(tick_steps=16)
+        srv.run([job], realtime=False)
+        alone[job.jid] = job
+        srv.close()
+    jobs = make_jobs()
+    fp = sorted(G.footprint(j.system, 200) for j in jobs)
+    budget = fp[-1] + fp[-2] + (8 << 20)        # at most about two jobs resident at a time
+    print(f"case mixed budget={budget}", flush=True)
+    srv = S.Server(hbm_budget=budget, tick_steps=7)
+    stats = srv.run(jobs, realtime=False)
+    srv.close()
+    assert stats["jobs"] == 4
+    admitted = sorted(j.admitted for j in jobs)
+    assert admitted[-1] > admitted[0]            # some job waited for memory
+    for j in jobs:
+        a = alone[j.jid]
+        np.testing.assert_array_equal(j.result["pos"], a.result["pos"])
+        np.testing.assert_array_equal(j.result["vel"], a.result["vel"])
+        np.testing.assert_array_equal(j.result["box"], a.result["box"])
+        assert j.thermo[-1] == a.thermo[-1]
+
+
+def test_serving_stream_latency_accounting():
+    """A short Poisson stream: every job completes, latency >= its own service time."""
+    stream = W.serving_jobs(12, seed=7, size_scale=0.02, step_scale=0.004, rate_hz=50.0)
+    jobs = S.jobs_from_stream(stream)
+    srv = S.Server(tick_steps=25)
+    stats = srv.run(jobs, realtime=True)
+    srv.close()
+    assert stats["jobs"] == 12
+    for j in jobs:
+        assert j.finished >= j.admitted >= j.arrival - 1e-3
+        assert j.thermo and j.thermo[-1]["step"] == j.nsteps
+    assert stats["p99_latency_s"] >= stats["p50_latency_s"] > 0
