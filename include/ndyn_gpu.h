@@ -198,6 +198,13 @@ int ndg_engine_counters(ndg_engine_t *e, int slot, int *nupdates, int *step, dou
 /* Mean full-list length over admitted systems at their last build. */
 int ndg_engine_list_stats(ndg_engine_t *e, double *mean_nbr, long long *atoms);
 
+/* ------------------------------------------------------------------ measurement */
+
+/* HBM bandwidth of a plain device copy of `bytes` (read + write bytes / time), averaged
+ * over iters launches.  Not part of the reference: the measured denominator of the
+ * roofline and the calibration point of the FETCH_SIZE / WRITE_SIZE counters. */
+int ndg_copy_bandwidth(size_t bytes, int iters, double *gbps);
+
 #ifdef __cplusplus
 }
 #endif

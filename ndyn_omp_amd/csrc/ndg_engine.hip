@@ -27,8 +27,8 @@ void launch_reduce(const EngineDev &E, int r0, int nr, int mode, int xflags, int
 void launch_apply_pending(const EngineDev &E, int r0, int nr, hipStream_t s);
 void launch_rebuild(const EngineDev &E, int t0, int nt, int r0, int nr, bool any_large,
                     hipStream_t s);
+void launch_copy(const float4 *src, float4 *dst, size_t n, hipStream_t s);
 enum : int { RED_FORCE = 1, RED_KICK = 2, RED_FLIP = 4, RED_STEP = 8, RED_CHECK = 16 };
-constexpr int kSmallSortHost = 8192;
 }  // namespace ndg
 
 using namespace ndg;
@@ -163,6 +163,7 @@ struct Slot {
   ndg_thermo last{};
   int err = 0;
   long long nbr_sum = 0;
+  bool auto_neimax = false;   // list capacity chosen by the engine: grow on overflow
 };
 
 }  // namespace
@@ -445,7 +446,7 @@ static int rebuild_tiles(ndg_engine *e) {
       if (!s.used || s.tpa != tpas[q]) continue;
       s.tile_base = (int)e->h_tiles.size();
       for (int lt = 0; lt < s.ntiles; ++lt) e->h_tiles.push_back(Tile{(int)r, lt * kTile, lt, 0});
-      if (s.np > kSmallSortHost) e->any_large = true;
+      if (!small_sort(s.np, s.ncell_cap)) e->any_large = true;
     }
     int nt = (int)e->h_tiles.size() - t0;
     if (nt > 0) {
@@ -729,7 +730,40 @@ static int download_state(ndg_engine *e, Slot &s, double *pos, double *vel, doub
 }
 
 // Allocate (or grow) a slot's block for the given list capacity and cell grid.
-static int alloc_slot(ndg_engine *e, int r, int neimax, int ncell_cap, int tpa) {
+// Copy a replica's state and per-type data from its old block to a new one (device to
+// device); lists and cell arrays are rebuilt afterwards.
+static int migrate_block(ndg_engine *e, const RepParams &o, const RepParams &n, int np,
+                         int ntypes, int npairs) {
+  hipStream_t st = e->stream;
+  const size_t d = sizeof(double) * (size_t)np;
+  const void *src[] = {o.px, o.py, o.pz, o.vx, o.vy, o.vz, o.ax, o.ay, o.az, o.fx, o.fy, o.fz,
+                       o.pene, o.pbx, o.pby, o.pbz};
+  void *dst[] = {n.px, n.py, n.pz, n.vx, n.vy, n.vz, n.ax, n.ay, n.az, n.fx, n.fy, n.fz,
+                 n.pene, n.pbx, n.pby, n.pbz};
+  for (int q = 0; q < 16; ++q) HIPCHK(hipMemcpyAsync(dst[q], src[q], d, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(n.f32a, o.f32a, 16 * (size_t)np, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(n.f32b, o.f32b, 16 * (size_t)np, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(n.x64a, o.x64a, 32 * (size_t)np, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(n.x32b, o.x32b, 16 * (size_t)np, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(n.anchor, o.anchor, 16 * (size_t)np, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(n.typ, o.typ, 4 * (size_t)np, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(n.oid, o.oid, 4 * (size_t)np, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync((void *)n.mass, o.mass, 8 * ntypes, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync((void *)n.pairmat, o.pairmat, 4 * ntypes * ntypes, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync((void *)n.rc_pair, o.rc_pair, 8 * npairs, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync((void *)n.rc2d, o.rc2d, 8 * npairs, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync((void *)n.rc2f, o.rc2f, 4 * npairs, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync((void *)n.vdr2, o.vdr2, 4 * npairs, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(n.ring, o.ring, sizeof(ndg_thermo) * kRing, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemsetAsync(n.cell_count, 0, 4 * (size_t)n.ncell_cap, st));
+  HIPCHK(hipMemsetAsync(n.cell_fill, 0, 4 * (size_t)n.ncell_cap, st));
+  HIPCHK(hipMemsetAsync(n.nnb, 0, 4 * (size_t)np, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return NDG_OK;
+}
+
+static int alloc_slot(ndg_engine *e, int r, int neimax, int ncell_cap, int tpa,
+                      bool migrate = false) {
   Slot &s = e->slots[r];
   Slot probe = s;
   probe.neimax = neimax;
@@ -739,11 +773,9 @@ static int alloc_slot(ndg_engine *e, int r, int neimax, int ncell_cap, int tpa) 
   void *blk;
   int rc = dev_alloc(e, need, &blk);
   if (rc) return rc;
-  if (s.block) {
-    HIPCHK(hipStreamSynchronize(e->stream));
-    dev_release(e, s.block, s.bytes);
-  }
   const RepParams keep = s.P;
+  void *old_block = s.block;
+  const size_t old_bytes = s.bytes;
   s.neimax = neimax;
   s.ncell_cap = ncell_cap;
   s.tpa = tpa;
@@ -756,6 +788,11 @@ static int alloc_slot(ndg_engine *e, int r, int neimax, int ncell_cap, int tpa) 
   s.P.tabP64 = keep.tabP64;
   s.P.tabF2 = keep.tabF2;
   s.P.tabP2 = keep.tabP2;
+  if (old_block) {
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (migrate && (rc = migrate_block(e, keep, s.P, s.np, s.ntypes, s.npairs))) return rc;
+    dev_release(e, old_block, old_bytes);
+  }
   e->tiles_dirty = true;
   return NDG_OK;
 }
@@ -778,7 +815,7 @@ static int build_slot_now(ndg_engine *e, int r, double skin) {
   if ((rc = upload_params(e, r))) return rc;
   if ((rc = put_state_dev(e, r, &S))) return rc;
   EngineDev E = dev_view(e);
-  launch_rebuild(E, s.tile_base, s.ntiles, r, 1, s.np > kSmallSortHost, e->stream);
+  launch_rebuild(E, s.tile_base, s.ntiles, r, 1, !small_sort(s.np, s.ncell_cap), e->stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(e->stream));
   if ((rc = get_state_dev(e, r, &S))) return rc;
@@ -905,7 +942,7 @@ static void enqueue_step_single(ndg_engine *e, int r, const TickPlan &tp) {
   S.step = sl.step;
   const int f = step_flags(sl.P, S);
   if (!(f & F_ACTIVE)) return;
-  if (f & F_CHECK) launch_rebuild(E, sl.tile_base, sl.ntiles, r, 1, sl.np > kSmallSortHost, s);
+  if (f & F_CHECK) launch_rebuild(E, sl.tile_base, sl.ntiles, r, 1, !small_sort(sl.np, sl.ncell_cap), s);
   launch_drift(E, e->path, sl.tile_base, sl.ntiles, 0, s);
   hipEvent_t a = nullptr;
   time_begin(e, &a);
@@ -923,6 +960,20 @@ static std::string tick_signature(const std::vector<TickPlan> &plans) {
   return sig;
 }
 
+// Build the lists of slot r; when the capacity was chosen by the engine, an overflow
+// grows it by 1.5x (state migrated device to device) and builds again.
+static int build_with_growth(ndg_engine *e, int r, double skin) {
+  Slot &s = e->slots[r];
+  int rc = build_slot_now(e, r, skin);
+  for (int attempt = 0; rc == NDG_ERR_NEIMAX && s.auto_neimax && attempt < 6; ++attempt) {
+    const int neimax = s.neimax + s.neimax / 2 + 8;
+    if ((rc = alloc_slot(e, r, neimax, s.ncell_cap, s.tpa, true))) return rc;
+    if ((rc = upload_params(e, r))) return rc;
+    rc = build_slot_now(e, r, skin);
+  }
+  return rc;
+}
+
 // Drain device state and thermo rings after a chunk.
 static int drain(ndg_engine *e) {
   const int nr = (int)e->slots.size();
@@ -935,6 +986,13 @@ static int drain(ndg_engine *e) {
     Slot &s = e->slots[r];
     if (!s.used) continue;
     s.nbr_sum = S[r].nbr_sum;
+    if (S[r].err == NDG_ERR_NEIMAX && s.auto_neimax && !s.err) {
+      // engine-chosen capacity: grow the list and rebuild at the current positions; the
+      // replica then computes the step whose build failed
+      int rc = build_with_growth(e, r, s.P.skin);
+      if (rc) return rc;
+      if ((rc = get_state_dev(e, r, &S[r]))) return rc;
+    }
     if (S[r].err && !s.err) {
       s.err = S[r].err;
       if (!first_err) first_err = s.err;
@@ -948,9 +1006,9 @@ static int drain(ndg_engine *e) {
       s.last = s.pending_rec.back();
       s.nrec_seen = S[r].nrec;
     }
-    // the device is the authority on the step counter
+    // the device is the authority on the step counter and the end of the Run
     s.step = S[r].step;
-    if (S[r].done) s.finished = true;
+    s.finished = S[r].done != 0 || s.step - s.run_step0 >= s.run_nsteps;
   }
   time_collect(e);
   if (first_err) return fail(first_err, err_text(first_err));
@@ -1158,6 +1216,7 @@ static int begin_run(ndg_engine *e, int r, HostState &h, const ndg_run_params &r
   }
   int neimax = neimax_req > 0 ? neimax_req : estimate_neimax(rcut, skin, h.np, box);
   if (neimax < 1) neimax = 1;
+  s.auto_neimax = neimax_req <= 0;
   const int tpa = e->tpa_override ? e->tpa_override : default_tpa(h.np);
   const int ncap = ncell_capacity(box, rcut, skin);
   const bool need_alloc = !s.block || neimax > s.neimax || ncap > s.ncell_cap || tpa != s.tpa;
@@ -1208,7 +1267,7 @@ static int begin_run(ndg_engine *e, int r, HostState &h, const ndg_run_params &r
   s.pending_rec.clear();
   s.err = 0;
   if ((rc = put_state_dev(e, r, &S))) return rc;
-  return build_slot_now(e, r, skin);
+  return build_with_growth(e, r, skin);
 }
 
 extern "C" int ndg_engine_admit(ndg_engine_t *e, int *slot, int np, int ntypes, int npairs,
@@ -1699,5 +1758,35 @@ extern "C" int ndg_run_end(ndg_t *h, int *nupdates, int *step) {
   if (rc) return rc;
   if (nupdates) *nupdates = S.nupdates;
   if (step) *step = S.step;
+  return NDG_OK;
+}
+
+// ------------------------------------------------------------------ measurement utility
+
+extern "C" int ndg_copy_bandwidth(size_t bytes, int iters, double *gbps) {
+  if (bytes < 1024 || iters < 1 || !gbps) return fail(NDG_ERR_ARG, "ndg_copy_bandwidth: bad arguments");
+  const size_t n = bytes / sizeof(float4);
+  float4 *a = nullptr, *b = nullptr;
+  hipStream_t st;
+  HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  HIPCHK(hipMalloc(&a, n * sizeof(float4)));
+  HIPCHK(hipMalloc(&b, n * sizeof(float4)));
+  HIPCHK(hipMemsetAsync(a, 0, n * sizeof(float4), st));
+  launch_copy(a, b, n, st);  // warm
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, st));
+  for (int k = 0; k < iters; ++k) launch_copy(a, b, n, st);
+  HIPCHK(hipEventRecord(e1, st));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  *gbps = 2.0 * (double)(n * sizeof(float4)) * iters / (ms * 1e-3) / 1e9;  // read + write
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(a);
+  hipFree(b);
+  hipStreamDestroy(st);
   return NDG_OK;
 }

@@ -857,57 +857,74 @@ __device__ inline void sort_cell_wave(const RepParams &R, int lo, int hi, int pa
   }
 }
 
-// Whole sort of one small replica inside one workgroup (replicas up to kSmallSort atoms).
-constexpr int kSmallSort = 8192;
-
+// Whole sort of one small replica (small_sort(np, ncell_cap)) inside one workgroup, with
+// the bookkeeping in LDS: cell keys and caller indices, cell counts and starts by LDS
+// atomics and a block scan, an unordered scatter of caller indices into cell order, and
+// the final slot of each atom = its cell's start + the number of atoms of the cell with a
+// smaller caller index.  Brute mode (one cell) puts every atom at its caller index.
 __global__ __launch_bounds__(1024) void k_rb_small(EngineDev E, int r0, int nr) {
-  __shared__ int sh_scan[1024];
-  __shared__ int sh_sort[16][2 * kCellLds];
+  __shared__ unsigned short s_key[kSmallSort], s_oid[kSmallSort], s_moid[kSmallSort];
+  __shared__ int s_start[kSmallCells + 1], s_fill[kSmallCells];
+  __shared__ int s_scan[1024];
   for (int r = r0 + blockIdx.x; r < r0 + nr; r += gridDim.x) {
     const RepParams &R = E.reps[r];
     RepState &S = E.st[r];
-    if (!S.rebuild || S.err || R.np > kSmallSort) continue;
+    if (!S.rebuild || S.err || !small_sort(R.np, R.ncell_cap)) continue;
     const int np = R.np, nb = blockDim.x, tid = threadIdx.x;
-    const int ncell = S.mcell[0] * S.mcell[1] * S.mcell[2];
-    for (int c = tid; c < ncell; c += nb) { R.cell_count[c] = 0; R.cell_fill[c] = 0; }
-    __syncthreads();
-    for (int i = tid; i < np; i += nb) {
-      const int c = cell_of(R, S, i);
-      R.key[i] = c;
-      atomicAdd(&R.cell_count[c], 1);
-    }
-    __syncthreads();
-    // exclusive scan: thread t owns a contiguous chunk of cells
-    const int per = (ncell + nb - 1) / nb;
-    const int c0 = tid * per, c1 = min(ncell, c0 + per);
-    int loc = 0;
-    for (int c = c0; c < c1; ++c) loc += R.cell_count[c];
-    sh_scan[tid] = loc;
-    __syncthreads();
-    for (int off = 1; off < nb; off <<= 1) {
-      int v = tid >= off ? sh_scan[tid - off] : 0;
-      __syncthreads();
-      sh_scan[tid] += v;
-      __syncthreads();
-    }
-    int run = sh_scan[tid] - loc;
-    for (int c = c0; c < c1; ++c) { R.cell_start[c] = run; run += R.cell_count[c]; }
-    if (tid == nb - 1) R.cell_start[ncell] = sh_scan[nb - 1];
-    __syncthreads();
-    for (int i = tid; i < np; i += nb) {
-      const int c = R.key[i];
-      R.slot_src[R.cell_start[c] + atomicAdd(&R.cell_fill[c], 1)] = i;
-    }
-    __syncthreads();
     const int parity = S.parity;
     if (S.brute) {
-      // one cell holding every atom: the final slot of an atom is its caller index
       for (int i = tid; i < np; i += nb) permute_atom(R, i, R.oid[i], parity);
+      if (tid == 0) {
+        R.cell_start[0] = 0;
+        R.cell_start[1] = np;
+      }
     } else {
-      const int nwave = nb >> 6, wid = tid >> 6;
-      for (int c = wid; c < ncell; c += nwave) {
-        const int lo = R.cell_start[c], hi = R.cell_start[c + 1];
-        if (hi > lo) sort_cell_wave(R, lo, hi, parity, sh_sort[wid], sh_sort[wid] + kCellLds);
+      const int ncell = S.mcell[0] * S.mcell[1] * S.mcell[2];
+      for (int c = tid; c < ncell; c += nb) { s_start[c] = 0; s_fill[c] = 0; }
+      __syncthreads();
+      for (int p = tid; p < np; p += nb) {
+        const int c = cell_of(R, S, p);
+        s_key[p] = (unsigned short)c;
+        s_oid[p] = (unsigned short)R.oid[p];
+        atomicAdd(&s_start[c], 1);
+      }
+      __syncthreads();
+      // exclusive scan of the counts: thread t owns a contiguous chunk of cells
+      const int per = (ncell + nb - 1) / nb;
+      const int c0 = min(ncell, tid * per), c1 = min(ncell, c0 + per);
+      int loc = 0;
+      for (int c = c0; c < c1; ++c) loc += s_start[c];
+      s_scan[tid] = loc;
+      __syncthreads();
+      for (int off = 1; off < nb; off <<= 1) {
+        const int v = tid >= off ? s_scan[tid - off] : 0;
+        __syncthreads();
+        s_scan[tid] += v;
+        __syncthreads();
+      }
+      int run = s_scan[tid] - loc;
+      for (int c = c0; c < c1; ++c) {
+        const int cnt = s_start[c];
+        s_start[c] = run;
+        R.cell_start[c] = run;
+        run += cnt;
+      }
+      if (tid == nb - 1) {
+        s_start[ncell] = s_scan[nb - 1];
+        R.cell_start[ncell] = s_scan[nb - 1];
+      }
+      __syncthreads();
+      for (int p = tid; p < np; p += nb) {
+        const int c = s_key[p];
+        s_moid[s_start[c] + atomicAdd(&s_fill[c], 1)] = s_oid[p];
+      }
+      __syncthreads();
+      for (int p = tid; p < np; p += nb) {
+        const int c = s_key[p], lo = s_start[c], hi = s_start[c + 1];
+        const int me = s_oid[p];
+        int rank = 0;
+        for (int k = lo; k < hi; ++k) rank += s_moid[k] < me;
+        permute_atom(R, p, lo + rank, parity);
       }
     }
     __syncthreads();
@@ -928,7 +945,7 @@ __global__ __launch_bounds__(kTile) void k_rb_bin(EngineDev E, int t0, int nt) {
     const Tile T = E.tiles[t];
     const RepParams &R = E.reps[T.rep];
     const RepState &S = E.st[T.rep];
-    if (!S.rebuild || S.err || R.np <= kSmallSort) continue;
+    if (!S.rebuild || S.err || small_sort(R.np, R.ncell_cap)) continue;
     const int i = T.atom0 + threadIdx.x;
     if (i >= R.np) continue;
     const int c = cell_of(R, S, i);
@@ -942,7 +959,7 @@ __global__ __launch_bounds__(1024) void k_rb_scan(EngineDev E, int r0, int nr) {
   for (int r = r0 + blockIdx.x; r < r0 + nr; r += gridDim.x) {
     const RepParams &R = E.reps[r];
     RepState &S = E.st[r];
-    if (!S.rebuild || S.err || R.np <= kSmallSort) continue;
+    if (!S.rebuild || S.err || small_sort(R.np, R.ncell_cap)) continue;
     const int nb = blockDim.x, tid = threadIdx.x;
     const int ncell = S.mcell[0] * S.mcell[1] * S.mcell[2];
     const int per = (ncell + nb - 1) / nb;
@@ -980,7 +997,7 @@ __global__ __launch_bounds__(kTile) void k_rb_scatter(EngineDev E, int t0, int n
     const Tile T = E.tiles[t];
     const RepParams &R = E.reps[T.rep];
     const RepState &S = E.st[T.rep];
-    if (!S.rebuild || S.err || R.np <= kSmallSort) continue;
+    if (!S.rebuild || S.err || small_sort(R.np, R.ncell_cap)) continue;
     const int i = T.atom0 + threadIdx.x;
     if (i >= R.np) continue;
     const int c = R.key[i];
@@ -997,7 +1014,7 @@ __global__ __launch_bounds__(kTile) void k_rb_rank(EngineDev E, int t0, int nt) 
     const Tile T = E.tiles[t];
     const RepParams &R = E.reps[T.rep];
     const RepState &S = E.st[T.rep];
-    if (!S.rebuild || S.err || R.np <= kSmallSort) continue;
+    if (!S.rebuild || S.err || small_sort(R.np, R.ncell_cap)) continue;
     if (S.brute) {
       const int i = T.atom0 + threadIdx.x;
       if (i < R.np) permute_atom(R, i, R.oid[i], S.parity);
@@ -1021,7 +1038,7 @@ __global__ __launch_bounds__(kTile) void k_rb_place(EngineDev E, int t0, int nt)
     const Tile T = E.tiles[t];
     const RepParams &R = E.reps[T.rep];
     const RepState &S = E.st[T.rep];
-    if (!S.rebuild || S.err || R.np <= kSmallSort) continue;
+    if (!S.rebuild || S.err || small_sort(R.np, R.ncell_cap)) continue;
     const int p = T.atom0 + threadIdx.x;
     if (p >= R.np) continue;
     place_atom(R, S, p);
@@ -1030,169 +1047,357 @@ __global__ __launch_bounds__(kTile) void k_rb_place(EngineDev E, int t0, int nt)
 
 // Full Verlet list (both directions) of each atom: the reference's pair test
 // (src/nd.f90:1358-1377,1404-1420) over the 27 surrounding cells (or all atoms in brute
-// mode).  Work unit: one wave per kListAtoms consecutive home atoms (so even a small
-// replica spreads over many waves).  The wave walks the candidates in canonical order
-// (stencil cell (dz,dy,dx) lexicographic, then slot), 64 per step with coalesced loads,
-// and appends accepted ones in order (ballot prefix).  Entries carry the cell/image
-// offset code used by the mixed force kernel.
-constexpr int kListAtoms = 4;
-constexpr int kListTasks = kTile / kListAtoms;   // wave tasks per tile
+// mode).  Work unit: one wave per chunk of up to kChunk home atoms of one cell, staged in
+// LDS.  The wave walks the cell's candidate stream (stencil cell (dz,dy,dx)
+// lexicographic, then slot) 64 candidates at a time, lane = candidate, and for each home
+// atom in turn tests the batch and appends the accepted candidates in lane order (ballot
+// prefix), so every list is in canonical order.
+//
+// Test.  The reference computes d2 from scaled positions with anint images.  When the
+// grid guarantees that the stencil representation of a pair is its minimum image for
+// every pair inside the cutoff ((M-2) w - 2.5e-4 box > cut on each axis; the 2.5e-4
+// covers the 0.4999 of the binning), d2 is formed instead from the cell-anchored
+// coordinates, candidate moved by its stencil offset: three subtractions and three
+// multiply-adds per pair.  The two forms differ by a few 1e-12 A^2; a pair whose anchored
+// d2 lies within 1e-9 c2 of the list cutoff (or of PTrMin for same-cell pairs) is decided
+// by the reference's own formula, so the accepted set is the reference's.  Other grids
+// (brute mode, M = 3 with w close to the cutoff) use the reference's formula for every
+// pair.  Entries carry the cell/image offset code of the mixed force kernel (ox = -dx).
+constexpr int kListWaves = 8;    // waves per workgroup (one workgroup per tile)
+constexpr int kChunk = 32;       // home atoms per wave task (16 in brute mode)
 
-__global__ __launch_bounds__(64) void k_rb_list(EngineDev E, int t0, int nt) {
-  __shared__ int s_lo[27], s_cnt[27], s_pref[28];
-  __shared__ double s_cut2[64];
-  const int lane = threadIdx.x;
-  // round-robin tasks (not XCD-contiguous): a rebuilding replica spreads over the chip
-  const int te = nt * kListTasks, ts = gridDim.x;
-  int cur_rep = -1;
-  for (int task = blockIdx.x; task < te; task += ts) {
-    const Tile T = E.tiles[t0 + task / kListTasks];
-    const RepParams &R = E.reps[T.rep];
-    RepState &S = E.st[T.rep];
-    if (!S.rebuild || S.err) continue;
-    const int h0 = T.atom0 + (task % kListTasks) * kListAtoms;
-    if (h0 >= R.np) continue;
-    const int h1 = min(h0 + kListAtoms, R.np);
-    if (T.rep != cur_rep) {
-      // the reference's per-pair list cutoff (src/nd.f90:1361,1375-1377), per type pair
-      const int ti = lane / 8 + 1, tj = lane % 8 + 1;
-      double c2 = -1.0;
-      if (ti <= R.ntypes && tj <= R.ntypes) {
-        const int pij = R.pairmat[(ti - 1) + (tj - 1) * R.ntypes];
-        const int pji = R.pairmat[(tj - 1) + (ti - 1) * R.ntypes];
-        const int pm = pij > pji ? pij : pji;
-        if (pm > 0) {
-          const double rcs = R.skin + R.rc_pair[pm - 1];
-          c2 = rcs * rcs;
-        }
+struct alignas(16) HomeRec {
+  double x, y, z;                // cell-anchored position
+  int ih, th;                    // slot, type
+  double px, py, pz;             // scaled position (reference test)
+  int ax, ay, az, pad;           // anchor
+};
+
+struct ListWave {
+  HomeRec home[kChunk];
+  int lo[27], pref[28];
+};
+
+__device__ inline unsigned long long ballot64(bool p) { return __ballot(p); }
+
+__device__ inline int lane_prefix(unsigned long long m) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+// The reference's d2 (scaled difference, anint image, box scale, sum in order).
+__device__ inline double ref_d2(double xi, double yi, double zi, double xj, double yj,
+                                double zj, double b0, double b1, double b2, double &nx,
+                                double &ny, double &nz) {
+  const double dpx = xi - xj, dpy = yi - yj, dpz = zi - zj;
+  nx = round(dpx);
+  ny = round(dpy);
+  nz = round(dpz);
+  const double r0 = b0 * (dpx - nx), r1 = b1 * (dpy - ny), r2 = b2 * (dpz - nz);
+  double d2 = r0 * r0;
+  d2 = d2 + r1 * r1;
+  d2 = d2 + r2 * r2;
+  return d2;
+}
+
+// list cutoff^2 per type pair (8 x 8, -1: no pair) of the workgroup's replica
+__shared__ double s_lcut2[64];
+
+// Global-memory view of a pointer read from the parameter block (global_* instead of
+// flat_* instructions).
+template <class T>
+__device__ inline __attribute__((address_space(1))) T *gmem(T *p) {
+  return (__attribute__((address_space(1))) T *)p;
+}
+
+struct ListCtx {
+  unsigned *list;
+  size_t kstride, tstride;       // entry stride (kTile tpa), tile stride of the list
+  int tpa, tsh, neimax;
+  double ptrmin;
+  const double *px, *py, *pz;
+  bool uni;                      // one list cutoff for every type pair
+  double c2u;
+};
+
+// Append the accepted lanes of batch mask `acc` to the list of home h (slot ih).
+__device__ inline void append(const ListCtx &C, int &nv, int h, int ih, unsigned long long acc,
+                              unsigned ent, bool bad_any) {
+  const int lane = threadIdx.x & 63;
+  int nov = __builtin_amdgcn_readlane(nv, h);
+  if (nov >= 0) {
+    const int nacc = __popcll(acc);
+    if (bad_any || nov + nacc > C.neimax) {
+      nov = -1;
+    } else if (nacc) {
+      if ((acc >> lane) & 1ull) {
+        auto bh = gmem(C.list + (size_t)(ih >> 7) * C.tstride + (ih & (kTile - 1)) * C.tpa);
+        const unsigned k = (unsigned)(nov + lane_prefix(acc));
+        bh[((k >> C.tsh) << (7 + C.tsh)) | (k & (unsigned)(C.tpa - 1))] = ent;
       }
-      s_cut2[lane] = c2;
-      cur_rep = T.rep;
+      nov += nacc;
     }
-    const int mx = S.mcell[0], my = S.mcell[1], mz = S.mcell[2];
-    const double b0 = S.box[0], b1 = S.box[1], b2 = S.box[2];
-    const int tpa = R.tpa;
-    const bool brute = S.brute != 0;
-    long long mine = 0;
-    int cell_loaded = -1;
-    for (int i = h0; i < h1; ++i) {
-      const double xi = R.px[i], yi = R.py[i], zi = R.pz[i];
-      const int ti = R.typ[i];
-      const int4 Ai = R.anchor[i];
-      int nrange;
-      if (brute) {
-        nrange = 1;
-        if (cell_loaded != 0 && lane == 0) {
-          s_lo[0] = 0;
-          s_cnt[0] = R.np;
-          s_pref[0] = 0;
-          s_pref[1] = R.np;
-        }
-        cell_loaded = 0;
-      } else {
-        nrange = 27;
-        if (Ai.w != cell_loaded) {
-          if (lane < 27) {
-            const int c = Ai.w;
-            const int cx = c % mx, cy = (c / mx) % my, cz = c / (mx * my);
-            const int dz = lane / 9 - 1, dy = (lane / 3) % 3 - 1, dx = lane % 3 - 1;
-            int ux = cx + dx, uy = cy + dy, uz = cz + dz;
-            ux += ux < 0 ? mx : (ux >= mx ? -mx : 0);
-            uy += uy < 0 ? my : (uy >= my ? -my : 0);
-            uz += uz < 0 ? mz : (uz >= mz ? -mz : 0);
-            const int cu = ux + uy * mx + uz * mx * my;
-            s_lo[lane] = R.cell_start[cu];
-            s_cnt[lane] = R.cell_start[cu + 1] - R.cell_start[cu];
-          }
-          __syncthreads();
-          if (lane == 0) {
-            s_pref[0] = 0;
-            for (int q = 0; q < 27; ++q) s_pref[q + 1] = s_pref[q] + s_cnt[q];
-          }
-          cell_loaded = Ai.w;
-        }
-      }
-      __syncthreads();  // one wave: orders the LDS writes above before the reads below
-      unsigned *L = R.list + ((size_t)(i / kTile) * R.list_chunks) * (kTile * tpa) + (i % kTile) * tpa;
-      int n = 0;
-      bool over = false;
-      // the 27 cells (or all atoms) as one candidate stream in canonical order; two
-      // batches of 64: every load issued before either is tested
-      const int total = s_pref[nrange];
-      for (int g0 = 0; g0 < total && !over; g0 += 128) {
-        int u[2], tj[2], qq[2];
-        double qx[2], qy[2], qz[2];
-        bool in[2];
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-          const int g = g0 + 64 * b + lane;
-          in[b] = g < total;
-          int q = 0;
-          if (!brute) {
-            // last q with s_pref[q] <= g (binary search over 28 prefix entries)
-            int lo_ = 0, hi_ = 27;
-            while (lo_ < hi_) {
-              const int mid = (lo_ + hi_ + 1) >> 1;
-              if (s_pref[mid] <= g) lo_ = mid; else hi_ = mid - 1;
-            }
-            q = lo_ > 26 ? 26 : lo_;
-          }
-          qq[b] = q;
-          u[b] = in[b] ? s_lo[q] + (g - s_pref[q]) : i;
-          qx[b] = R.px[u[b]];
-          qy[b] = R.py[u[b]];
-          qz[b] = R.pz[u[b]];
-          tj[b] = R.typ[u[b]];
-        }
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-          bool acc = false, bad = false;
-          unsigned ent = 0;
-          if (in[b] && u[b] != i) {
-            const double c2 = s_cut2[(ti - 1) * 8 + tj[b] - 1];
-            if (c2 >= 0.0) {
-              const double dpx = xi - qx[b], dpy = yi - qy[b], dpz = zi - qz[b];
-              const double nx = round(dpx), ny = round(dpy), nz = round(dpz);
-              const double r0 = b0 * (dpx - nx), r1 = b1 * (dpy - ny), r2 = b2 * (dpz - nz);
-              double d2 = r0 * r0;
-              d2 = d2 + r1 * r1;
-              d2 = d2 + r2 * r2;
-              const bool excl = !brute && qq[b] == 13 && d2 < R.ptrmin;
-              if (!excl && !(d2 > c2)) {
-                acc = true;
-                const int4 Au = R.anchor[u[b]];
-                const int ox = (Ai.x - Au.x) - (int)nx * mx;
-                const int oy = (Ai.y - Au.y) - (int)ny * my;
-                const int oz = (Ai.z - Au.z) - (int)nz * mz;
-                // offsets beyond one cell cannot occur for mcell >= 3 or brute grids
-                bad = ox < -1 || ox > 1 || oy < -1 || oy > 1 || oz < -1 || oz > 1;
-                const unsigned code = (unsigned)((ox + 1) | ((oy + 1) << 2) | ((oz + 1) << 4));
-                ent = (unsigned)u[b] | (code << kCodeShift);
-              }
-            }
-          }
-          const unsigned long long bal = __ballot(acc);
-          const int nacc = __popcll(bal);
-          if (__ballot(bad) || n + nacc > R.neimax) {
-            over = true;
-            break;
-          }
-          if (acc) {
-            const int k = n + __popcll(bal & ((1ull << lane) - 1ull));
-            L[(size_t)(k / tpa) * (kTile * tpa) + k % tpa] = ent;
-          }
-          n += nacc;
-        }
-      }
-      if (lane == 0) {
-        if (over) atomicMax(&S.err, NDG_ERR_NEIMAX);
-        R.nnb[i] = over ? 0 : n;
-      }
-      mine += over ? 0 : n;
-      __syncthreads();
-    }
-    if (lane == 0 && mine) atomicAdd((unsigned long long *)&S.nbr_sum, (unsigned long long)mine);
+    if (lane == h) nv = nov;
   }
+}
+
+__device__ long long list_chunk(const RepParams &R, RepState &S, ListWave &L, const ListCtx &C,
+                                bool anch, int lo, int cnt, int cell) {
+  const int lane = threadIdx.x & 63;
+  lo = __builtin_amdgcn_readfirstlane(lo);
+  cnt = __builtin_amdgcn_readfirstlane(cnt);
+  const bool brute = cell < 0;
+  const int mx = S.mcell[0], my = S.mcell[1], mz = S.mcell[2];
+  const double b0 = S.box[0], b1 = S.box[1], b2 = S.box[2];
+  const double w0 = b0 / mx, w1 = b1 / my, w2 = b2 / mz;
+  if (lane < cnt) {
+    const int ih = lo + lane;
+    const double4 xa = R.x64a[ih];
+    const int4 A = R.anchor[ih];
+    HomeRec &hr = L.home[lane];
+    hr.x = xa.x;
+    hr.y = xa.y;
+    hr.z = xa.z;
+    hr.ih = ih;
+    hr.th = bits_type(xa.w);
+    hr.px = C.px[ih];
+    hr.py = C.py[ih];
+    hr.pz = C.pz[ih];
+    hr.ax = A.x;
+    hr.ay = A.y;
+    hr.az = A.z;
+  }
+  int total;
+  if (brute) {
+    total = R.np;
+  } else {
+    int q_cnt = 0;
+    if (lane < 27) {
+      const int cx = cell % mx, cy = (cell / mx) % my, cz = cell / (mx * my);
+      const int dz = lane / 9 - 1, dy = (lane / 3) % 3 - 1, dx = lane % 3 - 1;
+      int ux = cx + dx, uy = cy + dy, uz = cz + dz;
+      ux += ux < 0 ? mx : (ux >= mx ? -mx : 0);
+      uy += uy < 0 ? my : (uy >= my ? -my : 0);
+      uz += uz < 0 ? mz : (uz >= mz ? -mz : 0);
+      const int cu = ux + uy * mx + uz * mx * my;
+      L.lo[lane] = R.cell_start[cu];
+      q_cnt = R.cell_start[cu + 1] - R.cell_start[cu];
+    }
+    int run = q_cnt;
+    for (int off = 1; off < 32; off <<= 1) {
+      const int v = __shfl_up(run, off);
+      if (lane >= off) run += v;
+    }
+    if (lane < 27) L.pref[lane + 1] = run;
+    if (lane == 0) L.pref[0] = 0;
+    total = __shfl(run, 26);
+  }
+  total = __builtin_amdgcn_readfirstlane(total);
+  wave_sync();
+  const double ptrmin = C.ptrmin;
+  const bool uni = C.uni;
+  const double c2u = C.c2u;
+  const double c2lo_u = c2u - 1e-9 * fabs(c2u), c2hi_u = c2u + 1e-9 * fabs(c2u);
+  // PTrMin band: the same width as the cutoff band of the largest cutoff
+  const double pmlo = ptrmin - 1e-9 * fabs(c2u), pmhi = ptrmin + 1e-9 * fabs(c2u);
+  const auto gx64a = gmem((const double *)R.x64a);
+  const auto gpx = gmem(C.px), gpy = gmem(C.py), gpz = gmem(C.pz);
+  int nv = 0;   // lane h: list length of home h, -1 after an overflow
+  for (int g0 = 0; g0 < total; g0 += 64) {
+    const int g = g0 + lane;
+    const bool inb = g < total;
+    int u = lo, q = 13;
+    if (inb) {
+      if (brute) {
+        u = g;
+      } else {
+        int a_ = 0, b_ = 26;   // last stencil cell q with pref[q] <= g
+        while (a_ < b_) {
+          const int mid = (a_ + b_ + 1) >> 1;
+          if (L.pref[mid] <= g) a_ = mid; else b_ = mid - 1;
+        }
+        q = a_;
+        u = L.lo[q] + (g - L.pref[q]);
+      }
+    }
+    const bool same = !brute && q == 13;
+    const unsigned long long valid = ballot64(inb);
+    const unsigned long long samem = ballot64(inb && same);
+    const double4 xu = make_double4(gx64a[4 * u], gx64a[4 * u + 1], gx64a[4 * u + 2],
+                                    gx64a[4 * u + 3]);
+    const int tj = bits_type(xu.w);
+    if (anch) {
+      const int dxq = q % 3 - 1, dyq = (q / 3) % 3 - 1, dzq = q / 9 - 1;
+      // the candidate in the frame of the home cell
+      const double ux_ = xu.x + dxq * w0, uy_ = xu.y + dyq * w1, uz_ = xu.z + dzq * w2;
+      const unsigned ent = (unsigned)u |
+          ((unsigned)((1 - dxq) | ((1 - dyq) << 2) | ((1 - dzq) << 4)) << kCodeShift);
+      for (int h = 0; h < cnt; ++h) {
+        const HomeRec &hr = L.home[h];
+        const int ih = lo + h;
+        const double dx = hr.x - ux_, dy = hr.y - uy_, dz = hr.z - uz_;
+        const double d2 = fma(dx, dx, fma(dy, dy, dz * dz));
+        double c2 = c2u, c2lo = c2lo_u, c2hi = c2hi_u;
+        if (!uni) {
+          c2 = s_lcut2[(__builtin_amdgcn_readfirstlane(hr.th) - 1) * 8 + tj - 1];
+          c2lo = c2 - 1e-9 * fabs(c2);
+          c2hi = c2 + 1e-9 * fabs(c2);
+        }
+        const unsigned long long live = valid & ~ballot64(u == ih) & ballot64(c2 >= 0.0);
+        const unsigned long long lo_m = ballot64(d2 < c2lo);
+        const unsigned long long hi_m = ballot64(d2 <= c2hi);
+        unsigned long long acc = live & lo_m, amb = live & hi_m & ~lo_m;
+        if (samem) {
+          const unsigned long long plo = ballot64(d2 < pmlo);
+          const unsigned long long phi = ballot64(d2 <= pmhi);
+          amb = (amb & ~(samem & plo)) | (acc & samem & phi & ~plo);
+          acc &= ~(samem & phi);
+        }
+        if (amb) {
+          bool ok = false;
+          if ((amb >> lane) & 1ull) {
+            double nx, ny, nz;
+            const double e2 = ref_d2(hr.px, hr.py, hr.pz, gpx[u], gpy[u], gpz[u], b0, b1,
+                                     b2, nx, ny, nz);
+            ok = !(same && e2 < ptrmin) && !(e2 > c2);
+          }
+          acc |= ballot64(ok);
+        }
+        append(C, nv, h, ih, acc, ent, false);
+      }
+    } else {
+      const double pxu = gpx[u], pyu = gpy[u], pzu = gpz[u];
+      const auto ga = gmem((const int *)R.anchor) + 4 * u;
+      const int4 Au = make_int4(ga[0], ga[1], ga[2], ga[3]);
+      for (int h = 0; h < cnt; ++h) {
+        const HomeRec &hr = L.home[h];
+        const int ih = lo + h;
+        double c2 = c2u;
+        if (!uni) c2 = s_lcut2[(__builtin_amdgcn_readfirstlane(hr.th) - 1) * 8 + tj - 1];
+        double nx, ny, nz;
+        const double d2 = ref_d2(hr.px, hr.py, hr.pz, pxu, pyu, pzu, b0, b1, b2, nx, ny, nz);
+        const bool ok = ((valid >> lane) & 1ull) && u != ih && c2 >= 0.0 &&
+                        !(same && d2 < ptrmin) && !(d2 > c2);
+        const int ox = (hr.ax - Au.x) - (int)nx * mx;
+        const int oy = (hr.ay - Au.y) - (int)ny * my;
+        const int oz = (hr.az - Au.z) - (int)nz * mz;
+        // offsets beyond one cell cannot occur for mcell >= 3 or brute grids
+        const bool bad = ok && (ox < -1 || ox > 1 || oy < -1 || oy > 1 || oz < -1 || oz > 1);
+        const unsigned ent = (unsigned)u |
+            ((unsigned)(((ox + 1) & 3) | (((oy + 1) & 3) << 2) | (((oz + 1) & 3) << 4)) << kCodeShift);
+        append(C, nv, h, ih, ballot64(ok), ent, ballot64(bad) != 0);
+      }
+    }
+  }
+  wave_sync();   // the home records are read by every lane above
+  const bool act = lane < cnt;
+  if (act) R.nnb[lo + lane] = nv < 0 ? 0 : nv;
+  if (ballot64(act && nv < 0) && lane == 0) atomicMax(&S.err, NDG_ERR_NEIMAX);
+  long long mine = act && nv > 0 ? nv : 0;
+  for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off);
+  return mine;
+}
+
+// One workgroup per tile: the chunks of the cells whose first slot lies in the tile (the
+// tile's own slots in brute mode), dealt to its waves in turn.  Workgroups of tiles that
+// need no rebuild exit at once, so the hardware spreads the rebuilding ones over the chip.
+__global__ __launch_bounds__(64 * kListWaves) void k_rb_list(EngineDev E, int t0, int nt) {
+  __shared__ ListWave s_wave[kListWaves];
+  __shared__ double s_cmax, s_cmin;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int t = t0 + blockIdx.x;
+  if (t >= t0 + nt) return;
+  const Tile T = E.tiles[t];
+  const RepParams &R = E.reps[T.rep];
+  RepState &S = E.st[T.rep];
+  // S.rebuild does not change during this kernel (S.err can be raised meanwhile)
+  if (!S.rebuild) return;
+  if (threadIdx.x < 64) {
+    // the reference's per-pair list cutoff (src/nd.f90:1361,1375-1377), per type pair
+    const int ti = lane / 8 + 1, tj = lane % 8 + 1;
+    double c2 = -1.0;
+    const bool in = ti <= R.ntypes && tj <= R.ntypes;
+    if (in) {
+      const int pij = R.pairmat[(ti - 1) + (tj - 1) * R.ntypes];
+      const int pji = R.pairmat[(tj - 1) + (ti - 1) * R.ntypes];
+      const int pm = pij > pji ? pij : pji;
+      if (pm > 0) {
+        const double rcs = R.skin + R.rc_pair[pm - 1];
+        c2 = rcs * rcs;
+      }
+    }
+    s_lcut2[lane] = c2;
+    double mxv = c2, mnv = in ? c2 : 1e300;
+    for (int off = 32; off > 0; off >>= 1) {
+      mxv = fmax(mxv, shfl_xor_d(mxv, off));
+      mnv = fmin(mnv, shfl_xor_d(mnv, off));
+    }
+    if (lane == 0) {
+      s_cmax = mxv;
+      s_cmin = mnv;
+    }
+  }
+  __syncthreads();
+  ListCtx C;
+  C.list = R.list;
+  C.tpa = R.tpa;
+  C.tsh = C.tpa == 1 ? 0 : (C.tpa == 2 ? 1 : (C.tpa == 4 ? 2 : 3));
+  C.kstride = (size_t)kTile * C.tpa;
+  C.tstride = (size_t)R.list_chunks * C.kstride;
+  C.neimax = R.neimax;
+  C.ptrmin = R.ptrmin;
+  C.px = R.px;
+  C.py = R.py;
+  C.pz = R.pz;
+  C.uni = s_cmax == s_cmin;
+  C.c2u = s_cmax;
+  ListWave &L = s_wave[wid];
+  long long mine = 0;
+  if (S.brute) {
+    constexpr int kb = kChunk / 2;
+    for (int k = wid; k < kTile / kb; k += kListWaves) {
+      const int lo = T.atom0 + kb * k;
+      if (lo < R.np) mine += list_chunk(R, S, L, C, false, lo, min(kb, R.np - lo), -1);
+    }
+  } else {
+    const int mx = S.mcell[0], my = S.mcell[1], mz = S.mcell[2];
+    const double cut = sqrt(s_cmax) * (1.0 + 1e-6);
+    const bool anch = (mx - 2) * (S.box[0] / mx) - 2.5e-4 * S.box[0] > cut &&
+                      (my - 2) * (S.box[1] / my) - 2.5e-4 * S.box[1] > cut &&
+                      (mz - 2) * (S.box[2] / mz) - 2.5e-4 * S.box[2] > cut;
+    const int ncell = mx * my * mz;
+    int c = __builtin_amdgcn_readfirstlane(R.anchor[T.atom0].w);
+    if (R.cell_start[c] < T.atom0) ++c;
+    int k = 0;
+    for (; c < ncell; ++c) {
+      const int c_lo = __builtin_amdgcn_readfirstlane(R.cell_start[c]);
+      if (c_lo >= T.atom0 + kTile) break;
+      const int c_hi = __builtin_amdgcn_readfirstlane(R.cell_start[c + 1]);
+      const int nch = (c_hi - c_lo + kChunk - 1) / kChunk;
+      for (int j = 0; j < nch; ++j, ++k) {
+        if (k % kListWaves != wid) continue;
+        // near-equal chunks of the cell
+        const int a0 = c_lo + (int)((long long)(c_hi - c_lo) * j / nch);
+        const int a1 = c_lo + (int)((long long)(c_hi - c_lo) * (j + 1) / nch);
+        mine += list_chunk(R, S, L, C, anch, a0, a1 - a0, c);
+      }
+    }
+  }
+  if (lane == 0 && mine) atomicAdd((unsigned long long *)&S.nbr_sum, (unsigned long long)mine);
+}
+
+// ------------------------------------------------------------------ bandwidth probe
+
+// Plain streaming copy (16 B per lane per iteration): the reference point for HBM
+// bandwidth and for scaling the FETCH_SIZE / WRITE_SIZE counters.
+__global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ src,
+                                              float4 *__restrict__ dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+void launch_copy(const float4 *src, float4 *dst, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy, dim3(256 * 32), dim3(256), 0, s, src, dst, n);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -1276,7 +1481,7 @@ void launch_rebuild(const EngineDev &E, int t0, int nt, int r0, int nr, bool any
     hipLaunchKernelGGL(k_rb_rank, dim3(g), dim3(kTile), 0, s, E, t0, nt);
     hipLaunchKernelGGL(k_rb_place, dim3(g), dim3(kTile), 0, s, E, t0, nt);
   }
-  hipLaunchKernelGGL(k_rb_list, dim3(grid_for(nt * kListTasks, 32)), dim3(64), 0, s, E, t0, nt);
+  hipLaunchKernelGGL(k_rb_list, dim3(nt), dim3(64 * kListWaves), 0, s, E, t0, nt);
 }
 
 }  // namespace ndg

@@ -109,6 +109,8 @@ def load_library():
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = C.c_int
+    L.ndg_copy_bandwidth.argtypes = [C.c_size_t, C.c_int, P(C.c_double)]
+    L.ndg_copy_bandwidth.restype = C.c_int
     L.ndg_footprint.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
     L.ndg_footprint.restype = C.c_size_t
     L.ndg_last_error.argtypes = []
@@ -371,3 +373,10 @@ class BatchEngine:
 
 def footprint(s: W.System, neimax: int) -> int:
     return int(load_library().ndg_footprint(s.np, s.ntypes, s.npairs, s.ptsize, int(neimax)))
+
+
+def copy_bandwidth(nbytes=1 << 30, iters=20) -> float:
+    """Measured device copy bandwidth in GB/s (read + write bytes)."""
+    g = C.c_double(0)
+    _check(load_library().ndg_copy_bandwidth(C.c_size_t(nbytes), int(iters), C.byref(g)))
+    return g.value

@@ -1,0 +1,102 @@
+"""Per-kernel summary of a rocprofv3 kernel trace (CSV) over the batched stepping window.
+
+    python -m ndyn_omp_amd.trace_summary <prefix>_kernel_trace.csv [--pmc fetch.csv write.csv]
+
+The window runs from the first to the last launch of `k_drift_mixed` whose grid is the
+largest seen (the batched launches over every replica), which leaves out admission-time
+list builds and the single-replica launches of the force-error check.  For each kernel it
+prints launches, total, average and per-step time (steps = batched drift launches).
+
+With --pmc, it also prints FETCH_SIZE / WRITE_SIZE per dispatch (KiB) of each kernel from
+two `rocprofv3 --pmc` counter collections.
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import statistics
+from collections import defaultdict
+
+
+def load_trace(path):
+    rows = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            rows.append(dict(name=r["Kernel_Name"], t0=int(r["Start_Timestamp"]),
+                             t1=int(r["End_Timestamp"]), grid=int(r["Grid_Size_X"]),
+                             wg=int(r["Workgroup_Size_X"])))
+    rows.sort(key=lambda r: r["t0"])
+    return rows
+
+
+def window(rows, anchor="k_drift_mixed"):
+    drifts = [r for r in rows if r["name"] == anchor]
+    if not drifts:
+        return rows, 0, 0
+    gmax = max(r["grid"] for r in drifts)
+    big = [r for r in drifts if r["grid"] == gmax]
+    t_lo, t_hi = big[0]["t0"], big[-1]["t1"]
+    last_kick = [r for r in rows if r["name"] == "k_kick_mixed" and r["grid"] == gmax
+                 and r["t0"] >= t_hi]
+    if last_kick:
+        t_hi = last_kick[0]["t1"]
+    return [r for r in rows if r["t0"] >= t_lo and r["t1"] <= t_hi], len(big), gmax
+
+
+def summarize(rows, steps):
+    by = defaultdict(list)
+    for r in rows:
+        by[r["name"]].append(r)
+    out = []
+    for name, rs in by.items():
+        d = [(r["t1"] - r["t0"]) * 1e-6 for r in rs]
+        full = [x for x, r in zip(d, rs) if r["grid"] == max(q["grid"] for q in rs)]
+        out.append(dict(name=name, calls=len(d), total_ms=sum(d), avg_ms=statistics.mean(d),
+                        full_grid=max(r["grid"] for r in rs), full_calls=len(full),
+                        full_avg_ms=statistics.mean(full),
+                        per_step_ms=sum(d) / steps if steps else 0.0))
+    out.sort(key=lambda x: -x["total_ms"])
+    return out
+
+
+def load_pmc(path, counter):
+    vals = defaultdict(list)
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r.get("Counter_Name") == counter:
+                vals[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+    return vals
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("trace")
+    ap.add_argument("--anchor", default="k_drift_mixed")
+    ap.add_argument("--pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
+    a = ap.parse_args()
+    rows = load_trace(a.trace)
+    win, steps, gmax = window(rows, a.anchor)
+    span = (win[-1]["t1"] - win[0]["t0"]) * 1e-6 if win else 0.0
+    print(f"window: {steps} batched steps ({a.anchor} grid {gmax}), {span:.2f} ms of GPU time "
+          f"span, {span / max(steps, 1):.4f} ms/step")
+    print(f"{'kernel':28s} {'calls':>7s} {'total ms':>10s} {'avg ms':>9s} {'full-grid avg':>14s} "
+          f"{'ms/step':>9s}")
+    busy = 0.0
+    for k in summarize(win, steps):
+        busy += k["total_ms"]
+        print(f"{k['name'][:28]:28s} {k['calls']:7d} {k['total_ms']:10.3f} {k['avg_ms']:9.4f} "
+              f"{k['full_avg_ms']:9.4f} ({k['full_calls']:3d}) {k['per_step_ms']:9.4f}")
+    print(f"{'(sum of kernel time)':28s} {'':7s} {busy:10.3f} {'':9s} {'':14s} "
+          f"{busy / max(steps, 1):9.4f}")
+    if a.pmc:
+        fetch = load_pmc(a.pmc[0], "FETCH_SIZE")
+        write = load_pmc(a.pmc[1], "WRITE_SIZE")
+        print("\nper-dispatch HBM counters (KiB): kernel, dispatches, FETCH_SIZE avg, WRITE_SIZE avg")
+        for name in sorted(set(fetch) | set(write)):
+            f, w = fetch.get(name, []), write.get(name, [])
+            print(f"{name[:28]:28s} {len(f):5d} {statistics.mean(f) if f else 0:14.1f} "
+                  f"{statistics.mean(w) if w else 0:14.1f}")
+
+
+if __name__ == "__main__":
+    main()

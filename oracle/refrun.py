@@ -66,8 +66,9 @@ def run_harness(mode, s, timeout=3600, **kw):
                    OMP_PROC_BIND=os.environ.get("OMP_PROC_BIND", "close"))
         p = subprocess.run([HARNESS, fin, fout], cwd=d, env=env, capture_output=True,
                            text=True, timeout=timeout)
-        if p.returncode != 0:
-            raise RuntimeError(p.stdout + p.stderr)
+        # a Fortran STOP (e.g. 'too many neighbours in list') exits with status 0
+        if p.returncode != 0 or "STOP" in p.stderr:
+            raise RuntimeError(p.stdout[-2000:] + p.stderr)
         return open(fout, "rb").read(), p.stdout
 
 
