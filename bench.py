@@ -222,6 +222,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=100)
     ap.add_argument("--graph-steps", type=int, default=0)
     ap.add_argument("--tpa", type=int, default=0)
+    ap.add_argument("--skin", type=float, default=None,
+                    help="override the workload's list skin (A)")
     ap.add_argument("--cpu-budget-s", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -240,6 +242,9 @@ def main():
         serve_bench(args)
         return
     systems, desc, cfg = workload_systems(args.workload)
+    if args.skin is not None:
+        cfg["skin"] = args.skin
+        desc += f" (skin {args.skin} A)"
     atoms = sum(s.np for s in systems)
     log(f"bench: workload {args.workload}: {len(systems)} systems, {atoms} atoms")
 

@@ -256,7 +256,7 @@ static void grid_for_box(const double box[3], double rcut, double skin, int m[3]
 static size_t carve(Slot &s, int path, char *block) {
   const int np = s.np;
   const int ntiles = (np + kTile - 1) / kTile;
-  const int C = (s.neimax + s.tpa - 1) / s.tpa;
+  const int C = (s.neimax + s.tpa - 1) / s.tpa + kListPad;
   Layout L;
   size_t o_mass = L.add(sizeof(double) * s.ntypes);
   size_t o_pm = L.add(sizeof(int) * s.ntypes * s.ntypes);
@@ -270,6 +270,7 @@ static size_t carve(Slot &s, int path, char *block) {
   size_t o_f32b = L.add(sizeof(float4) * np);
   size_t o_x64a = L.add(sizeof(double4) * np);
   size_t o_x32b = L.add(sizeof(float4) * np);
+  size_t o_xq = L.add(sizeof(uint4) * np);
   size_t o_anc = L.add(sizeof(int4) * np);
   size_t o_typ = L.add(sizeof(int) * np);
   size_t o_oid = L.add(sizeof(int) * np);
@@ -300,6 +301,7 @@ static size_t carve(Slot &s, int path, char *block) {
     P.f32b = (float4 *)(block + o_f32b);
     P.x64a = (double4 *)(block + o_x64a);
     P.x32b = (float4 *)(block + o_x32b);
+    P.xq = (uint4 *)(block + o_xq);
     P.anchor = (int4 *)(block + o_anc);
     P.typ = (int *)(block + o_typ);
     P.oid = (int *)(block + o_oid);
@@ -744,6 +746,7 @@ static int migrate_block(ndg_engine *e, const RepParams &o, const RepParams &n, 
   HIPCHK(hipMemcpyAsync(n.f32a, o.f32a, 16 * (size_t)np, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(n.f32b, o.f32b, 16 * (size_t)np, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(n.x64a, o.x64a, 32 * (size_t)np, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(n.xq, o.xq, 16 * (size_t)np, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(n.x32b, o.x32b, 16 * (size_t)np, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(n.anchor, o.anchor, 16 * (size_t)np, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(n.typ, o.typ, 4 * (size_t)np, hipMemcpyDeviceToDevice, st));

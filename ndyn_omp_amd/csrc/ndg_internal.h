@@ -18,6 +18,7 @@ namespace ndg {
 constexpr int kTile = 128;        // atoms per tile (unit of work of every per-atom kernel)
 constexpr int kCodeShift = 26;    // list entry = j | code << 26, code = image/cell offset
 constexpr unsigned kIndexMask = (1u << kCodeShift) - 1u;
+constexpr int kListPad = 8;       // spare list chunks per tile (unrolled reads past the end)
 constexpr int kRing = 1024;       // thermo records kept per replica between host drains
 constexpr int kPartials = 12;     // doubles per tile: ke, pe, vir[6], d1, d2, spare[2]
 
@@ -66,6 +67,7 @@ struct RepParams {
   float4 *f32a, *f32b;      // mixed path force (xyz) + per-atom energy (w), two parities
   double4 *x64a;            // mixed path cell-anchored coordinates (xyz, A) + type bits (w)
   float4 *x32b;             // FP32 copy of x64a at the last build (displacement check)
+  uint4 *xq;                // 16-byte copy of x64a gathered by the mixed force kernel
   int4 *anchor;             // integer anchor per axis (image*M + cell) and linear cell (w)
   int *typ, *oid, *nnb;
   unsigned *list;

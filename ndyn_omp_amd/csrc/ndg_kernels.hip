@@ -30,6 +30,13 @@ __device__ inline void tile_range(int t0, int nt, int &begin, int &end, int &ste
   step = nwx;
 }
 
+// Global-memory view of a pointer read from the parameter block (global_* instead of
+// flat_* instructions).
+template <class T>
+__device__ inline __attribute__((address_space(1))) T *gmem(T *p) {
+  return (__attribute__((address_space(1))) T *)p;
+}
+
 __device__ inline double shfl_xor_d(double v, int m) {
   int2 a = *reinterpret_cast<int2 *>(&v);
   a.x = __shfl_xor(a.x, m);
@@ -104,6 +111,46 @@ __device__ inline double anchored(double p, int c, int m, double w) {
 __device__ inline double type_bits(int t) { return __longlong_as_double((long long)t); }
 __device__ inline int bits_type(double w) { return (int)__double_as_longlong(w); }
 
+// 16-byte form of a cell-anchored position, read by the mixed force kernel (one gather
+// per neighbour instead of two).  Each coordinate x is the FP32 value h = fl32(x) plus the
+// residual x - h (exact in FP64) in units of q = 2^(e-33), where 2^e >= w/2 + skin + 1 A
+// bounds |x| between rebuilds: |x - h| <= ulp32(x)/2 <= 2^(e-25) = 256 q.  Word 3 holds
+// the residuals as signed 10/10/9-bit fields and the type - 1 in its top 3 bits; the
+// decoded position h + r q is within q/2 ~ 1e-9 A of x.
+__device__ inline double xq_unit(const RepState &S, double skin) {
+  double w = 0.0;
+  for (int a = 0; a < 3; ++a) w = fmax(w, S.box[a] / S.mcell[a]);
+  int e;
+  frexp(0.5 * w + skin + 1.0, &e);   // 2^(e-1) <= bound < 2^e
+  return ldexp(1.0, e - 33);
+}
+
+__device__ inline uint4 xq_pack(double x, double y, double z, int ty, double q) {
+  const double c[3] = {x, y, z};
+  unsigned h[3];
+  int r[3];
+  const int lim[3] = {511, 511, 255};
+  for (int a = 0; a < 3; ++a) {
+    const float f = (float)c[a];
+    h[a] = __float_as_uint(f);
+    int k = (int)rint((c[a] - (double)f) / q);
+    r[a] = k < -lim[a] ? -lim[a] : (k > lim[a] ? lim[a] : k);
+  }
+  const unsigned w = ((unsigned)r[0] & 0x3FFu) | (((unsigned)r[1] & 0x3FFu) << 10) |
+                     (((unsigned)r[2] & 0x1FFu) << 20) | ((unsigned)(ty - 1) << 29);
+  return make_uint4(h[0], h[1], h[2], w);
+}
+
+// decoded coordinates (FP64) and type of a packed position
+__device__ inline void xq_unpack(const uint4 v, double q, double &x, double &y, double &z,
+                                 int &ty) {
+  const int rx = (int)(v.w << 22) >> 22, ry = (int)(v.w << 12) >> 22, rz = (int)(v.w << 3) >> 23;
+  x = fma((double)rx, q, (double)__uint_as_float(v.x));
+  y = fma((double)ry, q, (double)__uint_as_float(v.y));
+  z = fma((double)rz, q, (double)__uint_as_float(v.z));
+  ty = (int)(v.w >> 29) + 1;
+}
+
 // Reference binning (src/nd.f90:1240-1246): sr = p - anint(p); I = int((sr+0.4999) M).
 __device__ inline void bin_axis(double p, int m, int brute, int &k, int &c) {
   double n = round(p);
@@ -159,6 +206,7 @@ __global__ __launch_bounds__(kTile) void k_drift_mixed(EngineDev E, int t0, int 
       const double yl = anchored(py, A.y, m1, S.box[1] / m1);
       const double zl = anchored(pz, A.z, m2, S.box[2] / m2);
       R.x64a[i] = make_double4(xl, yl, zl, type_bits(ty));
+      R.xq[i] = xq_pack(xl, yl, zl, ty, xq_unit(S, R.skin));
       const float4 B = R.x32b[i];
       const double dx = (double)((float)xl - B.x), dy = (double)((float)yl - B.y),
                    dz = (double)((float)zl - B.z);
@@ -236,63 +284,98 @@ struct ForceShape {
   static constexpr int kNSub = kThreads / kSub;            // tiles per pass
 };
 
-template <int TPA, bool FULL, bool LDS_TAB>
+template <int TPA, bool FULL, bool LDS_TAB, bool ONE>
 __device__ inline void force_atom(const RepParams &R, int i, int sub, const double4 *shift,
                                   const double *rc2, const float *vdr2, const int *pm,
                                   const int *lt, const float *stab, const unsigned *L,
-                                  double &fx, double &fy, double &fz, double (&acc)[7]) {
+                                  double qunit, double &fx, double &fy, double &fz,
+                                  double (&acc)[7]) {
   constexpr int U = kUnroll;
-  const double4 xi = R.x64a[i];
-  const int ti = bits_type(xi.w);
-  const int n = R.nnb[i];
+  static_assert(U <= kListPad, "the list holds kListPad spare chunks for the unrolled reads");
+  const auto XQ = gmem((const unsigned *)R.xq);
+  const auto Lg = gmem(L);
+  const auto TF = gmem((const float *)R.tabF2);
+  const auto TP = gmem((const float *)R.tabP2);
+  double xix, xiy, xiz;
+  int ti;
+  xq_unpack(make_uint4(XQ[4 * i], XQ[4 * i + 1], XQ[4 * i + 2], XQ[4 * i + 3]), qunit, xix, xiy,
+            xiz, ti);
+  const int n = gmem(R.nnb)[i];
   const int nmine = n > sub ? (n - sub + TPA - 1) / TPA : 0;  // entries of this lane
-  const double4 *X = R.x64a;
-  const float2 *TF = R.tabF2;
-  const float2 *TP = R.tabP2;
-  const bool one = R.npairs == 1;
   const int nt_ = R.ntypes;
   const int tab = R.ptsize + 1;
   const int kmax = R.ptsize;
   const double ptrmin = R.ptrmin;
+  // single pair potential: its parameters stay in registers
+  const double rc2_0 = rc2[0];
+  const float vdr2_0 = vdr2[0];
+  const int lt_0 = LDS_TAB ? lt[0] : 0;
   // absent entries read the atom itself with a zero offset; the select drops them
   const unsigned pad = (unsigned)i | (21u << kCodeShift);
+  // the list entries of the next iteration are loaded one iteration ahead, so only the
+  // position gathers stand between an iteration's loads and its arithmetic; the list has
+  // kListPad spare chunks, so reads past the end stay in bounds
+  unsigned nxt[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) nxt[u] = nmine > 0 ? Lg[(size_t)u * (kTile * TPA)] : pad;
   for (int c = 0; c < nmine; c += U) {
     unsigned e[U];
-    double4 xj[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) e[u] = (c + u < nmine) ? L[(size_t)(c + u) * (kTile * TPA)] : pad;
+    for (int u = 0; u < U; ++u) e[u] = (c + u < nmine) ? nxt[u] : pad;
+    uint4 pj[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) xj[u] = X[e[u] & kIndexMask];
+    for (int u = 0; u < U; ++u) {
+      const unsigned j = e[u] & kIndexMask;
+      pj[u] = make_uint4(XQ[4 * j], XQ[4 * j + 1], XQ[4 * j + 2], XQ[4 * j + 3]);
+    }
+    double jx[U], jy[U], jz[U];
+    int tj[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) xq_unpack(pj[u], qunit, jx[u], jy[u], jz[u], tj[u]);
+    if (c + U < nmine) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) nxt[u] = Lg[(size_t)(c + U + u) * (kTile * TPA)];
+    }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const double4 sft = shift[e[u] >> kCodeShift];
-      const double dx = (xi.x - xj[u].x) + sft.x;
-      const double dy = (xi.y - xj[u].y) + sft.y;
-      const double dz = (xi.z - xj[u].z) + sft.z;
+      const double dx = (xix - jx[u]) + sft.x;
+      const double dy = (xiy - jy[u]) + sft.y;
+      const double dz = (xiz - jz[u]) + sft.z;
       const double d2 = fma(dx, dx, fma(dy, dy, dz * dz));
       int pid = 0;
-      if (!one) pid = pm[(ti - 1) + (bits_type(xj[u].w) - 1) * nt_] - 1;
-      const bool ok = (c + u < nmine) && !(d2 > rc2[pid]);
-      const float rk = (float)(d2 - ptrmin) * vdr2[pid] + 1.0f;
+      double rc2p = rc2_0;
+      float vd = vdr2_0;
+      int ltp = lt_0;
+      if (!ONE) {
+        pid = pm[(ti - 1) + (tj[u] - 1) * nt_] - 1;
+        rc2p = rc2[pid];
+        vd = vdr2[pid];
+        if (LDS_TAB) ltp = lt[pid];
+      }
+      const bool ok = (c + u < nmine) && !(d2 > rc2p);
+      const float rk = (float)(d2 - ptrmin) * vd + 1.0f;
       int k = (int)rk;
       k = k < 1 ? 1 : (k > kmax ? kmax : k);
       const float w = rk - (float)k;
       float fv;
       if (LDS_TAB) {
-        const float *Tl = stab + lt[pid];
+        const float *Tl = stab + ltp;
         const float F0 = Tl[k - 1], F1 = Tl[k];
         fv = fmaf(w, F1 - F0, F0);
       } else {
-        const float2 tf = TF[pid * tab + k - 1];
-        fv = fmaf(w, tf.y - tf.x, tf.x);
+        const int q = 2 * (pid * tab + k - 1);
+        const float F0 = TF[q], F1 = TF[q + 1];
+        fv = fmaf(w, F1 - F0, F0);
       }
       const double df = ok ? (double)(fv * __builtin_amdgcn_rcpf((float)d2)) : 0.0;
       fx = fma(dx, df, fx);
       fy = fma(dy, df, fy);
       fz = fma(dz, df, fz);
       if (FULL) {
-        const float2 tp = TP[pid * tab + k - 1];
-        const double pv = ok ? (double)fmaf(w, tp.y - tp.x, tp.x) : 0.0;
+        const int q = 2 * (pid * tab + k - 1);
+        const float P0 = TP[q], P1 = TP[q + 1];
+        const double pv = ok ? (double)fmaf(w, P1 - P0, P0) : 0.0;
         const double hdf = 0.5 * df;
         acc[0] += 0.5 * pv;
         acc[1] += dx * dx * hdf;
@@ -356,16 +439,25 @@ __global__ __launch_bounds__(ForceShape<TPA>::kThreads) void k_force_mixed(Engin
     }
     __syncthreads();
     const int i = T.atom0 + a;
+    const double qunit = have ? xq_unit(S, R.skin) : 1.0;
     double fx = 0.0, fy = 0.0, fz = 0.0;
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};  // e_i, vir xx yy zz xy xz yz (FULL)
     if (have && i < R.np) {
       const unsigned *L = R.list + ((size_t)T.ltile * R.list_chunks) * (kTile * TPA) + lt;
-      if (sh_uselds[sg])
-        force_atom<TPA, FULL, true>(R, i, sub, sh_shift[sg], sh_rc2[sg], sh_vdr2[sg], sh_pm[sg],
-                                    sh_lt[sg], sh_tab, L, fx, fy, fz, acc);
+      // both choices are the same for every wave of the tile
+      const bool lds = sh_uselds[sg] != 0, one = R.npairs == 1;
+      if (lds && one)
+        force_atom<TPA, FULL, true, true>(R, i, sub, sh_shift[sg], sh_rc2[sg], sh_vdr2[sg],
+                                          sh_pm[sg], sh_lt[sg], sh_tab, L, qunit, fx, fy, fz, acc);
+      else if (lds)
+        force_atom<TPA, FULL, true, false>(R, i, sub, sh_shift[sg], sh_rc2[sg], sh_vdr2[sg],
+                                           sh_pm[sg], sh_lt[sg], sh_tab, L, qunit, fx, fy, fz, acc);
+      else if (one)
+        force_atom<TPA, FULL, false, true>(R, i, sub, sh_shift[sg], sh_rc2[sg], sh_vdr2[sg],
+                                           sh_pm[sg], sh_lt[sg], sh_tab, L, qunit, fx, fy, fz, acc);
       else
-        force_atom<TPA, FULL, false>(R, i, sub, sh_shift[sg], sh_rc2[sg], sh_vdr2[sg], sh_pm[sg],
-                                     sh_lt[sg], sh_tab, L, fx, fy, fz, acc);
+        force_atom<TPA, FULL, false, false>(R, i, sub, sh_shift[sg], sh_rc2[sg], sh_vdr2[sg],
+                                            sh_pm[sg], sh_lt[sg], sh_tab, L, qunit, fx, fy, fz, acc);
     }
     if (TPA > 1) {
 #pragma unroll
@@ -797,6 +889,7 @@ __device__ inline void place_atom(const RepParams &R, const RepState &S, int p) 
                                  anchored(pz, cza, S.mcell[2], S.box[2] / S.mcell[2]),
                                  type_bits(ty));
   R.x64a[p] = x;
+  R.xq[p] = xq_pack(x.x, x.y, x.z, ty, xq_unit(S, R.skin));
   R.x32b[p] = make_float4((float)x.x, (float)x.y, (float)x.z, 0.f);
 }
 
@@ -1102,12 +1195,6 @@ __device__ inline double ref_d2(double xi, double yi, double zi, double xj, doub
 // list cutoff^2 per type pair (8 x 8, -1: no pair) of the workgroup's replica
 __shared__ double s_lcut2[64];
 
-// Global-memory view of a pointer read from the parameter block (global_* instead of
-// flat_* instructions).
-template <class T>
-__device__ inline __attribute__((address_space(1))) T *gmem(T *p) {
-  return (__attribute__((address_space(1))) T *)p;
-}
 
 struct ListCtx {
   unsigned *list;
