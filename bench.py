@@ -29,9 +29,10 @@ from ndyn_omp_amd import engine as G  # noqa: E402
 from ndyn_omp_amd import workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per force launch on the default workload from rocprofv3 FETCH_SIZE + WRITE_SIZE,
-# scaled by the same counters on a copy kernel of known size (profiles/); None until measured
-TRAFFIC_BYTES_PER_LAUNCH = None
+# HBM bytes per k_force_mixed launch, per workload, from rocprofv3 FETCH_SIZE + WRITE_SIZE
+# scaled by the same counters on k_copy of known size (1 GiB read + 1 GiB written):
+# FETCH_SIZE x 2.000, WRITE_SIZE x 1.000.  Source: profiles/k_force_mixed_c3_traffic.txt.
+TRAFFIC_BYTES_PER_LAUNCH = {"c3": 8.9385e8}
 METRIC = "atom-steps/s vs plain FP64 GPU path at <=1e-5 rel force err; p50/p99 job latency"
 
 
@@ -205,7 +206,8 @@ def roofline(mixed, copy_gbs):
     bpa = force_bytes_per_atom(mixed["mean_nbr"])
     achieved = bpa * mixed["atoms"] / (mixed["force_ms"] * 1e-3) / 1e9
     return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=TRAFFIC_BYTES_PER_LAUNCH,
+                frac=round(achieved / HBM_PEAK_GBS, 4),
+                traffic=TRAFFIC_BYTES_PER_LAUNCH.get(mixed.get("workload")),
                 kernel="k_force_mixed", bytes_per_atom=round(bpa, 1),
                 atoms_per_launch=mixed["atoms"], avg_launch_ms=round(mixed["force_ms"], 5),
                 measured_copy_gbs=round(copy_gbs, 1),
@@ -250,6 +252,9 @@ def main():
 
     eng, slots, mixed = run_engine(systems, cfg, G.PATH_MIXED, args.warmup, args.steps,
                                    args.timing_steps, args.graph_steps, args.tpa)
+    # the measured traffic belongs to the default settings of a workload only
+    default_run = args.skin is None and args.tpa == 0
+    mixed["workload"] = args.workload if default_run else None
     log(f"bench: mixed {mixed['atom_steps_per_s']:.4e} atom-steps/s, "
         f"{mixed['ms_per_step']:.3f} ms/step, force {mixed.get('force_ms', 0):.4f} ms, "
         f"rebuilds/step {mixed['rebuilds_per_step']:.2f} "

@@ -8,7 +8,9 @@ list builds and the single-replica launches of the force-error check.  For each 
 prints launches, total, average and per-step time (steps = batched drift launches).
 
 With --pmc, it also prints FETCH_SIZE / WRITE_SIZE per dispatch (KiB) of each kernel from
-two `rocprofv3 --pmc` counter collections.
+two `rocprofv3 --pmc` counter collections, and HBM bytes per dispatch scaled by what the
+same counters read for `k_copy`, which reads and writes --copy-bytes per dispatch
+(ndg_copy_bandwidth's 1 GiB copy): factor = known bytes / counter bytes.
 """
 from __future__ import annotations
 
@@ -60,12 +62,25 @@ def summarize(rows, steps):
 
 
 def load_pmc(path, counter):
-    vals = defaultdict(list)
+    """Counter value (summed over its instances) per dispatch: {kernel: [(grid, value)]}."""
+    per = defaultdict(float)
+    meta = {}
     with open(path) as f:
         for r in csv.DictReader(f):
             if r.get("Counter_Name") == counter:
-                vals[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+                key = (r["Kernel_Name"], r["Dispatch_Id"])
+                per[key] += float(r["Counter_Value"])
+                meta[key] = int(r["Grid_Size"])
+    vals = defaultdict(list)
+    for (name, _), v in per.items():
+        vals[name].append((meta[(name, _)], v))
     return vals
+
+
+def full_grid_mean(pairs):
+    g = max(p[0] for p in pairs)
+    v = [p[1] for p in pairs if p[0] == g]
+    return g, statistics.mean(v), len(v)
 
 
 def main():
@@ -73,6 +88,7 @@ def main():
     ap.add_argument("trace")
     ap.add_argument("--anchor", default="k_drift_mixed")
     ap.add_argument("--pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
+    ap.add_argument("--copy-bytes", type=float, default=float(1 << 30))
     a = ap.parse_args()
     rows = load_trace(a.trace)
     win, steps, gmax = window(rows, a.anchor)
@@ -91,11 +107,23 @@ def main():
     if a.pmc:
         fetch = load_pmc(a.pmc[0], "FETCH_SIZE")
         write = load_pmc(a.pmc[1], "WRITE_SIZE")
-        print("\nper-dispatch HBM counters (KiB): kernel, dispatches, FETCH_SIZE avg, WRITE_SIZE avg")
+        kf = kw = None
+        if "k_copy" in fetch and "k_copy" in write:
+            kf = a.copy_bytes / (full_grid_mean(fetch["k_copy"])[1] * 1024.0)
+            kw = a.copy_bytes / (full_grid_mean(write["k_copy"])[1] * 1024.0)
+            print(f"\ncopy calibration ({a.copy_bytes:.0f} B read + written per k_copy dispatch): "
+                  f"FETCH_SIZE x {kf:.3f}, WRITE_SIZE x {kw:.3f}")
+        print("\nlargest-grid dispatches: kernel, grid, dispatches, FETCH_SIZE KiB, WRITE_SIZE KiB"
+              + (", HBM bytes (calibrated)" if kf else ""))
         for name in sorted(set(fetch) | set(write)):
-            f, w = fetch.get(name, []), write.get(name, [])
-            print(f"{name[:28]:28s} {len(f):5d} {statistics.mean(f) if f else 0:14.1f} "
-                  f"{statistics.mean(w) if w else 0:14.1f}")
+            if name not in fetch or name not in write:
+                continue
+            g, fv, n = full_grid_mean(fetch[name])
+            _, wv, _ = full_grid_mean(write[name])
+            line = f"{name[:28]:28s} {g:9d} {n:4d} {fv:14.1f} {wv:14.1f}"
+            if kf:
+                line += f" {fv * 1024 * kf + wv * 1024 * kw:16.4e}"
+            print(line)
 
 
 if __name__ == "__main__":
