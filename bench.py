@@ -32,7 +32,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_force_mixed launch, per workload, from rocprofv3 FETCH_SIZE + WRITE_SIZE
 # scaled by the same counters on k_copy of known size (1 GiB read + 1 GiB written):
 # FETCH_SIZE x 2.000, WRITE_SIZE x 1.000.  Source: profiles/k_force_mixed_c3_traffic.txt.
-TRAFFIC_BYTES_PER_LAUNCH = {"c3": 8.9385e8}
+TRAFFIC_BYTES_PER_LAUNCH = {}
 METRIC = "atom-steps/s vs plain FP64 GPU path at <=1e-5 rel force err; p50/p99 job latency"
 
 
@@ -113,10 +113,16 @@ def make_rp(cfg, nsteps):
                         isobaric=cfg.get("iso_p"))
 
 
-def run_engine(systems, cfg, path, warmup, steps, timing_steps, graph_steps=0, tpa=0):
+def run_engine(systems, cfg, path, warmup, steps, timing_steps, graph_steps=0, tpa=0,
+               table=None):
     eng = G.BatchEngine(path=path, tpa=tpa, graph_steps=graph_steps)
     total = warmup + steps + timing_steps
-    slots = [eng.admit(s, make_rp(cfg, total)) for s in systems]
+    if table is not None:
+        table.apply(eng)
+        slots = [eng.admit(s, make_rp(dict(cfg, skin=table.skin_for(s.np, cfg["skin"])), total))
+                 for s in systems]
+    else:
+        slots = [eng.admit(s, make_rp(cfg, total)) for s in systems]
     atoms = sum(s.np for s in systems)
     eng.step(warmup)
     t0 = time.perf_counter()
@@ -226,6 +232,8 @@ def main():
     ap.add_argument("--tpa", type=int, default=0)
     ap.add_argument("--skin", type=float, default=None,
                     help="override the workload's list skin (A)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (TPA per size class; skin too for the mix)")
     ap.add_argument("--cpu-budget-s", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -250,10 +258,22 @@ def main():
     atoms = sum(s.np for s in systems)
     log(f"bench: workload {args.workload}: {len(systems)} systems, {atoms} atoms")
 
+    table, tune_s = None, 0.0
+    if not args.no_tune and args.tpa == 0:
+        from ndyn_omp_amd import autotune as AT
+        # the replica mix leaves the skin open; the single systems keep their configured one
+        skins = AT.DEFAULT_SKINS if args.workload == "c3" and args.skin is None else (cfg["skin"],)
+        # single systems: one class holds the whole batch
+        classes = AT.DEFAULT_CLASSES if len(systems) > 1 else (1 << 26,)
+        t_tune = time.perf_counter()
+        table = AT.tune(systems, lambda skin, n: make_rp(dict(cfg, skin=skin), n), skins=skins,
+                        classes=classes, log=log)
+        tune_s = time.perf_counter() - t_tune
+        log(f"bench: autotune {table.summary()} in {tune_s:.1f} s")
     eng, slots, mixed = run_engine(systems, cfg, G.PATH_MIXED, args.warmup, args.steps,
-                                   args.timing_steps, args.graph_steps, args.tpa)
+                                   args.timing_steps, args.graph_steps, args.tpa, table)
     # the measured traffic belongs to the default settings of a workload only
-    default_run = args.skin is None and args.tpa == 0
+    default_run = args.skin is None and args.tpa == 0 and not args.no_tune
     mixed["workload"] = args.workload if default_run else None
     log(f"bench: mixed {mixed['atom_steps_per_s']:.4e} atom-steps/s, "
         f"{mixed['ms_per_step']:.3f} ms/step, force {mixed.get('force_ms', 0):.4f} ms, "
@@ -304,6 +324,9 @@ def main():
                 mixed["atom_steps_per_s"] / plain["atom_steps_per_s"], 3),
             "force_rms_rel_err_vs_fp64": ferr,
             "graph_steps": args.graph_steps,
+            "autotune": table.summary() if table else None,
+            "autotune_s": round(tune_s, 1),
+            "plain_fp64_path": f"one system per launch, skin {cfg['skin']} A",
             "list_rebuilds_per_step": round(mixed["rebuilds_per_step"], 3),
             "rebuilt_atoms_per_step": round(mixed["rebuilt_atoms_per_step"], 1),
         },

@@ -188,6 +188,12 @@ int ndg_engine_release(ndg_engine_t *e, int slot);
 /* Launch configuration and timing.  tpa = threads per atom in the force kernel (0 =
  * per-size-class table); graph_steps = steps captured per hipGraph (0 = eager launches). */
 int ndg_engine_config(ndg_engine_t *e, int tpa, int graph_steps);
+/* Threads per atom of the force kernel by size class (no counterpart in the reference: the
+ * autotuner's choice for batched systems).  A system of np atoms admitted later uses
+ * tpa[k] of the first class k with np <= np_max[k] (np_max ascending, n <= 8); systems
+ * beyond the last class, or with n = 0, use the default.  An engine-wide tpa set by
+ * ndg_engine_config takes precedence. */
+int ndg_engine_size_classes(ndg_engine_t *e, int n, const int *np_max, const int *tpa);
 /* Average device time (ms) of the force kernel over launches since the last reset,
  * measured with HIP events on the engine's stream; count = launches timed. */
 int ndg_engine_kernel_time(ndg_engine_t *e, int reset, double *avg_ms, long long *count,

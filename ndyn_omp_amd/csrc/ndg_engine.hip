@@ -185,6 +185,7 @@ struct ndg_engine {
   std::map<uint64_t, TableBuf> tables;
   std::multimap<size_t, void *> free_blocks;
   int tpa_override = 0;
+  std::vector<std::pair<int, int>> tpa_classes;   // (np_max, tpa), np_max ascending
   int graph_steps = 0;
   // timing of the force kernel
   bool timing = false;
@@ -1220,7 +1221,14 @@ static int begin_run(ndg_engine *e, int r, HostState &h, const ndg_run_params &r
   int neimax = neimax_req > 0 ? neimax_req : estimate_neimax(rcut, skin, h.np, box);
   if (neimax < 1) neimax = 1;
   s.auto_neimax = neimax_req <= 0;
-  const int tpa = e->tpa_override ? e->tpa_override : default_tpa(h.np);
+  int tpa = e->tpa_override ? e->tpa_override : default_tpa(h.np);
+  if (!e->tpa_override && e->path == NDG_PATH_MIXED) {
+    for (const auto &c : e->tpa_classes)
+      if (h.np <= c.first) {
+        tpa = c.second;
+        break;
+      }
+  }
   const int ncap = ncell_capacity(box, rcut, skin);
   const bool need_alloc = !s.block || neimax > s.neimax || ncap > s.ncell_cap || tpa != s.tpa;
   int rc;
@@ -1377,6 +1385,21 @@ extern "C" int ndg_engine_config(ndg_engine_t *e, int tpa, int graph_steps) {
     return fail(NDG_ERR_ARG, "tpa must be 0,1,2,4,8");
   e->tpa_override = e->path == NDG_PATH_FP64 ? 1 : tpa;
   e->graph_steps = graph_steps < 0 ? 0 : graph_steps;
+  return NDG_OK;
+}
+
+extern "C" int ndg_engine_size_classes(ndg_engine_t *e, int n, const int *np_max, const int *tpa) {
+  if (!e || n < 0 || n > 8 || (n > 0 && (!np_max || !tpa)))
+    return fail(NDG_ERR_ARG, "ndg_engine_size_classes: bad arguments");
+  std::vector<std::pair<int, int>> c;
+  for (int k = 0; k < n; ++k) {
+    if (tpa[k] != 1 && tpa[k] != 2 && tpa[k] != 4 && tpa[k] != 8)
+      return fail(NDG_ERR_ARG, "tpa must be 1,2,4,8");
+    if (np_max[k] < 1 || (k > 0 && np_max[k] <= np_max[k - 1]))
+      return fail(NDG_ERR_ARG, "np_max must be positive and ascending");
+    c.push_back({np_max[k], tpa[k]});
+  }
+  e->tpa_classes = c;
   return NDG_OK;
 }
 

@@ -112,43 +112,26 @@ __device__ inline double type_bits(int t) { return __longlong_as_double((long lo
 __device__ inline int bits_type(double w) { return (int)__double_as_longlong(w); }
 
 // 16-byte form of a cell-anchored position, read by the mixed force kernel (one gather
-// per neighbour instead of two).  Each coordinate x is the FP32 value h = fl32(x) plus the
-// residual x - h (exact in FP64) in units of q = 2^(e-33), where 2^e >= w/2 + skin + 1 A
-// bounds |x| between rebuilds: |x - h| <= ulp32(x)/2 <= 2^(e-25) = 256 q.  Word 3 holds
-// the residuals as signed 10/10/9-bit fields and the type - 1 in its top 3 bits; the
-// decoded position h + r q is within q/2 ~ 1e-9 A of x.
+// per neighbour instead of two).  Each coordinate x is the 32-bit integer n = rint(x / q)
+// with q = 2^(e-30), where 2^e >= w/2 + skin + 1 A bounds |x| between rebuilds, so
+// |n| < 2^30 and a difference of two never overflows.  Word 3 holds type - 1.  The pair
+// displacement is then (n_i - n_j) q + offset, exact but for the rounding of x to q/2
+// (~4e-9 A for Cu cells).
 __device__ inline double xq_unit(const RepState &S, double skin) {
   double w = 0.0;
   for (int a = 0; a < 3; ++a) w = fmax(w, S.box[a] / S.mcell[a]);
   int e;
   frexp(0.5 * w + skin + 1.0, &e);   // 2^(e-1) <= bound < 2^e
-  return ldexp(1.0, e - 33);
+  return ldexp(1.0, e - 30);
 }
 
 __device__ inline uint4 xq_pack(double x, double y, double z, int ty, double q) {
-  const double c[3] = {x, y, z};
-  unsigned h[3];
-  int r[3];
-  const int lim[3] = {511, 511, 255};
-  for (int a = 0; a < 3; ++a) {
-    const float f = (float)c[a];
-    h[a] = __float_as_uint(f);
-    int k = (int)rint((c[a] - (double)f) / q);
-    r[a] = k < -lim[a] ? -lim[a] : (k > lim[a] ? lim[a] : k);
-  }
-  const unsigned w = ((unsigned)r[0] & 0x3FFu) | (((unsigned)r[1] & 0x3FFu) << 10) |
-                     (((unsigned)r[2] & 0x1FFu) << 20) | ((unsigned)(ty - 1) << 29);
-  return make_uint4(h[0], h[1], h[2], w);
-}
-
-// decoded coordinates (FP64) and type of a packed position
-__device__ inline void xq_unpack(const uint4 v, double q, double &x, double &y, double &z,
-                                 int &ty) {
-  const int rx = (int)(v.w << 22) >> 22, ry = (int)(v.w << 12) >> 22, rz = (int)(v.w << 3) >> 23;
-  x = fma((double)rx, q, (double)__uint_as_float(v.x));
-  y = fma((double)ry, q, (double)__uint_as_float(v.y));
-  z = fma((double)rz, q, (double)__uint_as_float(v.z));
-  ty = (int)(v.w >> 29) + 1;
+  const double lim = 1073741823.0;   // 2^30 - 1
+  const double iq = 1.0 / q;         // a power of two: exact
+  const int nx = (int)fmin(lim, fmax(-lim, rint(x * iq)));
+  const int ny = (int)fmin(lim, fmax(-lim, rint(y * iq)));
+  const int nz = (int)fmin(lim, fmax(-lim, rint(z * iq)));
+  return make_uint4((unsigned)nx, (unsigned)ny, (unsigned)nz, (unsigned)(ty - 1));
 }
 
 // Reference binning (src/nd.f90:1240-1246): sr = p - anint(p); I = int((sr+0.4999) M).
@@ -296,10 +279,8 @@ __device__ inline void force_atom(const RepParams &R, int i, int sub, const doub
   const auto Lg = gmem(L);
   const auto TF = gmem((const float *)R.tabF2);
   const auto TP = gmem((const float *)R.tabP2);
-  double xix, xiy, xiz;
-  int ti;
-  xq_unpack(make_uint4(XQ[4 * i], XQ[4 * i + 1], XQ[4 * i + 2], XQ[4 * i + 3]), qunit, xix, xiy,
-            xiz, ti);
+  const int nix = (int)XQ[4 * i], niy = (int)XQ[4 * i + 1], niz = (int)XQ[4 * i + 2];
+  const int ti = (int)XQ[4 * i + 3] + 1;
   const int n = gmem(R.nnb)[i];
   const int nmine = n > sub ? (n - sub + TPA - 1) / TPA : 0;  // entries of this lane
   const int nt_ = R.ntypes;
@@ -310,8 +291,9 @@ __device__ inline void force_atom(const RepParams &R, int i, int sub, const doub
   const double rc2_0 = rc2[0];
   const float vdr2_0 = vdr2[0];
   const int lt_0 = LDS_TAB ? lt[0] : 0;
-  // absent entries read the atom itself with a zero offset; the select drops them
-  const unsigned pad = (unsigned)i | (21u << kCodeShift);
+  // absent entries read the atom itself with the far offset of code 63, so the cutoff
+  // select drops them
+  const unsigned pad = (unsigned)i | (63u << kCodeShift);
   // the list entries of the next iteration are loaded one iteration ahead, so only the
   // position gathers stand between an iteration's loads and its arithmetic; the list has
   // kListPad spare chunks, so reads past the end stay in bounds
@@ -328,10 +310,6 @@ __device__ inline void force_atom(const RepParams &R, int i, int sub, const doub
       const unsigned j = e[u] & kIndexMask;
       pj[u] = make_uint4(XQ[4 * j], XQ[4 * j + 1], XQ[4 * j + 2], XQ[4 * j + 3]);
     }
-    double jx[U], jy[U], jz[U];
-    int tj[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) xq_unpack(pj[u], qunit, jx[u], jy[u], jz[u], tj[u]);
     if (c + U < nmine) {
 #pragma unroll
       for (int u = 0; u < U; ++u) nxt[u] = Lg[(size_t)(c + U + u) * (kTile * TPA)];
@@ -339,24 +317,23 @@ __device__ inline void force_atom(const RepParams &R, int i, int sub, const doub
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const double4 sft = shift[e[u] >> kCodeShift];
-      const double dx = (xix - jx[u]) + sft.x;
-      const double dy = (xiy - jy[u]) + sft.y;
-      const double dz = (xiz - jz[u]) + sft.z;
+      const double dx = fma((double)(nix - (int)pj[u].x), qunit, sft.x);
+      const double dy = fma((double)(niy - (int)pj[u].y), qunit, sft.y);
+      const double dz = fma((double)(niz - (int)pj[u].z), qunit, sft.z);
       const double d2 = fma(dx, dx, fma(dy, dy, dz * dz));
       int pid = 0;
       double rc2p = rc2_0;
       float vd = vdr2_0;
       int ltp = lt_0;
       if (!ONE) {
-        pid = pm[(ti - 1) + (tj[u] - 1) * nt_] - 1;
+        pid = pm[(ti - 1) + (int)pj[u].w * nt_] - 1;
         rc2p = rc2[pid];
         vd = vdr2[pid];
         if (LDS_TAB) ltp = lt[pid];
       }
-      const bool ok = (c + u < nmine) && !(d2 > rc2p);
+      const bool ok = !(d2 > rc2p);
       const float rk = (float)(d2 - ptrmin) * vd + 1.0f;
-      int k = (int)rk;
-      k = k < 1 ? 1 : (k > kmax ? kmax : k);
+      const int k = min(max((int)rk, 1), kmax);
       const float w = rk - (float)k;
       float fv;
       if (LDS_TAB) {
@@ -408,6 +385,11 @@ __global__ __launch_bounds__(ForceShape<TPA>::kThreads) void k_force_mixed(Engin
   const int nblk = (nt + NSUB - 1) / NSUB;
   int bb, be, bs;
   tile_range(0, nblk, bb, be, bs);
+  // replica whose parameters each sub-group holds in LDS (box and grid do not change
+  // during a launch, so a sub-group keeps them while its tiles stay in one replica)
+  int held[NSUB];
+#pragma unroll
+  for (int q = 0; q < NSUB; ++q) held[q] = -1;
   for (int blk = bb; blk < be; blk += bs) {
     const int t = t0 + blk * NSUB + sg;
     bool have = t < t0 + nt;
@@ -419,12 +401,25 @@ __global__ __launch_bounds__(ForceShape<TPA>::kThreads) void k_force_mixed(Engin
       const bool full = (f & (F_CU | F_CV)) != 0;
       have = (f & F_ACTIVE) && R.tpa == TPA && full == FULL;
     }
-    __syncthreads();
-    if (have && lt < 64) {
+    // the same decision in every thread: the tiles of this pass, read by all
+    bool renew = false;
+#pragma unroll
+    for (int q = 0; q < NSUB; ++q) {
+      const int tq = t0 + blk * NSUB + q;
+      const int rq = tq < t0 + nt ? E.tiles[tq].rep : -1;
+      renew = renew || (rq != held[q]);
+      held[q] = rq;
+    }
+    if (renew) __syncthreads();
+    // written for every existing tile, active or not, so that `held` stays true
+    if (renew && t < t0 + nt && lt < 64) {
       const int c = lt;
-      sh_shift[sg][c] = make_double4((double)((c & 3) - 1) * (S.box[0] / S.mcell[0]),
-                                     (double)(((c >> 2) & 3) - 1) * (S.box[1] / S.mcell[1]),
-                                     (double)(((c >> 4) & 3) - 1) * (S.box[2] / S.mcell[2]), 0.0);
+      // codes with a field of 3 never occur in a list; 63 marks an absent entry and
+      // places it far beyond every cutoff
+      sh_shift[sg][c] = c == 63 ? make_double4(1e30, 0.0, 0.0, 0.0)
+                                : make_double4((double)((c & 3) - 1) * (S.box[0] / S.mcell[0]),
+                                               (double)(((c >> 2) & 3) - 1) * (S.box[1] / S.mcell[1]),
+                                               (double)(((c >> 4) & 3) - 1) * (S.box[2] / S.mcell[2]), 0.0);
       if (c < R.npairs) {
         sh_rc2[sg][c] = R.rc2d[c];
         sh_vdr2[sg][c] = R.vdr2[c];
@@ -437,7 +432,7 @@ __global__ __launch_bounds__(ForceShape<TPA>::kThreads) void k_force_mixed(Engin
         sh_uselds[sg] = use;
       }
     }
-    __syncthreads();
+    if (renew) __syncthreads();
     const int i = T.atom0 + a;
     const double qunit = have ? xq_unit(S, R.skin) : 1.0;
     double fx = 0.0, fy = 0.0, fz = 0.0;
@@ -483,6 +478,7 @@ __global__ __launch_bounds__(ForceShape<TPA>::kThreads) void k_force_mixed(Engin
       for (int m = 32; m >= 1; m >>= 1)
 #pragma unroll
         for (int q = 0; q < 7; ++q) acc[q] += shfl_xor_d(acc[q], m);
+      __syncthreads();   // the previous pass has read sh_red
       if ((threadIdx.x & 63) == 0)
 #pragma unroll
         for (int q = 0; q < 7; ++q) sh_red[threadIdx.x >> 6][q] = acc[q];

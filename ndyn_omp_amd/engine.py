@@ -99,6 +99,7 @@ def load_library():
         "ndg_engine_fetch": [V, C.c_int, V, V, V, V],
         "ndg_engine_release": [V, C.c_int],
         "ndg_engine_config": [V, C.c_int, C.c_int],
+        "ndg_engine_size_classes": [V, C.c_int, ip, ip],
         "ndg_engine_timing": [V, C.c_int],
         "ndg_engine_kernel_time": [V, C.c_int, P(C.c_double), P(C.c_longlong), P(C.c_double),
                                    P(C.c_double)],
@@ -300,6 +301,13 @@ class BatchEngine:
 
     def config(self, tpa=0, graph_steps=0):
         _check(load_library().ndg_engine_config(self.e, int(tpa), int(graph_steps)))
+
+    def size_classes(self, classes):
+        """Threads per atom by size class for systems admitted from now on:
+        [(np_max, tpa), ...] with np_max ascending."""
+        npm = np.array([c[0] for c in classes], dtype=np.int32)
+        tpa = np.array([c[1] for c in classes], dtype=np.int32)
+        _check(load_library().ndg_engine_size_classes(self.e, len(classes), npm, tpa))
 
     def admit(self, s: W.System, rp: RunParams, neimax=0, raise_on_full=True):
         key = (s.kinds.tobytes(), s.params.tobytes(), s.ptrmin, s.ptsize)
