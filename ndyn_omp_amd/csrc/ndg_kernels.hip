@@ -37,6 +37,15 @@ __device__ inline __attribute__((address_space(1))) T *gmem(T *p) {
   return (__attribute__((address_space(1))) T *)p;
 }
 
+// A pointer that is the same in every lane, moved to scalar registers.
+template <class T>
+__device__ inline T *uniform_ptr(T *p) {
+  const unsigned long long v = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return (T *)(((unsigned long long)hi << 32) | lo);
+}
+
 __device__ inline double shfl_xor_d(double v, int m) {
   int2 a = *reinterpret_cast<int2 *>(&v);
   a.x = __shfl_xor(a.x, m);
@@ -275,7 +284,8 @@ __device__ inline void force_atom(const RepParams &R, int i, int sub, const doub
                                   double (&acc)[7]) {
   constexpr int U = kUnroll;
   static_assert(U <= kListPad, "the list holds kListPad spare chunks for the unrolled reads");
-  const auto XQ = gmem((const unsigned *)R.xq);
+  // the replica's arrays are the same for the whole wave (a wave never spans two tiles)
+  const auto XQ = gmem((const unsigned *)uniform_ptr(R.xq));
   const auto Lg = gmem(L);
   const auto TF = gmem((const float *)R.tabF2);
   const auto TP = gmem((const float *)R.tabP2);
@@ -285,7 +295,7 @@ __device__ inline void force_atom(const RepParams &R, int i, int sub, const doub
   const int nmine = n > sub ? (n - sub + TPA - 1) / TPA : 0;  // entries of this lane
   const int nt_ = R.ntypes;
   const int tab = R.ptsize + 1;
-  const int kmax = R.ptsize;
+  const float kmaxf = (float)R.ptsize;
   const double ptrmin = R.ptrmin;
   // single pair potential: its parameters stay in registers
   const double rc2_0 = rc2[0];
@@ -332,9 +342,13 @@ __device__ inline void force_atom(const RepParams &R, int i, int sub, const doub
         if (LDS_TAB) ltp = lt[pid];
       }
       const bool ok = !(d2 > rc2p);
+      // the reference's table position, in its single-precision operations
+      // (src/nd.f90:299): rk = real(d2 - PTrMin) * vPTdr2 + 1
       const float rk = (float)(d2 - ptrmin) * vd + 1.0f;
-      const int k = min(max((int)rk, 1), kmax);
-      const float w = rk - (float)k;
+      // k = clamp(int(rk), 1, kmax) by a median in FP32, then the fraction
+      const float kf = floorf(__builtin_amdgcn_fmed3f(rk, 1.0f, kmaxf));
+      const int k = (int)kf;
+      const float w = rk - kf;
       float fv;
       if (LDS_TAB) {
         const float *Tl = stab + ltp;
@@ -345,7 +359,7 @@ __device__ inline void force_atom(const RepParams &R, int i, int sub, const doub
         const float F0 = TF[q], F1 = TF[q + 1];
         fv = fmaf(w, F1 - F0, F0);
       }
-      const double df = ok ? (double)(fv * __builtin_amdgcn_rcpf((float)d2)) : 0.0;
+      const double df = (double)(ok ? fv * __builtin_amdgcn_rcpf((float)d2) : 0.0f);
       fx = fma(dx, df, fx);
       fy = fma(dy, df, fy);
       fz = fma(dz, df, fz);
@@ -897,22 +911,40 @@ __device__ inline int cell_of(const RepParams &R, const RepState &S, int i) {
   return kx + ky * S.mcell[0] + kz * S.mcell[0] * S.mcell[1];
 }
 
-// Final order of one cell's slots [lo, hi): ascending caller index.  Cells of up to 64
+// Order of atoms inside a cell: Morton index of the atom's 4 x 4 x 4 sub-cell, then
+// caller index.  Neighbouring slots are then close in space, so the lists of consecutive
+// atoms overlap (gathers of a wave share cache lines) and a run of consecutive candidates
+// spans a small box.  Key = sub-cell << 26 | caller index (caller indices < 2^26).
+__device__ inline unsigned order_key(const RepParams &R, const RepState &S, int i) {
+  const double p[3] = {R.px[i], R.py[i], R.pz[i]};
+  unsigned m = 0;
+  for (int a = 0; a < 3; ++a) {
+    const double sr = p[a] - round(p[a]);
+    const double f = (sr + 0.4999) * (double)S.mcell[a];
+    const double fr = f - floor(f);
+    const int q = min(3, max(0, (int)(fr * 4.0)));
+    m |= (unsigned)((q & 1) | ((q & 2) << 2)) << a;   // bits a and a + 3
+  }
+  return (m << kCodeShift) | (unsigned)R.oid[i];
+}
+
+// Final order of one cell's slots [lo, hi): ascending order_key.  Cells of up to 64
 // atoms are sorted inside one wave (bitonic network over shuffles); cells up to
 // kCellLds atoms by rank counts over a per-wave LDS copy; larger cells by rank counts over
 // global memory.  The atoms are then copied to their final slots in tmp.
 constexpr int kCellLds = 256;
 
-__device__ inline void sort_cell_wave(const RepParams &R, int lo, int hi, int parity, int *wsrc,
-                                      int *wkey) {
+__device__ inline void sort_cell_wave(const RepParams &R, const RepState &S, int lo, int hi,
+                                      int parity, int *wsrc, unsigned *wkey) {
   const int lane = threadIdx.x & 63;
   const int cnt = hi - lo;
   if (cnt <= 64) {
     int u = lane < cnt ? R.slot_src[lo + lane] : -1;
-    int key = lane < cnt ? R.oid[u] : 0x7fffffff;
+    unsigned key = lane < cnt ? order_key(R, S, u) : 0xffffffffu;
     for (int k = 2; k <= 64; k <<= 1)
       for (int j = k >> 1; j > 0; j >>= 1) {
-        const int okey = __shfl_xor(key, j), ou = __shfl_xor(u, j);
+        const unsigned okey = __shfl_xor(key, j);
+        const int ou = __shfl_xor(u, j);
         const bool up = (lane & k) == 0, lower = (lane & j) == 0;
         const bool take = lower ? (up ? okey < key : okey > key) : (up ? okey > key : okey < key);
         if (take) {
@@ -925,11 +957,11 @@ __device__ inline void sort_cell_wave(const RepParams &R, int lo, int hi, int pa
     for (int v = lane; v < cnt; v += 64) {
       const int u = R.slot_src[lo + v];
       wsrc[v] = u;
-      wkey[v] = R.oid[u];
+      wkey[v] = order_key(R, S, u);
     }
     wave_sync();
     for (int v = lane; v < cnt; v += 64) {
-      const int me = wkey[v];
+      const unsigned me = wkey[v];
       int rank = 0;
       for (int w = 0; w < cnt; ++w) rank += wkey[w] < me;
       permute_atom(R, wsrc[v], lo + rank, parity);
@@ -938,21 +970,22 @@ __device__ inline void sort_cell_wave(const RepParams &R, int lo, int hi, int pa
   } else {
     for (int v = lo + lane; v < hi; v += 64) {
       const int i = R.slot_src[v];
-      const int me = R.oid[i];
+      const unsigned me = order_key(R, S, i);
       int rank = 0;
-      for (int w = lo; w < hi; ++w) rank += R.oid[R.slot_src[w]] < me;
+      for (int w = lo; w < hi; ++w) rank += order_key(R, S, R.slot_src[w]) < me;
       permute_atom(R, i, lo + rank, parity);
     }
   }
 }
 
 // Whole sort of one small replica (small_sort(np, ncell_cap)) inside one workgroup, with
-// the bookkeeping in LDS: cell keys and caller indices, cell counts and starts by LDS
-// atomics and a block scan, an unordered scatter of caller indices into cell order, and
-// the final slot of each atom = its cell's start + the number of atoms of the cell with a
-// smaller caller index.  Brute mode (one cell) puts every atom at its caller index.
+// the bookkeeping in LDS: cell numbers and order keys, cell counts and starts by LDS
+// atomics and a block scan, an unordered scatter of order keys into cell order, and the
+// final slot of each atom = its cell's start + the number of atoms of the cell with a
+// smaller order key.  Brute mode (one cell) puts every atom at its caller index.
 __global__ __launch_bounds__(1024) void k_rb_small(EngineDev E, int r0, int nr) {
-  __shared__ unsigned short s_key[kSmallSort], s_oid[kSmallSort], s_moid[kSmallSort];
+  __shared__ unsigned short s_key[kSmallSort];
+  __shared__ unsigned s_oid[kSmallSort], s_moid[kSmallSort];
   __shared__ int s_start[kSmallCells + 1], s_fill[kSmallCells];
   __shared__ int s_scan[1024];
   for (int r = r0 + blockIdx.x; r < r0 + nr; r += gridDim.x) {
@@ -974,7 +1007,7 @@ __global__ __launch_bounds__(1024) void k_rb_small(EngineDev E, int r0, int nr) 
       for (int p = tid; p < np; p += nb) {
         const int c = cell_of(R, S, p);
         s_key[p] = (unsigned short)c;
-        s_oid[p] = (unsigned short)R.oid[p];
+        s_oid[p] = order_key(R, S, p);
         atomicAdd(&s_start[c], 1);
       }
       __syncthreads();
@@ -1010,7 +1043,7 @@ __global__ __launch_bounds__(1024) void k_rb_small(EngineDev E, int r0, int nr) 
       __syncthreads();
       for (int p = tid; p < np; p += nb) {
         const int c = s_key[p], lo = s_start[c], hi = s_start[c + 1];
-        const int me = s_oid[p];
+        const unsigned me = s_oid[p];
         int rank = 0;
         for (int k = lo; k < hi; ++k) rank += s_moid[k] < me;
         permute_atom(R, p, lo + rank, parity);
@@ -1115,7 +1148,8 @@ __global__ __launch_bounds__(kTile) void k_rb_rank(EngineDev E, int t0, int nt) 
     const int wid = threadIdx.x >> 6, nwave = blockDim.x >> 6;
     for (c += wid; c < ncell && R.cell_start[c] < T.atom0 + kTile; c += nwave) {
       const int lo = R.cell_start[c], hi = R.cell_start[c + 1];
-      if (hi > lo) sort_cell_wave(R, lo, hi, S.parity, sh_sort[wid], sh_sort[wid] + kCellLds);
+      if (hi > lo)
+        sort_cell_wave(R, S, lo, hi, S.parity, sh_sort[wid], (unsigned *)(sh_sort[wid] + kCellLds));
     }
   }
 }
