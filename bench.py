@@ -1,6 +1,6 @@
 """Benchmark of the MD-step engine on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c1|c2|c4]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c1|c2|c4|accuracy]
 
 Prints one JSON line.  Default workload (the one the metric is quoted on): 512
 independent periodic systems of uneven size (atom counts log-uniform in [500, 20000],
@@ -220,6 +220,35 @@ def roofline(mixed, copy_gbs):
                 frac_of_measured_copy=round(achieved / copy_gbs, 4))
 
 
+def accuracy_bench(args):
+    """Mixed path vs FP64 path per configuration: force / per-atom energy / pressure
+    errors on identical positions, and NVE energy drift per 10^4 steps (both paths)."""
+    from ndyn_omp_amd import accuracy as A
+    nsteps = args.drift_steps
+    cases = [
+        ("c0/c2 Morse Cu (reference table)", W.cu_morse(20, temp_k=600.0, seed=71), 0.5, W.DT_1FS),
+        ("c0/c2 Morse Cu (energy-consistent table)",
+         W.cu_morse(20, temp_k=300.0, seed=72, kind=W.MORSE_PHYS), 0.5, W.DT_1FS),
+        ("c1 LJ argon", W.ar_lj(20, temp_k=90.0, seed=73), 0.4, W.DT_2FS),
+        ("c3 Buckingham alloy", W.buck_alloy((14, 14, 14), temp_k=900.0, seed=74), 0.5, W.DT_1FS),
+    ]
+    res = []
+    for name, s, skin, dt in cases:
+        r = dict(case=name, atoms=s.np, dt_fs=round(dt / W.DT_1FS, 3))
+        r.update(A.pair_errors(A.thermal_snapshot(s, 200, dt, skin), skin))
+        for path, key in ((G.PATH_MIXED, "nve_mixed"), (G.PATH_FP64, "nve_fp64")):
+            r[key] = A.nve_drift(s, path, nsteps, dt, skin)
+        log(f"bench: {name}: force err {r['force_rms_rel']:.2e}, e_i err "
+            f"{r['energy_per_atom_max_abs_ev']:.2e} eV, pressure err {r['pressure_rel']:.2e}, "
+            f"drift/1e4 mixed {r['nve_mixed']['drift_per_atom_per_1e4_ev']:.2e} "
+            f"fp64 {r['nve_fp64']['drift_per_atom_per_1e4_ev']:.2e} eV/atom")
+        res.append(r)
+    print(json.dumps({"metric": "mixed-path accuracy vs FP64 path: force/energy/pressure "
+                                "error and NVE drift per 1e4 steps",
+                      "unit": "eV/atom per 1e4 steps", "n_gpus": 1, "steps": nsteps,
+                      "data": "synthetic", "cases": res}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -242,6 +271,7 @@ def main():
     ap.add_argument("--step-scale", type=float, default=0.02)
     ap.add_argument("--capacity", type=float, default=7e8)
     ap.add_argument("--tick-steps", type=int, default=50)
+    ap.add_argument("--drift-steps", type=int, default=10000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; the multi-GPU brick "
@@ -250,6 +280,9 @@ def main():
     G.load_library()
     if args.workload == "c4":
         serve_bench(args)
+        return
+    if args.workload == "accuracy":
+        accuracy_bench(args)
         return
     systems, desc, cfg = workload_systems(args.workload)
     if args.skin is not None:
