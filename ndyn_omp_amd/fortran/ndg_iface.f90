@@ -155,6 +155,158 @@ module ndg_iface
     type(c_ptr) function ndg_last_error() bind(C, name='ndg_last_error')
       import :: c_ptr
     end function
+
+    integer(c_int) function ndg_set_box(h, box) bind(C, name='ndg_set_box')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: box(3)
+    end function
+
+    integer(c_int) function ndg_get_box(h, box) bind(C, name='ndg_get_box')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: box(3)
+    end function
+
+    ! neighbour list in caller order: counts(np), and nbr(cap, np) when present
+    integer(c_int) function ndg_get_neighbours(h, counts, nbr, cap) &
+        bind(C, name='ndg_get_neighbours')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), intent(out) :: counts(*)
+      integer(c_int), intent(out), optional :: nbr(*)
+      integer(c_int), value :: cap
+    end function
+
+    ! ---------------------------------------------------------------- batched engine
+    ! many independent systems advanced per launch (see include/ndyn_gpu.h)
+
+    integer(c_int) function ndg_engine_create(e, path, hbm_budget) &
+        bind(C, name='ndg_engine_create')
+      import :: c_int, c_ptr, c_size_t
+      type(c_ptr), intent(out) :: e
+      integer(c_int), value :: path
+      integer(c_size_t), value :: hbm_budget
+    end function
+
+    integer(c_int) function ndg_engine_destroy(e) bind(C, name='ndg_engine_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+    end function
+
+    integer(c_size_t) function ndg_footprint(np, ntypes, npairs, ptsize, neimax) &
+        bind(C, name='ndg_footprint')
+      import :: c_int, c_size_t
+      integer(c_int), value :: np, ntypes, npairs, ptsize, neimax
+    end function
+
+    integer(c_int) function ndg_engine_admit(e, slot, np, ntypes, npairs, ptsize, neimax, &
+        P, F, rc2, dr2, ptrmin, pairmat, pairparams, mass, pos, vel, typ, box, rp) &
+        bind(C, name='ndg_engine_admit')
+      import :: c_int, c_double, c_ptr, ndg_run_params
+      type(c_ptr), value :: e
+      integer(c_int), intent(out) :: slot
+      integer(c_int), value :: np, ntypes, npairs, ptsize, neimax
+      real(c_double), intent(in) :: P(*), F(*), rc2(*), dr2(*)
+      real(c_double), value :: ptrmin
+      integer(c_int), intent(in) :: pairmat(*), typ(*)
+      real(c_double), intent(in) :: pairparams(*), mass(*), pos(*), vel(*), box(3)
+      type(ndg_run_params), intent(in) :: rp
+    end function
+
+    integer(c_int) function ndg_engine_step(e, nsteps, ndone) bind(C, name='ndg_engine_step')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), value :: nsteps
+      integer(c_int), intent(out) :: ndone
+    end function
+
+    integer(c_int) function ndg_engine_status(e, slot, steps_done, finished, last, err) &
+        bind(C, name='ndg_engine_status')
+      import :: c_int, c_ptr, ndg_thermo
+      type(c_ptr), value :: e
+      integer(c_int), value :: slot
+      integer(c_int), intent(out) :: steps_done, finished, err
+      type(ndg_thermo), intent(out) :: last
+    end function
+
+    integer(c_int) function ndg_engine_thermo(e, slot, rec, max_rec, nrec) &
+        bind(C, name='ndg_engine_thermo')
+      import :: c_int, c_ptr, ndg_thermo
+      type(c_ptr), value :: e
+      integer(c_int), value :: slot, max_rec
+      type(ndg_thermo), intent(out) :: rec(*)
+      integer(c_int), intent(out) :: nrec
+    end function
+
+    integer(c_int) function ndg_engine_fetch(e, slot, pos, vel, force, box) &
+        bind(C, name='ndg_engine_fetch')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), value :: slot
+      real(c_double), intent(out) :: pos(*), vel(*), force(*), box(3)
+    end function
+
+    integer(c_int) function ndg_engine_release(e, slot) bind(C, name='ndg_engine_release')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), value :: slot
+    end function
+
+    integer(c_int) function ndg_engine_config(e, tpa, graph_steps) &
+        bind(C, name='ndg_engine_config')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), value :: tpa, graph_steps
+    end function
+
+    integer(c_int) function ndg_engine_size_classes(e, n, np_max, tpa) &
+        bind(C, name='ndg_engine_size_classes')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), value :: n
+      integer(c_int), intent(in) :: np_max(*), tpa(*)
+    end function
+
+    integer(c_int) function ndg_engine_kernel_time(e, reset, avg_ms, count, pairs, atoms) &
+        bind(C, name='ndg_engine_kernel_time')
+      import :: c_int, c_double, c_long_long, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), value :: reset
+      real(c_double), intent(out) :: avg_ms, pairs, atoms
+      integer(c_long_long), intent(out) :: count
+    end function
+
+    integer(c_int) function ndg_engine_timing(e, enable) bind(C, name='ndg_engine_timing')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), value :: enable
+    end function
+
+    integer(c_int) function ndg_engine_counters(e, slot, nupdates, step, mean_nbr) &
+        bind(C, name='ndg_engine_counters')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), value :: slot
+      integer(c_int), intent(out) :: nupdates, step
+      real(c_double), intent(out) :: mean_nbr
+    end function
+
+    integer(c_int) function ndg_engine_list_stats(e, mean_nbr, atoms) &
+        bind(C, name='ndg_engine_list_stats')
+      import :: c_int, c_double, c_long_long, c_ptr
+      type(c_ptr), value :: e
+      real(c_double), intent(out) :: mean_nbr
+      integer(c_long_long), intent(out) :: atoms
+    end function
+
+    integer(c_int) function ndg_copy_bandwidth(bytes, iters, gbps) &
+        bind(C, name='ndg_copy_bandwidth')
+      import :: c_int, c_double, c_size_t
+      integer(c_size_t), value :: bytes
+      integer(c_int), value :: iters
+      real(c_double), intent(out) :: gbps
+    end function
   end interface
 
 contains

@@ -44,3 +44,15 @@ def test_library_loads_and_tables_match_oracle(oracle):
         b = oracle.pair_tables([kind], prm, 0.25, 5000)
         for x, y in zip(a, b):
             np.testing.assert_allclose(x, y, rtol=1e-14, atol=1e-15)
+
+
+def test_fortran_module_binds_every_entry_point():
+    """ndyn_omp_amd/fortran/ndg_iface.f90 declares a bind(C) interface for every
+    function of include/ndyn_gpu.h."""
+    import re
+    hdr = open(os.path.join(ROOT, "include", "ndyn_gpu.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    decl = set(re.findall(r"^\s*(?:int|size_t|const char \*)\s*(ndg_\w+)\s*\(", hdr, flags=re.M))
+    f90 = open(os.path.join(ROOT, "ndyn_omp_amd", "fortran", "ndg_iface.f90")).read()
+    bound = set(re.findall(r"name='(ndg_\w+)'", f90))
+    assert decl and decl - bound == set()
