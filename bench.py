@@ -266,10 +266,11 @@ def main():
     ap.add_argument("--cpu-budget-s", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--jobs", type=int, default=200)
+    ap.add_argument("--jobs", type=int, default=2000)
     ap.add_argument("--size-scale", type=float, default=0.25)
-    ap.add_argument("--step-scale", type=float, default=0.02)
-    ap.add_argument("--capacity", type=float, default=7e8)
+    ap.add_argument("--step-scale", type=float, default=0.1)
+    # saturated wall-clock throughput of this job sample (arrival rate far above it)
+    ap.add_argument("--capacity", type=float, default=1.15e9)
     ap.add_argument("--tick-steps", type=int, default=50)
     ap.add_argument("--drift-steps", type=int, default=10000)
     args = ap.parse_args()

@@ -25,7 +25,7 @@ void launch_force(const EngineDev &E, int path, int tpa, bool full, int t0, int 
 void launch_reduce(const EngineDev &E, int r0, int nr, int mode, int xflags, int trigger,
                    hipStream_t s);
 void launch_apply_pending(const EngineDev &E, int r0, int nr, hipStream_t s);
-void launch_rebuild(const EngineDev &E, int t0, int nt, int r0, int nr, bool any_large,
+void launch_rebuild(const EngineDev &E, int t0, int nt, int r0, int nr,
                     hipStream_t s);
 void launch_copy(const float4 *src, float4 *dst, size_t n, hipStream_t s);
 enum : int { RED_FORCE = 1, RED_KICK = 2, RED_FLIP = 4, RED_STEP = 8, RED_CHECK = 16 };
@@ -181,7 +181,6 @@ struct ndg_engine {
   std::vector<std::pair<int, int>> tpa_ranges;  // (tpa, t0) in tile order, nt follows
   std::vector<int> tpa_counts;
   bool tiles_dirty = true;
-  bool any_large = false;
   std::map<uint64_t, TableBuf> tables;
   std::multimap<size_t, void *> free_blocks;
   int tpa_override = 0;
@@ -440,7 +439,6 @@ static int rebuild_tiles(ndg_engine *e) {
   e->h_tiles.clear();
   e->tpa_ranges.clear();
   e->tpa_counts.clear();
-  e->any_large = false;
   const int tpas[4] = {1, 2, 4, 8};
   for (int q = 0; q < 4; ++q) {
     int t0 = (int)e->h_tiles.size();
@@ -449,7 +447,6 @@ static int rebuild_tiles(ndg_engine *e) {
       if (!s.used || s.tpa != tpas[q]) continue;
       s.tile_base = (int)e->h_tiles.size();
       for (int lt = 0; lt < s.ntiles; ++lt) e->h_tiles.push_back(Tile{(int)r, lt * kTile, lt, 0});
-      if (!small_sort(s.np, s.ncell_cap)) e->any_large = true;
     }
     int nt = (int)e->h_tiles.size() - t0;
     if (nt > 0) {
@@ -819,7 +816,7 @@ static int build_slot_now(ndg_engine *e, int r, double skin) {
   if ((rc = upload_params(e, r))) return rc;
   if ((rc = put_state_dev(e, r, &S))) return rc;
   EngineDev E = dev_view(e);
-  launch_rebuild(E, s.tile_base, s.ntiles, r, 1, !small_sort(s.np, s.ncell_cap), e->stream);
+  launch_rebuild(E, s.tile_base, s.ntiles, r, 1, e->stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(e->stream));
   if ((rc = get_state_dev(e, r, &S))) return rc;
@@ -921,7 +918,7 @@ static void enqueue_step_batched(ndg_engine *e, const TickPlan &tp) {
   EngineDev E = dev_view(e);
   const int nt = (int)e->h_tiles.size(), nr = (int)e->slots.size();
   hipStream_t s = e->stream;
-  if (tp.check) launch_rebuild(E, 0, nt, 0, nr, e->any_large, s);
+  if (tp.check) launch_rebuild(E, 0, nt, 0, nr, s);
   launch_drift(E, e->path, 0, nt, 0, s);
   hipEvent_t a = nullptr;
   time_begin(e, &a);
@@ -946,7 +943,7 @@ static void enqueue_step_single(ndg_engine *e, int r, const TickPlan &tp) {
   S.step = sl.step;
   const int f = step_flags(sl.P, S);
   if (!(f & F_ACTIVE)) return;
-  if (f & F_CHECK) launch_rebuild(E, sl.tile_base, sl.ntiles, r, 1, !small_sort(sl.np, sl.ncell_cap), s);
+  if (f & F_CHECK) launch_rebuild(E, sl.tile_base, sl.ntiles, r, 1, s);
   launch_drift(E, e->path, sl.tile_base, sl.ntiles, 0, s);
   hipEvent_t a = nullptr;
   time_begin(e, &a);

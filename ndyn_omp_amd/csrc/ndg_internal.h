@@ -112,13 +112,6 @@ struct EngineDev {
 
 constexpr int kLdsTableBytes = 64 * 1024;   // LDS budget of the staged table set
 
-// Replicas sorted by cell inside one workgroup (LDS bookkeeping); larger ones use the
-// grid-wide sort.  Cell keys and caller indices are stored as 16-bit values.
-constexpr int kSmallSort = 8192;     // atoms
-constexpr int kSmallCells = 4096;    // cell-grid capacity
-__host__ __device__ inline bool small_sort(int np, int ncell_cap) {
-  return np <= kSmallSort && ncell_cap <= kSmallCells;
-}
 
 // Flags of the step a replica is about to compute (src/nd.f90:137-151).
 __host__ __device__ inline int step_flags(const RepParams &R, const RepState &S) {

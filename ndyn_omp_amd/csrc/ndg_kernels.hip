@@ -978,88 +978,10 @@ __device__ inline void sort_cell_wave(const RepParams &R, const RepState &S, int
   }
 }
 
-// Whole sort of one small replica (small_sort(np, ncell_cap)) inside one workgroup, with
-// the bookkeeping in LDS: cell numbers and order keys, cell counts and starts by LDS
-// atomics and a block scan, an unordered scatter of order keys into cell order, and the
-// final slot of each atom = its cell's start + the number of atoms of the cell with a
-// smaller order key.  Brute mode (one cell) puts every atom at its caller index.
-__global__ __launch_bounds__(1024) void k_rb_small(EngineDev E, int r0, int nr) {
-  __shared__ unsigned short s_key[kSmallSort];
-  __shared__ unsigned s_oid[kSmallSort], s_moid[kSmallSort];
-  __shared__ int s_start[kSmallCells + 1], s_fill[kSmallCells];
-  __shared__ int s_scan[1024];
-  for (int r = r0 + blockIdx.x; r < r0 + nr; r += gridDim.x) {
-    const RepParams &R = E.reps[r];
-    RepState &S = E.st[r];
-    if (!S.rebuild || S.err || !small_sort(R.np, R.ncell_cap)) continue;
-    const int np = R.np, nb = blockDim.x, tid = threadIdx.x;
-    const int parity = S.parity;
-    if (S.brute) {
-      for (int i = tid; i < np; i += nb) permute_atom(R, i, R.oid[i], parity);
-      if (tid == 0) {
-        R.cell_start[0] = 0;
-        R.cell_start[1] = np;
-      }
-    } else {
-      const int ncell = S.mcell[0] * S.mcell[1] * S.mcell[2];
-      for (int c = tid; c < ncell; c += nb) { s_start[c] = 0; s_fill[c] = 0; }
-      __syncthreads();
-      for (int p = tid; p < np; p += nb) {
-        const int c = cell_of(R, S, p);
-        s_key[p] = (unsigned short)c;
-        s_oid[p] = order_key(R, S, p);
-        atomicAdd(&s_start[c], 1);
-      }
-      __syncthreads();
-      // exclusive scan of the counts: thread t owns a contiguous chunk of cells
-      const int per = (ncell + nb - 1) / nb;
-      const int c0 = min(ncell, tid * per), c1 = min(ncell, c0 + per);
-      int loc = 0;
-      for (int c = c0; c < c1; ++c) loc += s_start[c];
-      s_scan[tid] = loc;
-      __syncthreads();
-      for (int off = 1; off < nb; off <<= 1) {
-        const int v = tid >= off ? s_scan[tid - off] : 0;
-        __syncthreads();
-        s_scan[tid] += v;
-        __syncthreads();
-      }
-      int run = s_scan[tid] - loc;
-      for (int c = c0; c < c1; ++c) {
-        const int cnt = s_start[c];
-        s_start[c] = run;
-        R.cell_start[c] = run;
-        run += cnt;
-      }
-      if (tid == nb - 1) {
-        s_start[ncell] = s_scan[nb - 1];
-        R.cell_start[ncell] = s_scan[nb - 1];
-      }
-      __syncthreads();
-      for (int p = tid; p < np; p += nb) {
-        const int c = s_key[p];
-        s_moid[s_start[c] + atomicAdd(&s_fill[c], 1)] = s_oid[p];
-      }
-      __syncthreads();
-      for (int p = tid; p < np; p += nb) {
-        const int c = s_key[p], lo = s_start[c], hi = s_start[c + 1];
-        const unsigned me = s_oid[p];
-        int rank = 0;
-        for (int k = lo; k < hi; ++k) rank += s_moid[k] < me;
-        permute_atom(R, p, lo + rank, parity);
-      }
-    }
-    __syncthreads();
-    for (int p = tid; p < np; p += nb) place_atom(R, S, p);
-    __syncthreads();
-    if (tid == 0) {
-      S.nbr_sum = 0;
-      for (int a = 0; a < 3; ++a) S.box_build[a] = S.box[a];
-    }
-  }
-}
-
-// Large replicas: the same sort in five grid-wide phases.
+// Cell sort of the rebuilding replicas in five grid-wide phases over their tiles:
+// histogram, per-replica scan, scatter, per-cell order (one wave per cell), placement.
+// (A whole replica sorted inside one workgroup, with its bookkeeping in LDS, was slower
+// on the replica mix at every size: one workgroup serialises what the tiles share.)
 __global__ __launch_bounds__(kTile) void k_rb_bin(EngineDev E, int t0, int nt) {
   int tb, te, ts;
   tile_range(t0, nt, tb, te, ts);
@@ -1067,7 +989,7 @@ __global__ __launch_bounds__(kTile) void k_rb_bin(EngineDev E, int t0, int nt) {
     const Tile T = E.tiles[t];
     const RepParams &R = E.reps[T.rep];
     const RepState &S = E.st[T.rep];
-    if (!S.rebuild || S.err || small_sort(R.np, R.ncell_cap)) continue;
+    if (!S.rebuild || S.err) continue;
     const int i = T.atom0 + threadIdx.x;
     if (i >= R.np) continue;
     const int c = cell_of(R, S, i);
@@ -1081,7 +1003,7 @@ __global__ __launch_bounds__(1024) void k_rb_scan(EngineDev E, int r0, int nr) {
   for (int r = r0 + blockIdx.x; r < r0 + nr; r += gridDim.x) {
     const RepParams &R = E.reps[r];
     RepState &S = E.st[r];
-    if (!S.rebuild || S.err || small_sort(R.np, R.ncell_cap)) continue;
+    if (!S.rebuild || S.err) continue;
     const int nb = blockDim.x, tid = threadIdx.x;
     const int ncell = S.mcell[0] * S.mcell[1] * S.mcell[2];
     const int per = (ncell + nb - 1) / nb;
@@ -1119,7 +1041,7 @@ __global__ __launch_bounds__(kTile) void k_rb_scatter(EngineDev E, int t0, int n
     const Tile T = E.tiles[t];
     const RepParams &R = E.reps[T.rep];
     const RepState &S = E.st[T.rep];
-    if (!S.rebuild || S.err || small_sort(R.np, R.ncell_cap)) continue;
+    if (!S.rebuild || S.err) continue;
     const int i = T.atom0 + threadIdx.x;
     if (i >= R.np) continue;
     const int c = R.key[i];
@@ -1136,7 +1058,7 @@ __global__ __launch_bounds__(kTile) void k_rb_rank(EngineDev E, int t0, int nt) 
     const Tile T = E.tiles[t];
     const RepParams &R = E.reps[T.rep];
     const RepState &S = E.st[T.rep];
-    if (!S.rebuild || S.err || small_sort(R.np, R.ncell_cap)) continue;
+    if (!S.rebuild || S.err) continue;
     if (S.brute) {
       const int i = T.atom0 + threadIdx.x;
       if (i < R.np) permute_atom(R, i, R.oid[i], S.parity);
@@ -1161,7 +1083,7 @@ __global__ __launch_bounds__(kTile) void k_rb_place(EngineDev E, int t0, int nt)
     const Tile T = E.tiles[t];
     const RepParams &R = E.reps[T.rep];
     const RepState &S = E.st[T.rep];
-    if (!S.rebuild || S.err || small_sort(R.np, R.ncell_cap)) continue;
+    if (!S.rebuild || S.err) continue;
     const int p = T.atom0 + threadIdx.x;
     if (p >= R.np) continue;
     place_atom(R, S, p);
@@ -1585,19 +1507,15 @@ void launch_apply_pending(const EngineDev &E, int r0, int nr, hipStream_t s) {
   hipLaunchKernelGGL(k_apply_pending, dim3(nr < 1024 ? nr : 1024), dim3(256), 0, s, E, r0, nr);
 }
 
-void launch_rebuild(const EngineDev &E, int t0, int nt, int r0, int nr, bool any_large,
-                    hipStream_t s) {
+void launch_rebuild(const EngineDev &E, int t0, int nt, int r0, int nr, hipStream_t s) {
   if (nt <= 0) return;
   const int gr = nr < 1024 ? nr : 1024;
-  hipLaunchKernelGGL(k_rb_small, dim3(gr), dim3(1024), 0, s, E, r0, nr);
-  if (any_large) {
-    const int g = grid_for(nt, 8);
-    hipLaunchKernelGGL(k_rb_bin, dim3(g), dim3(kTile), 0, s, E, t0, nt);
-    hipLaunchKernelGGL(k_rb_scan, dim3(gr), dim3(1024), 0, s, E, r0, nr);
-    hipLaunchKernelGGL(k_rb_scatter, dim3(g), dim3(kTile), 0, s, E, t0, nt);
-    hipLaunchKernelGGL(k_rb_rank, dim3(g), dim3(kTile), 0, s, E, t0, nt);
-    hipLaunchKernelGGL(k_rb_place, dim3(g), dim3(kTile), 0, s, E, t0, nt);
-  }
+  const int g = grid_for(nt, 8);
+  hipLaunchKernelGGL(k_rb_bin, dim3(g), dim3(kTile), 0, s, E, t0, nt);
+  hipLaunchKernelGGL(k_rb_scan, dim3(gr), dim3(1024), 0, s, E, r0, nr);
+  hipLaunchKernelGGL(k_rb_scatter, dim3(g), dim3(kTile), 0, s, E, t0, nt);
+  hipLaunchKernelGGL(k_rb_rank, dim3(g), dim3(kTile), 0, s, E, t0, nt);
+  hipLaunchKernelGGL(k_rb_place, dim3(g), dim3(kTile), 0, s, E, t0, nt);
   hipLaunchKernelGGL(k_rb_list, dim3(nt), dim3(64 * kListWaves), 0, s, E, t0, nt);
 }
 
