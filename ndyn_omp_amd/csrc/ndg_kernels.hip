@@ -161,7 +161,7 @@ __device__ inline void bin_axis(double p, int m, int brute, int &k, int &c) {
 // ------------------------------------------------------------------ integrator (mixed)
 
 // integrate1 (src/nd.f90:1029-1046) with the pending thermostat scale of the previous
-// step folded in; writes the FP32 cell-anchored copy and the displacement top-2.
+// step folded in; writes the packed cell-anchored copy (xq) and the displacement top-2.
 __global__ __launch_bounds__(kTile) void k_drift_mixed(EngineDev E, int t0, int nt, int xflags) {
   __shared__ double sh[8];
   int tb, te, ts;
@@ -197,7 +197,8 @@ __global__ __launch_bounds__(kTile) void k_drift_mixed(EngineDev E, int t0, int 
       const double xl = anchored(px, A.x, m0, S.box[0] / m0);
       const double yl = anchored(py, A.y, m1, S.box[1] / m1);
       const double zl = anchored(pz, A.z, m2, S.box[2] / m2);
-      R.x64a[i] = make_double4(xl, yl, zl, type_bits(ty));
+      // x64a (FP64 anchored) is written only at a rebuild, for the list builder; the
+      // force kernel reads the packed copy
       R.xq[i] = xq_pack(xl, yl, zl, ty, xq_unit(S, R.skin));
       const float4 B = R.x32b[i];
       const double dx = (double)((float)xl - B.x), dy = (double)((float)yl - B.y),
@@ -1121,6 +1122,8 @@ struct alignas(16) HomeRec {
 struct ListWave {
   HomeRec home[kChunk];
   int lo[27], pref[28];
+  unsigned long long mask[kChunk];   // accepted candidates of the batch, per home atom
+  unsigned ent[64];                  // list entry of each candidate of the batch
 };
 
 __device__ inline unsigned long long ballot64(bool p) { return __ballot(p); }
@@ -1268,8 +1271,9 @@ __device__ long long list_chunk(const RepParams &R, RepState &S, ListWave &L, co
       const int dxq = q % 3 - 1, dyq = (q / 3) % 3 - 1, dzq = q / 9 - 1;
       // the candidate in the frame of the home cell
       const double ux_ = xu.x + dxq * w0, uy_ = xu.y + dyq * w1, uz_ = xu.z + dzq * w2;
-      const unsigned ent = (unsigned)u |
+      L.ent[lane] = (unsigned)u |
           ((unsigned)((1 - dxq) | ((1 - dyq) << 2) | ((1 - dzq) << 4)) << kCodeShift);
+      // test: one ballot of the batch per home atom (no appends here)
       for (int h = 0; h < cnt; ++h) {
         const HomeRec &hr = L.home[h];
         const int ih = lo + h;
@@ -1301,8 +1305,28 @@ __device__ long long list_chunk(const RepParams &R, RepState &S, ListWave &L, co
           }
           acc |= ballot64(ok);
         }
-        append(C, nv, h, ih, acc, ent, false);
+        L.mask[h] = acc;   // the same value from every lane
       }
+      wave_sync();
+      // append: lane = home atom, the set bits of its mask in lane (= stream) order
+      if (lane < cnt && nv >= 0) {
+        unsigned long long m = L.mask[lane];
+        if (nv + __popcll(m) > C.neimax) {
+          nv = -1;
+        } else {
+          const int ih = lo + lane;
+          const auto bh = gmem(C.list + (size_t)(ih >> 7) * C.tstride + (ih & (kTile - 1)) * C.tpa);
+          int k = nv;
+          while (m) {
+            const int b = __builtin_ctzll(m);
+            m &= m - 1;
+            bh[(((unsigned)k >> C.tsh) << (7 + C.tsh)) | ((unsigned)k & (unsigned)(C.tpa - 1))] = L.ent[b];
+            ++k;
+          }
+          nv = k;
+        }
+      }
+      wave_sync();   // the next batch rewrites mask and ent
     } else {
       const double pxu = gpx[u], pyu = gpy[u], pzu = gpz[u];
       const auto ga = gmem((const int *)R.anchor) + 4 * u;
