@@ -261,8 +261,11 @@ def main():
     ap.add_argument("--tpa", type=int, default=0)
     ap.add_argument("--skin", type=float, default=None,
                     help="override the workload's list skin (A)")
-    ap.add_argument("--no-tune", action="store_true",
-                    help="skip the autotuner (TPA per size class; skin too for the mix)")
+    ap.add_argument("--tune", choices=("saved", "run", "off"), default="saved",
+                    help="autotuner table (TPA per size class; skin too for the mix): the "
+                         "one saved for this workload (tuned on MI355X), a fresh run, or none")
+    ap.add_argument("--no-tune", action="store_true", help="same as --tune off")
+    ap.add_argument("--save-tune", default=None, help="write the table used to this file")
     ap.add_argument("--cpu-budget-s", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -292,22 +295,34 @@ def main():
     atoms = sum(s.np for s in systems)
     log(f"bench: workload {args.workload}: {len(systems)} systems, {atoms} atoms")
 
-    table, tune_s = None, 0.0
-    if not args.no_tune and args.tpa == 0:
+    table, tune_s, tune_src = None, 0.0, None
+    mode = "off" if args.no_tune or args.tpa != 0 else args.tune
+    saved = os.path.join(ROOT, "ndyn_omp_amd", "tuning", f"{args.workload}.json")
+    if mode == "saved" and not os.path.exists(saved):
+        mode = "run"
+    if mode != "off":
         from ndyn_omp_amd import autotune as AT
-        # the replica mix leaves the skin open; the single systems keep their configured one
-        skins = AT.DEFAULT_SKINS if args.workload == "c3" and args.skin is None else (cfg["skin"],)
-        # single systems: one class holds the whole batch
-        classes = AT.DEFAULT_CLASSES if len(systems) > 1 else (1 << 26,)
-        t_tune = time.perf_counter()
-        table = AT.tune(systems, lambda skin, n: make_rp(dict(cfg, skin=skin), n), skins=skins,
-                        classes=classes, log=log)
-        tune_s = time.perf_counter() - t_tune
-        log(f"bench: autotune {table.summary()} in {tune_s:.1f} s")
+        if mode == "saved":
+            table = AT.TuneTable.from_json(open(saved).read())
+            tune_src = os.path.relpath(saved, ROOT)
+        else:
+            # the replica mix leaves the skin open; single systems keep their configured one
+            skins = AT.DEFAULT_SKINS if args.workload == "c3" and args.skin is None else (cfg["skin"],)
+            # single systems: one class holds the whole batch
+            classes = AT.DEFAULT_CLASSES if len(systems) > 1 else (1 << 26,)
+            t_tune = time.perf_counter()
+            table = AT.tune(systems, lambda skin, n: make_rp(dict(cfg, skin=skin), n),
+                            skins=skins, classes=classes, log=log)
+            tune_s = time.perf_counter() - t_tune
+            tune_src = f"tuned in this run ({tune_s:.1f} s)"
+        log(f"bench: autotune {table.summary()} ({tune_src})")
+        if args.save_tune:
+            with open(args.save_tune, "w") as f:
+                f.write(table.to_json())
     eng, slots, mixed = run_engine(systems, cfg, G.PATH_MIXED, args.warmup, args.steps,
                                    args.timing_steps, args.graph_steps, args.tpa, table)
     # the measured traffic belongs to the default settings of a workload only
-    default_run = args.skin is None and args.tpa == 0 and not args.no_tune
+    default_run = args.skin is None and args.tpa == 0 and mode == "saved"
     mixed["workload"] = args.workload if default_run else None
     log(f"bench: mixed {mixed['atom_steps_per_s']:.4e} atom-steps/s, "
         f"{mixed['ms_per_step']:.3f} ms/step, force {mixed.get('force_ms', 0):.4f} ms, "
@@ -359,7 +374,7 @@ def main():
             "force_rms_rel_err_vs_fp64": ferr,
             "graph_steps": args.graph_steps,
             "autotune": table.summary() if table else None,
-            "autotune_s": round(tune_s, 1),
+            "autotune_source": tune_src,
             "plain_fp64_path": f"one system per launch, skin {cfg['skin']} A",
             "list_rebuilds_per_step": round(mixed["rebuilds_per_step"], 3),
             "rebuilt_atoms_per_step": round(mixed["rebuilt_atoms_per_step"], 1),

@@ -2,9 +2,10 @@
 
     python -m ndyn_omp_amd.trace_summary <prefix>_kernel_trace.csv [--pmc fetch.csv write.csv]
 
-The window runs from the first to the last launch of `k_drift_mixed` whose grid is the
-largest seen (the batched launches over every replica), which leaves out admission-time
-list builds and the single-replica launches of the force-error check.  For each kernel it
+The window is the last contiguous run of `k_drift_mixed` launches with the largest grid
+seen (the batched launches over every replica; runs are split where launches are more
+than 50 ms apart), which leaves out autotuner trials, admission-time list builds and the
+single-replica launches of the force-error check.  For each kernel it
 prints launches, total, average and per-step time (steps = batched drift launches).
 
 With --pmc, it also prints FETCH_SIZE / WRITE_SIZE per dispatch (KiB) of each kernel from
@@ -31,12 +32,19 @@ def load_trace(path):
     return rows
 
 
-def window(rows, anchor="k_drift_mixed"):
+def window(rows, anchor="k_drift_mixed", gap_ms=50.0):
+    """The last contiguous run of batched steps: largest-grid anchor launches, split where
+    consecutive ones are more than gap_ms apart (admission of a new batch between runs)."""
     drifts = [r for r in rows if r["name"] == anchor]
     if not drifts:
         return rows, 0, 0
     gmax = max(r["grid"] for r in drifts)
     big = [r for r in drifts if r["grid"] == gmax]
+    start = 0
+    for k in range(1, len(big)):
+        if (big[k]["t0"] - big[k - 1]["t1"]) * 1e-6 > gap_ms:
+            start = k
+    big = big[start:]
     t_lo, t_hi = big[0]["t0"], big[-1]["t1"]
     last_kick = [r for r in rows if r["name"] == "k_kick_mixed" and r["grid"] == gmax
                  and r["t0"] >= t_hi]
