@@ -32,7 +32,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_force_mixed launch, per workload, from rocprofv3 FETCH_SIZE + WRITE_SIZE
 # scaled by the same counters on k_copy of known size (1 GiB read + 1 GiB written):
 # FETCH_SIZE x 2.000, WRITE_SIZE x 1.000.  Source: profiles/k_force_mixed_c3_traffic.txt.
-TRAFFIC_BYTES_PER_LAUNCH = {}
+TRAFFIC_BYTES_PER_LAUNCH = {"c3": 9.2613e8}
 METRIC = "atom-steps/s vs plain FP64 GPU path at <=1e-5 rel force err; p50/p99 job latency"
 
 
@@ -51,7 +51,8 @@ def workload_systems(name):
         systems = W.replica_mix(512)
         desc = ("512 uneven replicas: atoms log-uniform [500,20000] in whole FCC cells, "
                 "Morse Cu (a=3.615) or Buckingham alloy (a=5.6, 50/50 27/63.5 amu), "
-                "T~U[100,1200] K, dt 1 fs, NVE, skin 0.5 A")
+                "T~U[100,1200] K, dt 1 fs, NVE, list skin per size class from the autotuner "
+                "table (0.5 A without it)")
         return systems, desc, dict(dt=W.DT_1FS, skin=0.5)
     if name == "c1":
         s = W.ar_lj(40)
