@@ -25,8 +25,7 @@ void launch_force(const EngineDev &E, int path, int tpa, bool full, int t0, int 
 void launch_reduce(const EngineDev &E, int r0, int nr, int mode, int xflags, int trigger,
                    hipStream_t s);
 void launch_apply_pending(const EngineDev &E, int r0, int nr, hipStream_t s);
-void launch_rebuild(const EngineDev &E, int t0, int nt, int r0, int nr,
-                    hipStream_t s);
+void launch_rebuild(const EngineDev &E, int nt, int r0, int nr, hipStream_t s);
 void launch_copy(const float4 *src, float4 *dst, size_t n, hipStream_t s);
 enum : int { RED_FORCE = 1, RED_KICK = 2, RED_FLIP = 4, RED_STEP = 8, RED_CHECK = 16 };
 }  // namespace ndg
@@ -176,7 +175,9 @@ struct ndg_engine {
   RepParams *d_reps = nullptr;
   RepState *d_st = nullptr;
   Tile *d_tiles = nullptr;
-  int cap_slots = 0, cap_tiles = 0;
+  int *d_rb = nullptr;        // rebuild work list (count + tile indices), cap_tiles + 1 ints
+  int *d_tile0 = nullptr;     // first tile of each slot
+  int cap_slots = 0, cap_tiles = 0, cap_tile0 = 0;
   std::vector<Tile> h_tiles;
   std::vector<std::pair<int, int>> tpa_ranges;  // (tpa, t0) in tile order, nt follows
   std::vector<int> tpa_counts;
@@ -459,10 +460,26 @@ static int rebuild_tiles(ndg_engine *e) {
     if (e->d_tiles) {
       HIPCHK(hipStreamSynchronize(e->stream));
       hipFree(e->d_tiles);
+      hipFree(e->d_rb);
     }
     e->cap_tiles = std::max(nt, 2 * e->cap_tiles);
     HIPCHK(hipMalloc(&e->d_tiles, sizeof(Tile) * e->cap_tiles));
+    HIPCHK(hipMalloc(&e->d_rb, sizeof(int) * (e->cap_tiles + 1)));
   }
+  const int ns = (int)e->slots.size();
+  if (ns > e->cap_tile0) {
+    if (e->d_tile0) {
+      HIPCHK(hipStreamSynchronize(e->stream));
+      hipFree(e->d_tile0);
+    }
+    e->cap_tile0 = std::max(ns, 2 * e->cap_tile0);
+    HIPCHK(hipMalloc(&e->d_tile0, sizeof(int) * e->cap_tile0));
+  }
+  std::vector<int> tile0(ns, -1);
+  for (int r = 0; r < ns; ++r)
+    if (e->slots[r].used) tile0[r] = e->slots[r].tile_base;
+  if (ns > 0)
+    HIPCHK(hipMemcpyAsync(e->d_tile0, tile0.data(), sizeof(int) * ns, hipMemcpyHostToDevice, e->stream));
   if (nt > 0)
     HIPCHK(hipMemcpyAsync(e->d_tiles, e->h_tiles.data(), sizeof(Tile) * nt, hipMemcpyHostToDevice, e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
@@ -477,6 +494,8 @@ static EngineDev dev_view(ndg_engine *e) {
   E.reps = e->d_reps;
   E.st = e->d_st;
   E.tiles = e->d_tiles;
+  E.tile0 = e->d_tile0;
+  E.rb = e->d_rb;
   E.ntiles = (int)e->h_tiles.size();
   E.nreps = (int)e->slots.size();
   E.tabset = e->d_tabset;
@@ -816,7 +835,7 @@ static int build_slot_now(ndg_engine *e, int r, double skin) {
   if ((rc = upload_params(e, r))) return rc;
   if ((rc = put_state_dev(e, r, &S))) return rc;
   EngineDev E = dev_view(e);
-  launch_rebuild(E, s.tile_base, s.ntiles, r, 1, e->stream);
+  launch_rebuild(E, s.ntiles, r, 1, e->stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(e->stream));
   if ((rc = get_state_dev(e, r, &S))) return rc;
@@ -918,7 +937,7 @@ static void enqueue_step_batched(ndg_engine *e, const TickPlan &tp) {
   EngineDev E = dev_view(e);
   const int nt = (int)e->h_tiles.size(), nr = (int)e->slots.size();
   hipStream_t s = e->stream;
-  if (tp.check) launch_rebuild(E, 0, nt, 0, nr, s);
+  if (tp.check) launch_rebuild(E, nt, 0, nr, s);
   launch_drift(E, e->path, 0, nt, 0, s);
   hipEvent_t a = nullptr;
   time_begin(e, &a);
@@ -943,7 +962,7 @@ static void enqueue_step_single(ndg_engine *e, int r, const TickPlan &tp) {
   S.step = sl.step;
   const int f = step_flags(sl.P, S);
   if (!(f & F_ACTIVE)) return;
-  if (f & F_CHECK) launch_rebuild(E, sl.tile_base, sl.ntiles, r, 1, s);
+  if (f & F_CHECK) launch_rebuild(E, sl.ntiles, r, 1, s);
   launch_drift(E, e->path, sl.tile_base, sl.ntiles, 0, s);
   hipEvent_t a = nullptr;
   time_begin(e, &a);
@@ -1137,6 +1156,8 @@ extern "C" int ndg_engine_destroy(ndg_engine_t *e) {
   if (e->d_reps) hipFree(e->d_reps);
   if (e->d_st) hipFree(e->d_st);
   if (e->d_tiles) hipFree(e->d_tiles);
+  if (e->d_rb) hipFree(e->d_rb);
+  if (e->d_tile0) hipFree(e->d_tile0);
   if (e->d_tabset) hipFree(e->d_tabset);
   hipStreamDestroy(e->stream);
   delete e;

@@ -979,14 +979,66 @@ __device__ inline void sort_cell_wave(const RepParams &R, const RepState &S, int
   }
 }
 
+// Work list of a rebuild: the tiles of every replica in [r0, r0 + nr) that rebuilds in
+// this step, in slot order (rb[0] = count, rb[1 ..] = tile indices).  The rebuild kernels
+// below walk this list, so their workgroups never visit the tiles of replicas that keep
+// their lists (on a batch where a few replicas rebuild per step, those would be most).
+__global__ __launch_bounds__(1024) void k_rb_plan(EngineDev E, int r0, int nr) {
+  __shared__ int sh_n[1024], sh_t0[1024];
+  __shared__ int sh_base;
+  const int tid = threadIdx.x, nb = blockDim.x;
+  if (tid == 0) sh_base = 0;
+  __syncthreads();
+  for (int c0 = r0; c0 < r0 + nr; c0 += nb) {
+    const int r = c0 + tid;
+    int n = 0, tb = 0;
+    if (r < r0 + nr) {
+      const int t0 = E.tile0[r];
+      const RepState &S = E.st[r];
+      if (t0 >= 0 && S.rebuild && !S.err) {
+        n = E.reps[r].ntiles;
+        tb = t0;
+      }
+    }
+    sh_n[tid] = n;
+    sh_t0[tid] = tb;
+    __syncthreads();
+    // inclusive scan of the chunk's tile counts
+    for (int off = 1; off < nb; off <<= 1) {
+      const int v = tid >= off ? sh_n[tid - off] : 0;
+      __syncthreads();
+      sh_n[tid] += v;
+      __syncthreads();
+    }
+    const int base = sh_base, tot = sh_n[nb - 1];
+    // output position q of the chunk belongs to the first replica j with sh_n[j] > q
+    for (int q = tid; q < tot; q += nb) {
+      int lo = 0, hi = nb - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (sh_n[mid] > q) hi = mid; else lo = mid + 1;
+      }
+      const int first = lo == 0 ? 0 : sh_n[lo - 1];
+      E.rb[1 + base + q] = sh_t0[lo] + (q - first);
+    }
+    __syncthreads();
+    if (tid == 0) sh_base = base + tot;
+    __syncthreads();
+  }
+  if (tid == 0) E.rb[0] = sh_base;
+}
+
+// Tile k of the rebuild work list for this workgroup (uniform), or -1 past its end.
+__device__ inline int rb_tile(const EngineDev &E, int k) {
+  return k < __builtin_amdgcn_readfirstlane(E.rb[0]) ? __builtin_amdgcn_readfirstlane(E.rb[1 + k]) : -1;
+}
+
 // Cell sort of the rebuilding replicas in five grid-wide phases over their tiles:
 // histogram, per-replica scan, scatter, per-cell order (one wave per cell), placement.
 // (A whole replica sorted inside one workgroup, with its bookkeeping in LDS, was slower
 // on the replica mix at every size: one workgroup serialises what the tiles share.)
-__global__ __launch_bounds__(kTile) void k_rb_bin(EngineDev E, int t0, int nt) {
-  int tb, te, ts;
-  tile_range(t0, nt, tb, te, ts);
-  for (int t = tb; t < te; t += ts) {
+__global__ __launch_bounds__(kTile) void k_rb_bin(EngineDev E) {
+  for (int k = blockIdx.x, t; (t = rb_tile(E, k)) >= 0; k += gridDim.x) {
     const Tile T = E.tiles[t];
     const RepParams &R = E.reps[T.rep];
     const RepState &S = E.st[T.rep];
@@ -1035,10 +1087,8 @@ __global__ __launch_bounds__(1024) void k_rb_scan(EngineDev E, int r0, int nr) {
   }
 }
 
-__global__ __launch_bounds__(kTile) void k_rb_scatter(EngineDev E, int t0, int nt) {
-  int tb, te, ts;
-  tile_range(t0, nt, tb, te, ts);
-  for (int t = tb; t < te; t += ts) {
+__global__ __launch_bounds__(kTile) void k_rb_scatter(EngineDev E) {
+  for (int k = blockIdx.x, t; (t = rb_tile(E, k)) >= 0; k += gridDim.x) {
     const Tile T = E.tiles[t];
     const RepParams &R = E.reps[T.rep];
     const RepState &S = E.st[T.rep];
@@ -1051,11 +1101,9 @@ __global__ __launch_bounds__(kTile) void k_rb_scatter(EngineDev E, int t0, int n
 }
 
 // Each tile sorts the cells whose first slot lies in it (one wave per cell).
-__global__ __launch_bounds__(kTile) void k_rb_rank(EngineDev E, int t0, int nt) {
+__global__ __launch_bounds__(kTile) void k_rb_rank(EngineDev E) {
   __shared__ int sh_sort[kTile / 64][2 * kCellLds];
-  int tb, te, ts;
-  tile_range(t0, nt, tb, te, ts);
-  for (int t = tb; t < te; t += ts) {
+  for (int k = blockIdx.x, t; (t = rb_tile(E, k)) >= 0; k += gridDim.x) {
     const Tile T = E.tiles[t];
     const RepParams &R = E.reps[T.rep];
     const RepState &S = E.st[T.rep];
@@ -1077,10 +1125,8 @@ __global__ __launch_bounds__(kTile) void k_rb_rank(EngineDev E, int t0, int nt) 
   }
 }
 
-__global__ __launch_bounds__(kTile) void k_rb_place(EngineDev E, int t0, int nt) {
-  int tb, te, ts;
-  tile_range(t0, nt, tb, te, ts);
-  for (int t = tb; t < te; t += ts) {
+__global__ __launch_bounds__(kTile) void k_rb_place(EngineDev E) {
+  for (int k = blockIdx.x, t; (t = rb_tile(E, k)) >= 0; k += gridDim.x) {
     const Tile T = E.tiles[t];
     const RepParams &R = E.reps[T.rep];
     const RepState &S = E.st[T.rep];
@@ -1182,6 +1228,91 @@ __device__ inline void append(const ListCtx &C, int &nv, int h, int ih, unsigned
   }
 }
 
+// Cutoff bands and box of the anchored test.
+struct BandCtx {
+  double c2u, c2lo_u, c2hi_u;    // uniform list cutoff^2 and its band
+  double pmlo, pmhi, ptrmin;     // PTrMin band (same-cell pairs)
+  double b0, b1, b2;             // box (reference re-test)
+};
+
+// Test phase of one candidate batch on the anchored path: L.mask[h] = the lanes
+// (candidates) that enter the list of home h, for every home of the chunk.  Homes are
+// taken kH at a time: their LDS records are read together, then kH independent FP64
+// chains run; a candidate within the band of a cutoff is decided by the reference's
+// formula (rare, after the group).  SAME: the batch holds same-cell candidates (PTrMin
+// band); UNI: one list cutoff for every type pair.
+constexpr int kH = 4;
+
+template <bool SAME, bool UNI>
+__device__ inline void test_batch(ListWave &L, const __attribute__((address_space(1))) double *gpx,
+                                  const __attribute__((address_space(1))) double *gpy,
+                                  const __attribute__((address_space(1))) double *gpz,
+                                  const BandCtx &B, int lo, int cnt, int u, bool same,
+                                  double ux, double uy, double uz, int tj,
+                                  unsigned long long valid, unsigned long long samem) {
+  const int lane = threadIdx.x & 63;
+  for (int h0 = 0; h0 < cnt; h0 += kH) {
+    double hx[kH], hy[kH], hz[kH];
+    int hth[kH];
+#pragma unroll
+    for (int k = 0; k < kH; ++k) {
+      const HomeRec &hr = L.home[min(h0 + k, cnt - 1)];
+      hx[k] = hr.x;
+      hy[k] = hr.y;
+      hz[k] = hr.z;
+      hth[k] = hr.th;
+    }
+    unsigned long long acc[kH], amb[kH];
+#pragma unroll
+    for (int k = 0; k < kH; ++k) {
+      const int ih = lo + min(h0 + k, cnt - 1);
+      const double dx = hx[k] - ux, dy = hy[k] - uy, dz = hz[k] - uz;
+      const double d2 = fma(dx, dx, fma(dy, dy, dz * dz));
+      unsigned long long live = valid & ~ballot64(u == ih);
+      double c2lo = B.c2lo_u, c2hi = B.c2hi_u;
+      if (!UNI) {
+        const double c2 = s_lcut2[(hth[k] - 1) * 8 + tj - 1];
+        c2lo = c2 - 1e-9 * fabs(c2);
+        c2hi = c2 + 1e-9 * fabs(c2);
+        live &= ballot64(c2 >= 0.0);
+      }
+      const unsigned long long lo_m = ballot64(d2 < c2lo), hi_m = ballot64(d2 <= c2hi);
+      unsigned long long a = live & lo_m, m = live & hi_m & ~lo_m;
+      if (SAME) {
+        const unsigned long long plo = ballot64(d2 < B.pmlo), phi = ballot64(d2 <= B.pmhi);
+        m = (m & ~(samem & plo)) | (a & samem & phi & ~plo);
+        a &= ~(samem & phi);
+      }
+      acc[k] = a;
+      amb[k] = m;
+    }
+    unsigned long long any = 0;
+#pragma unroll
+    for (int k = 0; k < kH; ++k) any |= amb[k];
+    if (any) {
+      for (int k = 0; k < kH && h0 + k < cnt; ++k) {
+        if (!amb[k]) continue;
+        const HomeRec &hr = L.home[h0 + k];
+        bool ok = false;
+        if ((amb[k] >> lane) & 1ull) {
+          double c2 = B.c2u;
+          if (!UNI) c2 = s_lcut2[(hr.th - 1) * 8 + tj - 1];
+          double nx, ny, nz;
+          const double e2 = ref_d2(hr.px, hr.py, hr.pz, gpx[u], gpy[u], gpz[u], B.b0, B.b1,
+                                   B.b2, nx, ny, nz);
+          ok = !(same && e2 < B.ptrmin) && !(e2 > c2);
+        }
+        acc[k] |= ballot64(ok);
+      }
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < kH; ++k)
+        if (h0 + k < cnt) L.mask[h0 + k] = acc[k];
+    }
+  }
+}
+
 __device__ long long list_chunk(const RepParams &R, RepState &S, ListWave &L, const ListCtx &C,
                                 bool anch, int lo, int cnt, int cell) {
   const int lane = threadIdx.x & 63;
@@ -1238,17 +1369,24 @@ __device__ long long list_chunk(const RepParams &R, RepState &S, ListWave &L, co
   const double ptrmin = C.ptrmin;
   const bool uni = C.uni;
   const double c2u = C.c2u;
-  const double c2lo_u = c2u - 1e-9 * fabs(c2u), c2hi_u = c2u + 1e-9 * fabs(c2u);
+  BandCtx B;
+  B.c2u = c2u;
+  B.c2lo_u = c2u - 1e-9 * fabs(c2u);
+  B.c2hi_u = c2u + 1e-9 * fabs(c2u);
   // PTrMin band: the same width as the cutoff band of the largest cutoff
-  const double pmlo = ptrmin - 1e-9 * fabs(c2u), pmhi = ptrmin + 1e-9 * fabs(c2u);
+  B.pmlo = ptrmin - 1e-9 * fabs(c2u);
+  B.pmhi = ptrmin + 1e-9 * fabs(c2u);
+  B.ptrmin = ptrmin;
+  B.b0 = b0;
+  B.b1 = b1;
+  B.b2 = b2;
   const auto gx64a = gmem((const double *)R.x64a);
   const auto gpx = gmem(C.px), gpy = gmem(C.py), gpz = gmem(C.pz);
-  int nv = 0;   // lane h: list length of home h, -1 after an overflow
-  for (int g0 = 0; g0 < total; g0 += 64) {
-    const int g = g0 + lane;
-    const bool inb = g < total;
-    int u = lo, q = 13;
-    if (inb) {
+  // slot u and stencil cell q of stream position g (u = lo, q = 13 past the end)
+  auto locate = [&](int g, int &u, int &q) {
+    u = lo;
+    q = 13;
+    if (g < total) {
       if (brute) {
         u = g;
       } else {
@@ -1261,11 +1399,26 @@ __device__ long long list_chunk(const RepParams &R, RepState &S, ListWave &L, co
         u = L.lo[q] + (g - L.pref[q]);
       }
     }
+  };
+  auto load_x = [&](int u) {
+    return make_double4(gx64a[4 * u], gx64a[4 * u + 1], gx64a[4 * u + 2], gx64a[4 * u + 3]);
+  };
+  int nv = 0;   // lane h: list length of home h, -1 after an overflow
+  // the next batch's candidates are located and loaded while this batch is tested
+  int u, q;
+  locate(lane, u, q);
+  double4 xu = load_x(u);
+  for (int g0 = 0; g0 < total; g0 += 64) {
+    const bool inb = g0 + lane < total;
     const bool same = !brute && q == 13;
     const unsigned long long valid = ballot64(inb);
     const unsigned long long samem = ballot64(inb && same);
-    const double4 xu = make_double4(gx64a[4 * u], gx64a[4 * u + 1], gx64a[4 * u + 2],
-                                    gx64a[4 * u + 3]);
+    int un = lo, qn = 13;
+    double4 xn = xu;
+    if (g0 + 64 < total) {
+      locate(g0 + 64 + lane, un, qn);
+      xn = load_x(un);
+    }
     const int tj = bits_type(xu.w);
     if (anch) {
       const int dxq = q % 3 - 1, dyq = (q / 3) % 3 - 1, dzq = q / 9 - 1;
@@ -1274,41 +1427,16 @@ __device__ long long list_chunk(const RepParams &R, RepState &S, ListWave &L, co
       L.ent[lane] = (unsigned)u |
           ((unsigned)((1 - dxq) | ((1 - dyq) << 2) | ((1 - dzq) << 4)) << kCodeShift);
       // test: one ballot of the batch per home atom (no appends here)
-      for (int h = 0; h < cnt; ++h) {
-        const HomeRec &hr = L.home[h];
-        const int ih = lo + h;
-        const double dx = hr.x - ux_, dy = hr.y - uy_, dz = hr.z - uz_;
-        const double d2 = fma(dx, dx, fma(dy, dy, dz * dz));
-        double c2 = c2u, c2lo = c2lo_u, c2hi = c2hi_u;
-        if (!uni) {
-          c2 = s_lcut2[(__builtin_amdgcn_readfirstlane(hr.th) - 1) * 8 + tj - 1];
-          c2lo = c2 - 1e-9 * fabs(c2);
-          c2hi = c2 + 1e-9 * fabs(c2);
-        }
-        const unsigned long long live = valid & ~ballot64(u == ih) & ballot64(c2 >= 0.0);
-        const unsigned long long lo_m = ballot64(d2 < c2lo);
-        const unsigned long long hi_m = ballot64(d2 <= c2hi);
-        unsigned long long acc = live & lo_m, amb = live & hi_m & ~lo_m;
-        if (samem) {
-          const unsigned long long plo = ballot64(d2 < pmlo);
-          const unsigned long long phi = ballot64(d2 <= pmhi);
-          amb = (amb & ~(samem & plo)) | (acc & samem & phi & ~plo);
-          acc &= ~(samem & phi);
-        }
-        if (amb) {
-          bool ok = false;
-          if ((amb >> lane) & 1ull) {
-            double nx, ny, nz;
-            const double e2 = ref_d2(hr.px, hr.py, hr.pz, gpx[u], gpy[u], gpz[u], b0, b1,
-                                     b2, nx, ny, nz);
-            ok = !(same && e2 < ptrmin) && !(e2 > c2);
-          }
-          acc |= ballot64(ok);
-        }
-        L.mask[h] = acc;   // the same value from every lane
+      if (uni) {
+        if (samem) test_batch<true, true>(L, gpx, gpy, gpz, B, lo, cnt, u, same, ux_, uy_, uz_, tj, valid, samem);
+        else test_batch<false, true>(L, gpx, gpy, gpz, B, lo, cnt, u, same, ux_, uy_, uz_, tj, valid, samem);
+      } else {
+        if (samem) test_batch<true, false>(L, gpx, gpy, gpz, B, lo, cnt, u, same, ux_, uy_, uz_, tj, valid, samem);
+        else test_batch<false, false>(L, gpx, gpy, gpz, B, lo, cnt, u, same, ux_, uy_, uz_, tj, valid, samem);
       }
       wave_sync();
-      // append: lane = home atom, the set bits of its mask in lane (= stream) order
+      // append: lane = home atom, the set bits of its mask in lane (= stream) order, up to
+      // four entries per step (their LDS reads in flight together)
       if (lane < cnt && nv >= 0) {
         unsigned long long m = L.mask[lane];
         if (nv + __popcll(m) > C.neimax) {
@@ -1316,14 +1444,26 @@ __device__ long long list_chunk(const RepParams &R, RepState &S, ListWave &L, co
         } else {
           const int ih = lo + lane;
           const auto bh = gmem(C.list + (size_t)(ih >> 7) * C.tstride + (ih & (kTile - 1)) * C.tpa);
-          int k = nv;
+          const unsigned msk = (unsigned)(C.tpa - 1), sh = (unsigned)C.tsh;
+          unsigned k = (unsigned)nv;
           while (m) {
-            const int b = __builtin_ctzll(m);
-            m &= m - 1;
-            bh[(((unsigned)k >> C.tsh) << (7 + C.tsh)) | ((unsigned)k & (unsigned)(C.tpa - 1))] = L.ent[b];
-            ++k;
+            int b[4];
+            int nb = 0;
+#pragma unroll
+            for (int z = 0; z < 4; ++z) {
+              b[z] = m ? __builtin_ctzll(m) : 0;
+              nb += m ? 1 : 0;
+              m &= m - 1;
+            }
+            unsigned ev[4];
+#pragma unroll
+            for (int z = 0; z < 4; ++z) ev[z] = L.ent[b[z]];
+#pragma unroll
+            for (int z = 0; z < 4; ++z)
+              if (z < nb) bh[(((k + z) >> sh) << (7 + sh)) | ((k + z) & msk)] = ev[z];
+            k += (unsigned)nb;
           }
-          nv = k;
+          nv = (int)k;
         }
       }
       wave_sync();   // the next batch rewrites mask and ent
@@ -1350,6 +1490,9 @@ __device__ long long list_chunk(const RepParams &R, RepState &S, ListWave &L, co
         append(C, nv, h, ih, ballot64(ok), ent, ballot64(bad) != 0);
       }
     }
+    u = un;
+    q = qn;
+    xu = xn;
   }
   wave_sync();   // the home records are read by every lane above
   const bool act = lane < cnt;
@@ -1360,15 +1503,11 @@ __device__ long long list_chunk(const RepParams &R, RepState &S, ListWave &L, co
   return mine;
 }
 
-// One workgroup per tile: the chunks of the cells whose first slot lies in the tile (the
-// tile's own slots in brute mode), dealt to its waves in turn.  Workgroups of tiles that
-// need no rebuild exit at once, so the hardware spreads the rebuilding ones over the chip.
-__global__ __launch_bounds__(64 * kListWaves) void k_rb_list(EngineDev E, int t0, int nt) {
-  __shared__ ListWave s_wave[kListWaves];
-  __shared__ double s_cmax, s_cmin;
+// One tile of the list build, by the whole workgroup: the chunks of the cells whose first
+// slot lies in the tile (the tile's own slots in brute mode), dealt to its waves in turn.
+__device__ inline void list_tile(const EngineDev &E, int t, ListWave &L, double &s_cmax,
+                                 double &s_cmin) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int t = t0 + blockIdx.x;
-  if (t >= t0 + nt) return;
   const Tile T = E.tiles[t];
   const RepParams &R = E.reps[T.rep];
   RepState &S = E.st[T.rep];
@@ -1413,7 +1552,6 @@ __global__ __launch_bounds__(64 * kListWaves) void k_rb_list(EngineDev E, int t0
   C.pz = R.pz;
   C.uni = s_cmax == s_cmin;
   C.c2u = s_cmax;
-  ListWave &L = s_wave[wid];
   long long mine = 0;
   if (S.brute) {
     constexpr int kb = kChunk / 2;
@@ -1446,6 +1584,17 @@ __global__ __launch_bounds__(64 * kListWaves) void k_rb_list(EngineDev E, int t0
     }
   }
   if (lane == 0 && mine) atomicAdd((unsigned long long *)&S.nbr_sum, (unsigned long long)mine);
+}
+
+// At most 128 VGPRs (four waves per SIMD): two workgroups of a CU build lists at once.
+__global__ __launch_bounds__(64 * kListWaves) __attribute__((amdgpu_waves_per_eu(4)))
+void k_rb_list(EngineDev E) {
+  __shared__ ListWave s_wave[kListWaves];
+  __shared__ double s_cmax, s_cmin;
+  for (int k = blockIdx.x, t; (t = rb_tile(E, k)) >= 0; k += gridDim.x) {
+    __syncthreads();   // the previous tile's waves are done with the shared cutoffs
+    list_tile(E, t, s_wave[threadIdx.x >> 6], s_cmax, s_cmin);
+  }
 }
 
 // ------------------------------------------------------------------ bandwidth probe
@@ -1531,16 +1680,18 @@ void launch_apply_pending(const EngineDev &E, int r0, int nr, hipStream_t s) {
   hipLaunchKernelGGL(k_apply_pending, dim3(nr < 1024 ? nr : 1024), dim3(256), 0, s, E, r0, nr);
 }
 
-void launch_rebuild(const EngineDev &E, int t0, int nt, int r0, int nr, hipStream_t s) {
-  if (nt <= 0) return;
+void launch_rebuild(const EngineDev &E, int nt, int r0, int nr, hipStream_t s) {
+  // nt: tiles held by the replicas [r0, r0 + nr), a bound on the work list's length
+  if (nt <= 0 || nr <= 0) return;
   const int gr = nr < 1024 ? nr : 1024;
   const int g = grid_for(nt, 8);
-  hipLaunchKernelGGL(k_rb_bin, dim3(g), dim3(kTile), 0, s, E, t0, nt);
+  hipLaunchKernelGGL(k_rb_plan, dim3(1), dim3(1024), 0, s, E, r0, nr);
+  hipLaunchKernelGGL(k_rb_bin, dim3(g), dim3(kTile), 0, s, E);
   hipLaunchKernelGGL(k_rb_scan, dim3(gr), dim3(1024), 0, s, E, r0, nr);
-  hipLaunchKernelGGL(k_rb_scatter, dim3(g), dim3(kTile), 0, s, E, t0, nt);
-  hipLaunchKernelGGL(k_rb_rank, dim3(g), dim3(kTile), 0, s, E, t0, nt);
-  hipLaunchKernelGGL(k_rb_place, dim3(g), dim3(kTile), 0, s, E, t0, nt);
-  hipLaunchKernelGGL(k_rb_list, dim3(nt), dim3(64 * kListWaves), 0, s, E, t0, nt);
+  hipLaunchKernelGGL(k_rb_scatter, dim3(g), dim3(kTile), 0, s, E);
+  hipLaunchKernelGGL(k_rb_rank, dim3(g), dim3(kTile), 0, s, E);
+  hipLaunchKernelGGL(k_rb_place, dim3(g), dim3(kTile), 0, s, E);
+  hipLaunchKernelGGL(k_rb_list, dim3(grid_for(nt, 16)), dim3(64 * kListWaves), 0, s, E);
 }
 
 }  // namespace ndg
