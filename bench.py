@@ -32,7 +32,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_force_mixed launch, per workload, from rocprofv3 FETCH_SIZE + WRITE_SIZE
 # scaled by the same counters on k_copy of known size (1 GiB read + 1 GiB written):
 # FETCH_SIZE x 2.000, WRITE_SIZE x 1.000.  Source: profiles/k_force_mixed_c3_traffic.txt.
-TRAFFIC_BYTES_PER_LAUNCH = {"c3": 9.2613e8}
+TRAFFIC_BYTES_PER_LAUNCH = {}
 METRIC = "atom-steps/s vs plain FP64 GPU path at <=1e-5 rel force err; p50/p99 job latency"
 
 
@@ -41,9 +41,11 @@ def log(msg):
 
 
 def force_bytes_per_atom(mean_nbr):
-    """Algorithmic HBM bytes of one force evaluation per atom: list entries (4 B each),
-    own coordinate+type (16 B), list length (4 B), force+energy out (16 B)."""
-    return 4.0 * mean_nbr + 36.0
+    """Algorithmic HBM bytes per atom of one launch of the batched force kernel, which also
+    applies the kick: list entries (4 B each), own coordinate+type (16 B), list length
+    (4 B), force+energy out (16 B); kick: previous force (16 B), type (4 B), velocity
+    read and written (48 B)."""
+    return 4.0 * mean_nbr + 36.0 + 68.0
 
 
 def workload_systems(name):

@@ -20,8 +20,9 @@
 namespace ndg {
 void launch_drift(const EngineDev &E, int path, int t0, int nt, int xflags, hipStream_t s);
 void launch_kick(const EngineDev &E, int path, int t0, int nt, int xflags, hipStream_t s);
+// kick != 0 (mixed path): the force kernel also applies integrate2 to the atoms it updates
 void launch_force(const EngineDev &E, int path, int tpa, bool full, int t0, int nt, int xflags,
-                  hipStream_t s);
+                  hipStream_t s, int kick = 0);
 void launch_reduce(const EngineDev &E, int r0, int nr, int mode, int xflags, int trigger,
                    hipStream_t s);
 void launch_apply_pending(const EngineDev &E, int r0, int nr, hipStream_t s);
@@ -843,8 +844,8 @@ static int build_slot_now(ndg_engine *e, int r, double skin) {
   // no displacement yet
   std::vector<double> part((size_t)s.ntiles * kPartials, 0.0);
   for (int lt = 0; lt < s.ntiles; ++lt) {
-    part[(size_t)lt * kPartials + 8] = -1.0;
-    part[(size_t)lt * kPartials + 9] = -1.0;
+    part[(size_t)kPartD1 * s.ntiles + lt] = -1.0;
+    part[(size_t)kPartD2 * s.ntiles + lt] = -1.0;
   }
   HIPCHK(hipMemcpyAsync(s.P.partials, part.data(), 8 * part.size(), hipMemcpyHostToDevice, e->stream));
   if ((rc = put_state_dev(e, r, &S))) return rc;
@@ -939,17 +940,19 @@ static void enqueue_step_batched(ndg_engine *e, const TickPlan &tp) {
   hipStream_t s = e->stream;
   if (tp.check) launch_rebuild(E, nt, 0, nr, s);
   launch_drift(E, e->path, 0, nt, 0, s);
+  // mixed path: the kick runs inside the force kernel
+  const int fused = e->path == NDG_PATH_MIXED;
   hipEvent_t a = nullptr;
   time_begin(e, &a);
   for (size_t q = 0; q < e->tpa_ranges.size(); ++q) {
     launch_force(E, e->path, e->tpa_ranges[q].first, false, e->tpa_ranges[q].second,
-                 e->tpa_counts[q], 0, s);
+                 e->tpa_counts[q], 0, s, fused);
     if (tp.full)
       launch_force(E, e->path, e->tpa_ranges[q].first, true, e->tpa_ranges[q].second,
-                   e->tpa_counts[q], 0, s);
+                   e->tpa_counts[q], 0, s, fused);
   }
   time_end(e, a);
-  launch_kick(E, e->path, 0, nt, 0, s);
+  if (!fused) launch_kick(E, e->path, 0, nt, 0, s);
   launch_reduce(E, 0, nr, RED_FORCE | RED_KICK | RED_STEP, 0, 0, s);
 }
 

@@ -20,7 +20,12 @@ constexpr int kCodeShift = 26;    // list entry = j | code << 26, code = image/c
 constexpr unsigned kIndexMask = (1u << kCodeShift) - 1u;
 constexpr int kListPad = 8;       // spare list chunks per tile (unrolled reads past the end)
 constexpr int kRing = 1024;       // thermo records kept per replica between host drains
-constexpr int kPartials = 12;     // doubles per tile: ke, pe, vir[6], d1, d2, spare[2]
+// Per-tile partial sums, stored quantity-major (partials[q * ntiles + tile]) so that the
+// reduction over a replica's tiles reads contiguous memory.  Quantities: kinetic energy
+// (plain path), potential energy, virial[6], displacement top-2, then the mixed path's
+// kinetic energy by wave of the tile (2 TPA waves; the force kernel's epilogue writes them).
+constexpr int kPartKe = 0, kPartPe = 1, kPartD1 = 8, kPartD2 = 9, kPartKeW = 10;
+constexpr int kPartials = kPartKeW + 16;
 
 // step-flag bits (src/nd.f90:138-148 and the ensemble frequencies)
 enum : int { F_OUT = 1, F_CV = 2, F_CU = 4, F_CK = 8, F_CHECK = 16, F_ACTIVE = 32 };
@@ -77,7 +82,7 @@ struct RepParams {
   double *tmpd;             // 9 * np
   float4 *tmpf;             // 2 * np (both force buffers)
   int *tmpi;                // 2 * np
-  double *partials;         // ntiles * kPartials
+  double *partials;         // kPartials * ntiles, quantity-major
   ndg_thermo *ring;         // kRing
 };
 

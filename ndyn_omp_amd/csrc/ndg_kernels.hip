@@ -53,6 +53,13 @@ __device__ inline double shfl_xor_d(double v, int m) {
   return *reinterpret_cast<double *>(&a);
 }
 
+// Sum over the 64 lanes of a wave (xor tree); the result is in every lane.
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += shfl_xor_d(v, m);
+  return v;
+}
+
 // Deterministic block sum of NV doubles per thread; result valid in thread 0.
 template <int NV>
 __device__ inline void block_sum(double (&v)[NV], double *sh) {
@@ -212,15 +219,15 @@ __global__ __launch_bounds__(kTile) void k_drift_mixed(EngineDev E, int t0, int 
     }
     block_top2(t1, t2, sh);
     if (threadIdx.x == 0) {
-      R.partials[T.ltile * kPartials + 8] = t1;
-      R.partials[T.ltile * kPartials + 9] = t2;
+      R.partials[kPartD1 * R.ntiles + T.ltile] = t1;
+      R.partials[kPartD2 * R.ntiles + T.ltile] = t2;
     }
   }
 }
 
-// integrate2 (src/nd.f90:1091-1117): kick with the new force, kinetic energy partial.
+// integrate2 (src/nd.f90:1091-1117): kick with the new force, kinetic energy partials (one
+// per wave of the tile; the slots of a TPA > 1 layout beyond the first two are cleared).
 __global__ __launch_bounds__(kTile) void k_kick_mixed(EngineDev E, int t0, int nt, int xflags) {
-  __shared__ double sh[2];
   int tb, te, ts;
   tile_range(t0, nt, tb, te, ts);
   for (int t = tb; t < te; t += ts) {
@@ -250,8 +257,11 @@ __global__ __launch_bounds__(kTile) void k_kick_mixed(EngineDev E, int t0, int n
       }
     }
     if (f & F_CK) {
-      block_sum<1>(kt, sh);
-      if (threadIdx.x == 0) R.partials[T.ltile * kPartials + 0] = kt[0];
+      const double w = wave_sum(kt[0]);
+      const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+      if (lane == 0) R.partials[(kPartKeW + wid) * R.ntiles + T.ltile] = w;
+      if (lane >= 2 && lane < 2 * R.tpa && wid == 0)
+        R.partials[(kPartKeW + lane) * R.ntiles + T.ltile] = 0.0;
     }
   }
 }
@@ -381,9 +391,12 @@ __device__ inline void force_atom(const RepParams &R, int i, int sub, const doub
   }
 }
 
+// kick != 0: integrate2 follows in the same pass (v += dt/2 (F_new + F_old)/m, kinetic
+// energy partials), with the arithmetic and the summation order of k_kick_mixed.
 template <int TPA, bool FULL>
 __global__ __launch_bounds__(ForceShape<TPA>::kThreads) void k_force_mixed(EngineDev E, int t0,
-                                                                           int nt, int xflags) {
+                                                                           int nt, int xflags,
+                                                                           int kick) {
   using SH = ForceShape<TPA>;
   constexpr int NSUB = SH::kNSub, SUB = SH::kSub, NW = SH::kThreads / 64;
   extern __shared__ float sh_tab[];
@@ -411,8 +424,9 @@ __global__ __launch_bounds__(ForceShape<TPA>::kThreads) void k_force_mixed(Engin
     Tile T = have ? E.tiles[t] : Tile{0, 0, 0, 0};
     const RepParams &R = E.reps[T.rep];
     const RepState &S = E.st[T.rep];
+    int f = 0;
     if (have) {
-      const int f = active_flags(E, T.rep, xflags);
+      f = active_flags(E, T.rep, xflags);
       const bool full = (f & (F_CU | F_CV)) != 0;
       have = (f & F_ACTIVE) && R.tpa == TPA && full == FULL;
     }
@@ -480,9 +494,33 @@ __global__ __launch_bounds__(ForceShape<TPA>::kThreads) void k_force_mixed(Engin
           for (int q = 0; q < 7; ++q) acc[q] += shfl_xor_d(acc[q], m);
       }
     }
+    const float4 Fn = make_float4((float)fx, (float)fy, (float)fz, FULL ? (float)acc[0] : 0.f);
     if (have && i < R.np && sub == 0) {
       float4 *Fout = S.parity ? R.f32a : R.f32b;
-      Fout[i] = make_float4((float)fx, (float)fy, (float)fz, FULL ? (float)acc[0] : 0.f);
+      Fout[i] = Fn;
+    }
+    if (kick) {
+      double ke = 0.0;
+      if (have && i < R.np && sub == 0) {
+        const float4 Fo = (S.parity ? R.f32b : R.f32a)[i];
+        const int ty = R.typ[i];
+        const double m = R.mass[ty - 1];
+        const double rm = 1.0 / m;
+        const double hdt = 0.5 * R.dt;
+        double vx = R.vx[i] + hdt * ((double)Fn.x * rm + (double)Fo.x * rm);
+        double vy = R.vy[i] + hdt * ((double)Fn.y * rm + (double)Fo.y * rm);
+        double vz = R.vz[i] + hdt * ((double)Fn.z * rm + (double)Fo.z * rm);
+        R.vx[i] = vx; R.vy[i] = vy; R.vz[i] = vz;
+        double k = vx * vx;
+        k = k + vy * vy;
+        k = k + vz * vz;
+        ke = k * m;
+      }
+      // a wave lies inside one tile, so the condition is the same in all its lanes
+      if (have && (f & F_CK)) {
+        const double w = wave_sum(ke);
+        if ((lt & 63) == 0) R.partials[(kPartKeW + (lt >> 6)) * R.ntiles + T.ltile] = w;
+      }
     }
     if (FULL) {
       // fixed-order sum over the tile's waves
@@ -503,7 +541,7 @@ __global__ __launch_bounds__(ForceShape<TPA>::kThreads) void k_force_mixed(Engin
         for (int q = 0; q < 7; ++q) {
           double v = 0.0;
           for (int w = 0; w < SUB / 64; ++w) v += sh_red[w0 + w][q];
-          R.partials[T.ltile * kPartials + 1 + q] = v;
+          R.partials[(kPartPe + q) * R.ntiles + T.ltile] = v;
         }
       }
     }
@@ -550,8 +588,8 @@ __global__ __launch_bounds__(kTile) void k_drift_fp64(EngineDev E, int t0, int n
     }
     block_top2(t1, t2, sh);
     if (threadIdx.x == 0) {
-      R.partials[T.ltile * kPartials + 8] = t1;
-      R.partials[T.ltile * kPartials + 9] = t2;
+      R.partials[kPartD1 * R.ntiles + T.ltile] = t1;
+      R.partials[kPartD2 * R.ntiles + T.ltile] = t2;
     }
   }
 }
@@ -642,8 +680,8 @@ __global__ __launch_bounds__(kTile) void k_force_fp64(EngineDev E, int t0, int n
         double pr[2] = {acc[q], q + 1 < 7 ? acc[q + 1] : 0.0};
         block_sum<2>(pr, sh_red);
         if (threadIdx.x == 0) {
-          R.partials[T.ltile * kPartials + 1 + q] = pr[0];
-          if (q + 1 < 7) R.partials[T.ltile * kPartials + 2 + q] = pr[1];
+          R.partials[(kPartPe + q) * R.ntiles + T.ltile] = pr[0];
+          if (q + 1 < 7) R.partials[(kPartPe + q + 1) * R.ntiles + T.ltile] = pr[1];
         }
       }
     }
@@ -681,7 +719,7 @@ __global__ __launch_bounds__(kTile) void k_kick_fp64(EngineDev E, int t0, int nt
     }
     if (f & F_CK) {
       block_sum<1>(kt, sh);
-      if (threadIdx.x == 0) R.partials[T.ltile * kPartials + 0] = kt[0];
+      if (threadIdx.x == 0) R.partials[kPartKe * R.ntiles + T.ltile] = kt[0];
     }
   }
 }
@@ -789,9 +827,11 @@ __device__ void step_epilogue(const RepParams &R, RepState &S, int f, double t1,
 
 // One workgroup per replica: fixed-order sums of the tile partials, then (by mode) the
 // force / kick results, the drop-in rebuild check, or the full step epilogue.
-__global__ __launch_bounds__(256) void k_reduce(EngineDev E, int r0, int nr, int mode,
-                                                int xflags, int check_trigger) {
-  __shared__ double sh[8 * 8];
+// One workgroup of 1024 threads per replica: a replica of 2M atoms has 16k tiles, whose
+// partials are read in 16 coalesced rounds.
+__global__ __launch_bounds__(1024) void k_reduce(EngineDev E, int r0, int nr, int mode,
+                                                 int xflags, int check_trigger) {
+  __shared__ double sh[16 * 8];
   for (int r = r0 + blockIdx.x; r < r0 + nr; r += gridDim.x) {
     const RepParams &R = E.reps[r];
     RepState &S = E.st[r];
@@ -799,13 +839,29 @@ __global__ __launch_bounds__(256) void k_reduce(EngineDev E, int r0, int nr, int
     if (!(f & F_ACTIVE)) continue;
     double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     double t1 = -1.0, t2 = -1.0;
-    for (int lt = threadIdx.x; lt < R.ntiles; lt += blockDim.x) {
-      const double *p = R.partials + lt * kPartials;
-      if (f & F_CK) v[0] += p[0];
-      if (f & (F_CU | F_CV))
-        for (int q = 1; q < 8; ++q) v[q] += p[q];
-      top2_merge(t1, t2, p[8], p[9]);
+    const int ntl = R.ntiles;
+    const auto p = gmem((const double *)R.partials);
+    const int nkw = 2 * R.tpa;   // kinetic-energy slots of a mixed-path tile
+    // one quantity at a time, several tiles' loads in flight per thread
+    if (f & F_CK) {
+      if (R.path == NDG_PATH_MIXED) {
+#pragma unroll 4
+        for (int lt = threadIdx.x; lt < ntl; lt += blockDim.x)
+          for (int w = 0; w < nkw; ++w) v[0] += p[(kPartKeW + w) * ntl + lt];
+      } else {
+#pragma unroll 4
+        for (int lt = threadIdx.x; lt < ntl; lt += blockDim.x) v[0] += p[kPartKe * ntl + lt];
+      }
     }
+    if (f & (F_CU | F_CV)) {
+#pragma unroll 4
+      for (int lt = threadIdx.x; lt < ntl; lt += blockDim.x)
+#pragma unroll
+        for (int q = 1; q < 8; ++q) v[q] += p[q * ntl + lt];
+    }
+#pragma unroll 4
+    for (int lt = threadIdx.x; lt < ntl; lt += blockDim.x)
+      top2_merge(t1, t2, p[kPartD1 * ntl + lt], p[kPartD2 * ntl + lt]);
     block_sum<8>(v, sh);
     block_top2(t1, t2, sh);
     if (threadIdx.x == 0) {
@@ -1141,9 +1197,10 @@ __global__ __launch_bounds__(kTile) void k_rb_place(EngineDev E) {
 // (src/nd.f90:1358-1377,1404-1420) over the 27 surrounding cells (or all atoms in brute
 // mode).  Work unit: one wave per chunk of up to kChunk home atoms of one cell, staged in
 // LDS.  The wave walks the cell's candidate stream (stencil cell (dz,dy,dx)
-// lexicographic, then slot) 64 candidates at a time, lane = candidate, and for each home
-// atom in turn tests the batch and appends the accepted candidates in lane order (ballot
-// prefix), so every list is in canonical order.
+// lexicographic, then slot) 64 candidates at a time, lane = candidate.  On the anchored
+// path a batch is handled in two phases: every home atom's test leaves one ballot mask in
+// LDS, then lane = home atom appends the set bits in lane order, so every list is in
+// canonical order.  The exact path tests and appends one home atom at a time.
 //
 // Test.  The reference computes d2 from scaled positions with anint images.  When the
 // grid guarantees that the stencil representation of a pair is its minimum image for
@@ -1591,10 +1648,10 @@ __global__ __launch_bounds__(64 * kListWaves) __attribute__((amdgpu_waves_per_eu
 void k_rb_list(EngineDev E) {
   __shared__ ListWave s_wave[kListWaves];
   __shared__ double s_cmax, s_cmin;
-  for (int k = blockIdx.x, t; (t = rb_tile(E, k)) >= 0; k += gridDim.x) {
-    __syncthreads();   // the previous tile's waves are done with the shared cutoffs
-    list_tile(E, t, s_wave[threadIdx.x >> 6], s_cmax, s_cmin);
-  }
+  // one workgroup per entry of the work list (the grid covers every tile): the hardware
+  // hands tiles to CUs as they free up, and a wave without a chunk ends at once
+  const int t = rb_tile(E, blockIdx.x);
+  if (t >= 0) list_tile(E, t, s_wave[threadIdx.x >> 6], s_cmax, s_cmin);
 }
 
 // ------------------------------------------------------------------ bandwidth probe
@@ -1638,20 +1695,20 @@ void launch_kick(const EngineDev &E, int path, int t0, int nt, int xflags, hipSt
 }
 
 template <int TPA>
-static void launch_force_tpa(const EngineDev &E, bool full, int t0, int nt, int xflags,
+static void launch_force_tpa(const EngineDev &E, bool full, int t0, int nt, int xflags, int kick,
                              hipStream_t s) {
   using SH = ForceShape<TPA>;
   const int nblk = (nt + SH::kNSub - 1) / SH::kNSub;
   const int g = grid_for(nblk, 2);
   const size_t lds = sizeof(float) * (size_t)E.tabset_n;
   if (full)
-    hipLaunchKernelGGL((k_force_mixed<TPA, true>), dim3(g), dim3(SH::kThreads), lds, s, E, t0, nt, xflags);
+    hipLaunchKernelGGL((k_force_mixed<TPA, true>), dim3(g), dim3(SH::kThreads), lds, s, E, t0, nt, xflags, kick);
   else
-    hipLaunchKernelGGL((k_force_mixed<TPA, false>), dim3(g), dim3(SH::kThreads), lds, s, E, t0, nt, xflags);
+    hipLaunchKernelGGL((k_force_mixed<TPA, false>), dim3(g), dim3(SH::kThreads), lds, s, E, t0, nt, xflags, kick);
 }
 
 void launch_force(const EngineDev &E, int path, int tpa, bool full, int t0, int nt, int xflags,
-                  hipStream_t s) {
+                  hipStream_t s, int kick) {
   if (nt <= 0) return;
   if (path == NDG_PATH_FP64) {
     if (full)
@@ -1661,17 +1718,17 @@ void launch_force(const EngineDev &E, int path, int tpa, bool full, int t0, int 
     return;
   }
   switch (tpa) {
-    case 1: launch_force_tpa<1>(E, full, t0, nt, xflags, s); break;
-    case 2: launch_force_tpa<2>(E, full, t0, nt, xflags, s); break;
-    case 4: launch_force_tpa<4>(E, full, t0, nt, xflags, s); break;
-    default: launch_force_tpa<8>(E, full, t0, nt, xflags, s); break;
+    case 1: launch_force_tpa<1>(E, full, t0, nt, xflags, kick, s); break;
+    case 2: launch_force_tpa<2>(E, full, t0, nt, xflags, kick, s); break;
+    case 4: launch_force_tpa<4>(E, full, t0, nt, xflags, kick, s); break;
+    default: launch_force_tpa<8>(E, full, t0, nt, xflags, kick, s); break;
   }
 }
 
 void launch_reduce(const EngineDev &E, int r0, int nr, int mode, int xflags, int trigger,
                    hipStream_t s) {
   if (nr <= 0) return;
-  hipLaunchKernelGGL(k_reduce, dim3(nr < 1024 ? nr : 1024), dim3(256), 0, s, E, r0, nr, mode,
+  hipLaunchKernelGGL(k_reduce, dim3(nr < 1024 ? nr : 1024), dim3(1024), 0, s, E, r0, nr, mode,
                      xflags, trigger);
 }
 
@@ -1691,7 +1748,7 @@ void launch_rebuild(const EngineDev &E, int nt, int r0, int nr, hipStream_t s) {
   hipLaunchKernelGGL(k_rb_scatter, dim3(g), dim3(kTile), 0, s, E);
   hipLaunchKernelGGL(k_rb_rank, dim3(g), dim3(kTile), 0, s, E);
   hipLaunchKernelGGL(k_rb_place, dim3(g), dim3(kTile), 0, s, E);
-  hipLaunchKernelGGL(k_rb_list, dim3(grid_for(nt, 16)), dim3(64 * kListWaves), 0, s, E);
+  hipLaunchKernelGGL(k_rb_list, dim3(nt), dim3(64 * kListWaves), 0, s, E);
 }
 
 }  // namespace ndg

@@ -21,11 +21,20 @@ import statistics
 from collections import defaultdict
 
 
+def kernel_name(raw):
+    """'void ndg::k_force_mixed<1, false>(ndg::EngineDev, int, ...)' -> 'k_force_mixed<1, false>'."""
+    n = raw.strip()
+    if n.startswith("void "):
+        n = n[5:]
+    n = n.split("(")[0].replace("ndg::", "")
+    return n
+
+
 def load_trace(path):
     rows = []
     with open(path) as f:
         for r in csv.DictReader(f):
-            rows.append(dict(name=r["Kernel_Name"], t0=int(r["Start_Timestamp"]),
+            rows.append(dict(name=kernel_name(r["Kernel_Name"]), t0=int(r["Start_Timestamp"]),
                              t1=int(r["End_Timestamp"]), grid=int(r["Grid_Size_X"]),
                              wg=int(r["Workgroup_Size_X"])))
     rows.sort(key=lambda r: r["t0"])
@@ -46,10 +55,10 @@ def window(rows, anchor="k_drift_mixed", gap_ms=50.0):
             start = k
     big = big[start:]
     t_lo, t_hi = big[0]["t0"], big[-1]["t1"]
-    last_kick = [r for r in rows if r["name"] == "k_kick_mixed" and r["grid"] == gmax
-                 and r["t0"] >= t_hi]
-    if last_kick:
-        t_hi = last_kick[0]["t1"]
+    # the last step ends with its reduction
+    last_reduce = [r for r in rows if r["name"] == "k_reduce" and r["t0"] >= t_hi]
+    if last_reduce:
+        t_hi = last_reduce[0]["t1"]
     return [r for r in rows if r["t0"] >= t_lo and r["t1"] <= t_hi], len(big), gmax
 
 
@@ -76,7 +85,7 @@ def load_pmc(path, counter):
     with open(path) as f:
         for r in csv.DictReader(f):
             if r.get("Counter_Name") == counter:
-                key = (r["Kernel_Name"], r["Dispatch_Id"])
+                key = (kernel_name(r["Kernel_Name"]), r["Dispatch_Id"])
                 per[key] += float(r["Counter_Value"])
                 meta[key] = int(r["Grid_Size"])
     vals = defaultdict(list)
