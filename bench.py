@@ -32,7 +32,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_force_mixed launch, per workload, from rocprofv3 FETCH_SIZE + WRITE_SIZE
 # scaled by the same counters on k_copy of known size (1 GiB read + 1 GiB written):
 # FETCH_SIZE x 2.000, WRITE_SIZE x 1.000.  Source: profiles/k_force_mixed_c3_traffic.txt.
-TRAFFIC_BYTES_PER_LAUNCH = {}
+TRAFFIC_BYTES_PER_LAUNCH = {"c3": 1.0602e9}
 METRIC = "atom-steps/s vs plain FP64 GPU path at <=1e-5 rel force err; p50/p99 job latency"
 
 
@@ -181,31 +181,41 @@ def cpu_baseline(systems, cfg, budget_s):
                 if min(int(L / (float(s.params[:, 0].max()) + cfg["skin"])) for L in s.box) >= 3]
     order = np.argsort([s.np for s in runnable])
     pick = [runnable[i] for i in order[:: max(1, len(order) // 12)]][:12]
-    steps = 20
-    tot_as, tot_t = 0.0, 0.0
-    t_start = time.perf_counter()
-    for s in pick:
-        _, out = refrun.run_harness(refrun.MODE_RUN, s, nsteps=steps, outsteps=steps,
-                                    dt=cfg["dt"], skin=cfg["skin"], nthreads=threads,
-                                    iso_ke=cfg.get("iso_ke"), iso_p=cfg.get("iso_p"))
-        tot_t += refrun.step_loop_seconds(out)
-        tot_as += s.np * steps
-        if time.perf_counter() - t_start > budget_s:
-            break
-    # scale the step count to fill about budget_s of step loop
-    if tot_t > 0 and tot_t < 0.5 * budget_s:
-        steps2 = int(min(2000, steps * max(1.0, 0.8 * budget_s / tot_t)))
-        tot_as, tot_t = 0.0, 0.0
+    def sample(steps, deadline):
+        """Step-loop seconds and atom-steps over the sample; a system whose reference run
+        stops (e.g. its unchanged list lets an atom leave the box) is left out."""
+        t, n, used = 0.0, 0.0, 0
+        t0 = time.perf_counter()
         for s in pick:
-            _, out = refrun.run_harness(refrun.MODE_RUN, s, nsteps=steps2, outsteps=steps2,
-                                        dt=cfg["dt"], skin=cfg["skin"], nthreads=threads,
-                                        iso_ke=cfg.get("iso_ke"), iso_p=cfg.get("iso_p"))
-            tot_t += refrun.step_loop_seconds(out)
-            tot_as += s.np * steps2
-        steps = steps2
+            try:
+                _, out = refrun.run_harness(refrun.MODE_RUN, s, nsteps=steps, outsteps=steps,
+                                            dt=cfg["dt"], skin=cfg["skin"], nthreads=threads,
+                                            iso_ke=cfg.get("iso_ke"), iso_p=cfg.get("iso_p"))
+            except RuntimeError as err:
+                log(f"bench: cpu baseline: system of {s.np} atoms left out "
+                    f"({str(err).strip().splitlines()[-1][:80]})")
+                continue
+            t += refrun.step_loop_seconds(out)
+            n += s.np * steps
+            used += 1
+            if deadline and time.perf_counter() - t0 > deadline:
+                break
+        return t, n, used
+
+    steps = 20
+    tot_t, tot_as, used = sample(steps, budget_s)
+    # scale the step count to fill about budget_s of step loop (at most 400 steps: the
+    # reference keeps its first list, so long runs of hot replicas can stop)
+    if 0 < tot_t < 0.5 * budget_s:
+        steps2 = int(min(400, steps * max(1.0, 0.8 * budget_s / tot_t)))
+        t2, n2, used2 = sample(steps2, 0)
+        if t2 > 0:
+            tot_t, tot_as, used, steps = t2, n2, used2, steps2
+    if tot_t <= 0:
+        return None
     return dict(value=tot_as / tot_t, unit="atom-steps/s", cores=threads, kind="reference",
-                sample=f"{len(pick)} systems of the workload with MCELL>=3 "
-                       f"({sum(s.np for s in pick)} atoms), "
+                sample=f"{used} systems of the workload with MCELL>=3 "
+                       f"({int(round(tot_as / steps))} atoms), "
                        f"{steps} steps each, reference step-loop timer, OMP_NUM_THREADS={threads}")
 
 
