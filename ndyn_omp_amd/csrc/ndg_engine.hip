@@ -179,6 +179,7 @@ struct ndg_engine {
   int *d_rb = nullptr;        // rebuild work list (count + tile indices), cap_tiles + 1 ints
   int *d_tile0 = nullptr;     // first tile of each slot
   int cap_slots = 0, cap_tiles = 0, cap_tile0 = 0;
+  int max_rep_tiles = 0;
   std::vector<Tile> h_tiles;
   std::vector<std::pair<int, int>> tpa_ranges;  // (tpa, t0) in tile order, nt follows
   std::vector<int> tpa_counts;
@@ -477,8 +478,12 @@ static int rebuild_tiles(ndg_engine *e) {
     HIPCHK(hipMalloc(&e->d_tile0, sizeof(int) * e->cap_tile0));
   }
   std::vector<int> tile0(ns, -1);
+  e->max_rep_tiles = 0;
   for (int r = 0; r < ns; ++r)
-    if (e->slots[r].used) tile0[r] = e->slots[r].tile_base;
+    if (e->slots[r].used) {
+      tile0[r] = e->slots[r].tile_base;
+      e->max_rep_tiles = std::max(e->max_rep_tiles, e->slots[r].ntiles);
+    }
   if (ns > 0)
     HIPCHK(hipMemcpyAsync(e->d_tile0, tile0.data(), sizeof(int) * ns, hipMemcpyHostToDevice, e->stream));
   if (nt > 0)
@@ -497,6 +502,7 @@ static EngineDev dev_view(ndg_engine *e) {
   E.tiles = e->d_tiles;
   E.tile0 = e->d_tile0;
   E.rb = e->d_rb;
+  E.max_rep_tiles = e->max_rep_tiles;
   E.ntiles = (int)e->h_tiles.size();
   E.nreps = (int)e->slots.size();
   E.tabset = e->d_tabset;

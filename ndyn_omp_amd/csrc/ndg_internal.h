@@ -113,6 +113,7 @@ struct EngineDev {
   int nreps;
   const int *tile0;     // [nreps] first tile of each slot, -1 for an unused slot
   int *rb;              // rebuild work list: rb[0] = count, rb[1 .. count] = tile indices
+  int max_rep_tiles;    // tiles of the largest admitted replica
   const float *tabset;  // distinct FP32 F tables of all replicas, staged in LDS by the force kernel
   int tabset_n;         // floats in tabset (0: tables are read from global memory)
 };

@@ -834,8 +834,10 @@ __device__ void step_epilogue(const RepParams &R, RepState &S, int f, double t1,
 
 // One workgroup per replica: fixed-order sums of the tile partials, then (by mode) the
 // force / kick results, the drop-in rebuild check, or the full step epilogue.
-// One workgroup of 1024 threads per replica: a replica of 2M atoms has 16k tiles, whose
-// partials are read in 16 coalesced rounds.
+// One workgroup per replica.  A replica's partials are summed by the first T threads,
+// T = 1024 for a replica of more than 1024 tiles (2M atoms: 16 coalesced rounds) and 256
+// otherwise; T depends only on the replica, and the other threads of the workgroup add
+// exact zeros, so the sums do not depend on the launch's block size.
 __global__ __launch_bounds__(1024) void k_reduce(EngineDev E, int r0, int nr, int mode,
                                                  int xflags, int check_trigger) {
   __shared__ double sh[16 * 8];
@@ -849,25 +851,27 @@ __global__ __launch_bounds__(1024) void k_reduce(EngineDev E, int r0, int nr, in
     const int ntl = R.ntiles;
     const auto p = gmem((const double *)R.partials);
     const int nkw = 2 * R.tpa;   // kinetic-energy slots of a mixed-path tile
+    const int nthr = ntl > 1024 ? 1024 : 256;
+    const int t0 = (int)threadIdx.x < nthr ? (int)threadIdx.x : ntl;   // idle threads: no tiles
     // one quantity at a time, several tiles' loads in flight per thread
     if (f & F_CK) {
       if (R.path == NDG_PATH_MIXED) {
 #pragma unroll 4
-        for (int lt = threadIdx.x; lt < ntl; lt += blockDim.x)
+        for (int lt = t0; lt < ntl; lt += nthr)
           for (int w = 0; w < nkw; ++w) v[0] += p[(kPartKeW + w) * ntl + lt];
       } else {
 #pragma unroll 4
-        for (int lt = threadIdx.x; lt < ntl; lt += blockDim.x) v[0] += p[kPartKe * ntl + lt];
+        for (int lt = t0; lt < ntl; lt += nthr) v[0] += p[kPartKe * ntl + lt];
       }
     }
     if (f & (F_CU | F_CV)) {
 #pragma unroll 4
-      for (int lt = threadIdx.x; lt < ntl; lt += blockDim.x)
+      for (int lt = t0; lt < ntl; lt += nthr)
 #pragma unroll
         for (int q = 1; q < 8; ++q) v[q] += p[q * ntl + lt];
     }
 #pragma unroll 4
-    for (int lt = threadIdx.x; lt < ntl; lt += blockDim.x)
+    for (int lt = t0; lt < ntl; lt += nthr)
       top2_merge(t1, t2, p[kPartD1 * ntl + lt], p[kPartD2 * ntl + lt]);
     block_sum<8>(v, sh);
     block_top2(t1, t2, sh);
@@ -1738,7 +1742,8 @@ void launch_force(const EngineDev &E, int path, int tpa, bool full, int t0, int 
 void launch_reduce(const EngineDev &E, int r0, int nr, int mode, int xflags, int trigger,
                    hipStream_t s) {
   if (nr <= 0) return;
-  hipLaunchKernelGGL(k_reduce, dim3(nr < 1024 ? nr : 1024), dim3(1024), 0, s, E, r0, nr, mode,
+  const int threads = E.max_rep_tiles > 1024 ? 1024 : 256;
+  hipLaunchKernelGGL(k_reduce, dim3(nr < 1024 ? nr : 1024), dim3(threads), 0, s, E, r0, nr, mode,
                      xflags, trigger);
 }
 
