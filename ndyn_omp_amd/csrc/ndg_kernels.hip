@@ -364,8 +364,8 @@ __device__ inline void force_atom(const RepParams &R, int i, int sub, const doub
       const float w = rk - kf;
       float fv;
       if (LDS_TAB) {
-        const float *Tl = stab + ltp;
-        const float F0 = Tl[k - 1], F1 = Tl[k];
+        const float *Tm = stab + (ltp - 1);   // Tm[k] = T_{k-1}: address k * 4 + base
+        const float F0 = Tm[k], F1 = Tm[k + 1];
         fv = fmaf(w, F1 - F0, F0);
       } else {
         const int q = 2 * (pid * tab + k - 1);
@@ -1104,6 +1104,19 @@ __device__ inline int rb_tile(const EngineDev &E, int k) {
 // histogram, per-replica scan, scatter, per-cell order (one wave per cell), placement.
 // (A whole replica sorted inside one workgroup, with its bookkeeping in LDS, was slower
 // on the replica mix at every size: one workgroup serialises what the tiles share.)
+// Runs of equal cells within a wave (atoms stay nearly sorted by cell between rebuilds):
+// c = -1 marks a lane without an atom.  Returns the lane's run leader and the run length
+// (valid in the leader), so that one atomic serves the run.
+__device__ inline int cell_run(int c, int &len) {
+  const int lane = threadIdx.x & 63;
+  const int cp = __shfl_up(c, 1);
+  const unsigned long long lead = __ballot(lane == 0 || c != cp);
+  const unsigned long long after = lane == 63 ? 0ull : lead & (~0ull << (lane + 1));
+  const int next = after ? __builtin_ctzll(after) : 64;
+  len = next - lane;
+  return 63 - __builtin_clzll(lead & (lane == 63 ? ~0ull : ((2ull << lane) - 1)));
+}
+
 __global__ __launch_bounds__(kTile) void k_rb_bin(EngineDev E) {
   for (int k = blockIdx.x, t; (t = rb_tile(E, k)) >= 0; k += gridDim.x) {
     const Tile T = E.tiles[t];
@@ -1111,10 +1124,11 @@ __global__ __launch_bounds__(kTile) void k_rb_bin(EngineDev E) {
     const RepState &S = E.st[T.rep];
     if (!S.rebuild || S.err) continue;
     const int i = T.atom0 + threadIdx.x;
-    if (i >= R.np) continue;
-    const int c = cell_of(R, S, i);
-    R.key[i] = c;
-    atomicAdd(&R.cell_count[c], 1);
+    const int c = i < R.np ? cell_of(R, S, i) : -1;
+    if (c >= 0) R.key[i] = c;
+    int len;
+    const int ld = cell_run(c, len);
+    if (c >= 0 && ld == (int)(threadIdx.x & 63)) atomicAdd(&R.cell_count[c], len);
   }
 }
 
@@ -1161,9 +1175,14 @@ __global__ __launch_bounds__(kTile) void k_rb_scatter(EngineDev E) {
     const RepState &S = E.st[T.rep];
     if (!S.rebuild || S.err) continue;
     const int i = T.atom0 + threadIdx.x;
-    if (i >= R.np) continue;
-    const int c = R.key[i];
-    R.slot_src[R.cell_start[c] + atomicAdd(&R.cell_fill[c], 1)] = i;
+    const int c = i < R.np ? R.key[i] : -1;
+    int len;
+    const int ld = cell_run(c, len);
+    const int lane = threadIdx.x & 63;
+    int base = 0;
+    if (c >= 0 && ld == lane) base = atomicAdd(&R.cell_fill[c], len);
+    base = __shfl(base, ld);
+    if (c >= 0) R.slot_src[R.cell_start[c] + base + (lane - ld)] = i;
   }
 }
 
