@@ -1186,9 +1186,11 @@ __global__ __launch_bounds__(kTile) void k_rb_scatter(EngineDev E) {
   }
 }
 
-// Each tile sorts the cells whose first slot lies in it (one wave per cell).
-__global__ __launch_bounds__(kTile) void k_rb_rank(EngineDev E) {
-  __shared__ int sh_sort[kTile / 64][2 * kCellLds];
+// Each tile sorts the cells whose first slot lies in it (one wave per cell, kRankWaves
+// waves per tile, so that a tile's few cells are sorted side by side).
+constexpr int kRankWaves = 8;
+__global__ __launch_bounds__(64 * kRankWaves) void k_rb_rank(EngineDev E) {
+  __shared__ int sh_sort[kRankWaves][2 * kCellLds];
   for (int k = blockIdx.x, t; (t = rb_tile(E, k)) >= 0; k += gridDim.x) {
     const Tile T = E.tiles[t];
     const RepParams &R = E.reps[T.rep];
@@ -1196,7 +1198,7 @@ __global__ __launch_bounds__(kTile) void k_rb_rank(EngineDev E) {
     if (!S.rebuild || S.err) continue;
     if (S.brute) {
       const int i = T.atom0 + threadIdx.x;
-      if (i < R.np) permute_atom(R, i, R.oid[i], S.parity);
+      if (threadIdx.x < kTile && i < R.np) permute_atom(R, i, R.oid[i], S.parity);
       continue;
     }
     const int ncell = S.mcell[0] * S.mcell[1] * S.mcell[2];
@@ -1780,7 +1782,7 @@ void launch_rebuild(const EngineDev &E, int nt, int r0, int nr, hipStream_t s) {
   hipLaunchKernelGGL(k_rb_bin, dim3(g), dim3(kTile), 0, s, E);
   hipLaunchKernelGGL(k_rb_scan, dim3(gr), dim3(1024), 0, s, E, r0, nr);
   hipLaunchKernelGGL(k_rb_scatter, dim3(g), dim3(kTile), 0, s, E);
-  hipLaunchKernelGGL(k_rb_rank, dim3(g), dim3(kTile), 0, s, E);
+  hipLaunchKernelGGL(k_rb_rank, dim3(g), dim3(64 * kRankWaves), 0, s, E);
   hipLaunchKernelGGL(k_rb_place, dim3(g), dim3(kTile), 0, s, E);
   hipLaunchKernelGGL(k_rb_list, dim3(nt), dim3(64 * kListWaves), 0, s, E);
 }
