@@ -133,6 +133,7 @@ struct TableBuf {       // device copy of one replica's pair tables (shared when
   double *F64 = nullptr, *P64 = nullptr;
   float2 *F2 = nullptr, *P2 = nullptr;
   std::vector<std::vector<float>> F32;  // per pair, host copy for the LDS table set
+  std::vector<std::vector<float>> P32;  // per pair, FP32 potential table (same layout)
   std::vector<uint64_t> phash;          // per pair content hash
 };
 
@@ -201,7 +202,8 @@ struct ndg_engine {
   // distinct FP32 force tables of all replicas (staged in LDS by the force kernel)
   float *d_tabset = nullptr;
   size_t tabset_cap = 0;
-  int tabset_n = 0;
+  int tabset_n = 0;      // floats of the F half
+  int tabset_p = 0;      // 1: the energy/virial variant stages the P half too
 };
 
 struct ndg_system {
@@ -394,9 +396,12 @@ static int ensure_capacity(ndg_engine *e) {
 
 // Distinct FP32 force tables of all replicas, concatenated; each replica gets the offset
 // of each of its pairs (or -1 when the set exceeds the LDS budget).
+// The distinct FP32 force tables of all replicas, then their potential tables in the same
+// order: [F ... | P ...].  The force kernel stages the F half in LDS; its energy/virial
+// variant stages both halves when they fit (tabset_p).
 static int rebuild_tabset(ndg_engine *e) {
   std::map<uint64_t, int> off;
-  std::vector<float> all;
+  std::vector<float> all, allp;
   for (auto &s : e->slots) {
     if (!s.used || !s.table_key) continue;
     auto it = e->tables.find(s.table_key);
@@ -406,10 +411,13 @@ static int rebuild_tabset(ndg_engine *e) {
       if (!off.count(h)) {
         off[h] = (int)all.size();
         all.insert(all.end(), it->second.F32[p].begin(), it->second.F32[p].end());
+        allp.insert(allp.end(), it->second.P32[p].begin(), it->second.P32[p].end());
       }
     }
   }
   const bool fits = !all.empty() && all.size() * sizeof(float) <= (size_t)kLdsTableBytes;
+  const bool fits_p = fits && 2 * all.size() * sizeof(float) <= (size_t)kLdsTableBytesFull;
+  all.insert(all.end(), allp.begin(), allp.end());
   for (auto &s : e->slots) {
     for (int p = 0; p < 16; ++p) s.P.ltab[p] = -1;
     if (!s.used || !s.table_key || !fits) continue;
@@ -427,7 +435,8 @@ static int rebuild_tabset(ndg_engine *e) {
     }
     HIPCHK(hipMemcpyAsync(e->d_tabset, all.data(), sizeof(float) * all.size(), hipMemcpyHostToDevice, e->stream));
   }
-  e->tabset_n = fits ? (int)all.size() : 0;
+  e->tabset_n = fits ? (int)(all.size() / 2) : 0;
+  e->tabset_p = fits_p ? 1 : 0;
   for (size_t r = 0; r < e->slots.size(); ++r)
     if (e->slots[r].used && e->slots[r].block)
       HIPCHK(hipMemcpyAsync(e->d_reps + r, &e->slots[r].P, sizeof(RepParams), hipMemcpyHostToDevice, e->stream));
@@ -507,6 +516,7 @@ static EngineDev dev_view(ndg_engine *e) {
   E.nreps = (int)e->slots.size();
   E.tabset = e->d_tabset;
   E.tabset_n = e->tabset_n;
+  E.tabset_p = e->tabset_p;
   return E;
 }
 
@@ -563,11 +573,17 @@ static int attach_tables(ndg_engine *e, Slot &s, const HostState &h) {
     HIPCHK(hipMemcpyAsync(tb.P2, p2.data(), n * 8, hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     tb.F32.resize(h.npairs);
+    tb.P32.resize(h.npairs);
     tb.phash.resize(h.npairs);
     for (int p = 0; p < h.npairs; ++p) {
       tb.F32[p].resize(tab);
-      for (int k = 0; k < tab; ++k) tb.F32[p][k] = (float)h.F[(size_t)p * tab + k];
-      tb.phash[p] = fnv1a(tb.F32[p].data(), 4 * (size_t)tab);
+      tb.P32[p].resize(tab);
+      for (int k = 0; k < tab; ++k) {
+        tb.F32[p][k] = (float)h.F[(size_t)p * tab + k];
+        tb.P32[p][k] = (float)h.P[(size_t)p * tab + k];
+      }
+      tb.phash[p] = fnv1a(tb.F32[p].data(), 4 * (size_t)tab) ^
+                    (fnv1a(tb.P32[p].data(), 4 * (size_t)tab) * 0x100000001b3ull);
     }
     it = e->tables.emplace(key, tb).first;
   }

@@ -114,11 +114,15 @@ struct EngineDev {
   const int *tile0;     // [nreps] first tile of each slot, -1 for an unused slot
   int *rb;              // rebuild work list: rb[0] = count, rb[1 .. count] = tile indices
   int max_rep_tiles;    // tiles of the largest admitted replica
-  const float *tabset;  // distinct FP32 F tables of all replicas, staged in LDS by the force kernel
-  int tabset_n;         // floats in tabset (0: tables are read from global memory)
+  const float *tabset;  // distinct FP32 F tables of all replicas, then their P tables
+  int tabset_n;         // floats of the F half (0: tables are read from global memory)
+  int tabset_p;         // 1: the energy/virial force variant also stages the P half
 };
 
-constexpr int kLdsTableBytes = 64 * 1024;   // LDS budget of the staged table set
+// LDS budgets of the staged tables: with the force kernel's own LDS (at most ~15 KB) a
+// workgroup stays within 64 KB
+constexpr int kLdsTableBytes = 48 * 1024;       // F tables
+constexpr int kLdsTableBytesFull = 48 * 1024;   // F + P tables, energy/virial variant
 
 
 // Flags of the step a replica is about to compute (src/nd.f90:137-151).

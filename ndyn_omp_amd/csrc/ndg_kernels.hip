@@ -290,7 +290,7 @@ struct ForceShape {
 template <int TPA, bool FULL, bool LDS_TAB, bool ONE>
 __device__ inline void force_atom(const RepParams &R, int i, int sub, const double4 *shift,
                                   const double *rc2, const float *vdr2, const int *pm,
-                                  const int *lt, const float *stab, const unsigned *L,
+                                  const int *lt, const float *stab, int ptoff, const unsigned *L,
                                   double qunit, double &fx, double &fy, double &fz,
                                   double (&acc)[7]) {
   // the energy/virial variant holds seven more sums: an unroll of 2 keeps it at four
@@ -377,8 +377,16 @@ __device__ inline void force_atom(const RepParams &R, int i, int sub, const doub
       fy = fma(dy, df, fy);
       fz = fma(dz, df, fz);
       if (FULL) {
-        const int q = 2 * (pid * tab + k - 1);
-        const float P0 = TP[q], P1 = TP[q + 1];
+        float P0, P1;
+        if (LDS_TAB && ptoff >= 0) {   // P half of the staged table set (uniform choice)
+          const float *Pm = stab + (ptoff + ltp - 1);
+          P0 = Pm[k];
+          P1 = Pm[k + 1];
+        } else {
+          const int q = 2 * (pid * tab + k - 1);
+          P0 = TP[q];
+          P1 = TP[q + 1];
+        }
         const double pv = ok ? (double)fmaf(w, P1 - P0, P0) : 0.0;
         // energy and virial sums without their factor 1/2, applied once per atom (an
         // exact scaling)
@@ -411,7 +419,10 @@ __global__ __launch_bounds__(ForceShape<TPA>::kThreads) void k_force_mixed(Engin
   __shared__ int sh_lt[NSUB][16];
   __shared__ int sh_uselds[NSUB];
   __shared__ double sh_red[NW][8];
-  for (int q = threadIdx.x; q < E.tabset_n; q += blockDim.x) sh_tab[q] = E.tabset[q];
+  // the F half of the table set; the energy/virial variant also the P half when it fits
+  const int ptoff = FULL && E.tabset_p ? E.tabset_n : -1;
+  const int nstage = ptoff >= 0 ? 2 * E.tabset_n : E.tabset_n;
+  for (int q = threadIdx.x; q < nstage; q += blockDim.x) sh_tab[q] = E.tabset[q];
   const int sg = threadIdx.x / SUB, lt = threadIdx.x % SUB;
   const int a = lt / TPA, sub = lt % TPA;
   const int nblk = (nt + NSUB - 1) / NSUB;
@@ -476,16 +487,16 @@ __global__ __launch_bounds__(ForceShape<TPA>::kThreads) void k_force_mixed(Engin
       const bool lds = sh_uselds[sg] != 0, one = R.npairs == 1;
       if (lds && one)
         force_atom<TPA, FULL, true, true>(R, i, sub, sh_shift[sg], sh_rc2[sg], sh_vdr2[sg],
-                                          sh_pm[sg], sh_lt[sg], sh_tab, L, qunit, fx, fy, fz, acc);
+                                          sh_pm[sg], sh_lt[sg], sh_tab, ptoff, L, qunit, fx, fy, fz, acc);
       else if (lds)
         force_atom<TPA, FULL, true, false>(R, i, sub, sh_shift[sg], sh_rc2[sg], sh_vdr2[sg],
-                                           sh_pm[sg], sh_lt[sg], sh_tab, L, qunit, fx, fy, fz, acc);
+                                           sh_pm[sg], sh_lt[sg], sh_tab, ptoff, L, qunit, fx, fy, fz, acc);
       else if (one)
         force_atom<TPA, FULL, false, true>(R, i, sub, sh_shift[sg], sh_rc2[sg], sh_vdr2[sg],
-                                           sh_pm[sg], sh_lt[sg], sh_tab, L, qunit, fx, fy, fz, acc);
+                                           sh_pm[sg], sh_lt[sg], sh_tab, ptoff, L, qunit, fx, fy, fz, acc);
       else
         force_atom<TPA, FULL, false, false>(R, i, sub, sh_shift[sg], sh_rc2[sg], sh_vdr2[sg],
-                                            sh_pm[sg], sh_lt[sg], sh_tab, L, qunit, fx, fy, fz, acc);
+                                            sh_pm[sg], sh_lt[sg], sh_tab, ptoff, L, qunit, fx, fy, fz, acc);
       if (FULL)
 #pragma unroll
         for (int q = 0; q < 7; ++q) acc[q] *= 0.5;
@@ -1735,7 +1746,7 @@ static void launch_force_tpa(const EngineDev &E, bool full, int t0, int nt, int 
   using SH = ForceShape<TPA>;
   const int nblk = (nt + SH::kNSub - 1) / SH::kNSub;
   const int g = grid_for(nblk, 2);
-  const size_t lds = sizeof(float) * (size_t)E.tabset_n;
+  const size_t lds = sizeof(float) * (size_t)E.tabset_n * (full && E.tabset_p ? 2 : 1);
   if (full)
     hipLaunchKernelGGL((k_force_mixed<TPA, true>), dim3(g), dim3(SH::kThreads), lds, s, E, t0, nt, xflags, kick);
   else
