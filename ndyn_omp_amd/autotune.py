@@ -6,8 +6,8 @@ The settings are tuned on the batch they are for (or an even subsample of it whe
 larger than `max_atoms`), not on each class alone: in a full batch a step's rebuild time
 is set by the largest replica that rebuilds in that step, so a larger skin pays for
 itself only where it stops the rebuilds that set that time, which an isolated class
-cannot show.  The search is coordinate descent: start from TPA 1 and the first skin in
-every class; then, classes largest first, try each TPA and then each skin for that class
+cannot show.  The search is coordinate descent: start from TPA 1 and a skin of 0.5 A (or
+the first candidate) in every class; then, classes largest first, try each TPA and then each skin for that class
 with the others held, keeping a change only if it is more than `gain` faster.  Each trial
 advances the whole batch for `warmup + steps` steps (several rebuild intervals) and
 counts wall-clock atom-steps/s.
@@ -30,7 +30,8 @@ from . import workloads as W
 
 DEFAULT_CLASSES = (2048, 8192, 1 << 26)
 DEFAULT_TPAS = (1, 2, 4)
-DEFAULT_SKINS = (0.5, 0.75, 1.0, 1.25)
+DEFAULT_SKINS = (0.3, 0.4, 0.5, 0.75, 1.0, 1.25)
+START_SKIN = 0.5
 
 
 @dataclasses.dataclass
@@ -107,7 +108,8 @@ def tune(systems: Sequence[W.System], make_rp: Callable[[float, int], G.RunParam
     batch = subsample(systems, max_atoms)
     present = [any(lo < s.np <= hi for s in batch)
                for lo, hi in zip((0,) + tuple(classes[:-1]), classes)]
-    best = TuneTable([(int(c), 1, float(skins[0])) for c in classes])
+    start = START_SKIN if START_SKIN in skins else skins[0]
+    best = TuneTable([(int(c), 1, float(start)) for c in classes])
     trials = []
 
     def measure(t: TuneTable, what: str) -> float:
