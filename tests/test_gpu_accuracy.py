@@ -40,3 +40,19 @@ def test_nve_drift_energy_consistent_morse():
         assert abs(d[p]["drift_per_atom_ev"]) <= 1e-4
     m, f = d[G.PATH_MIXED]["drift_per_atom_ev"], d[G.PATH_FP64]["drift_per_atom_ev"]
     assert abs(m - f) <= 0.1 * abs(f) + 1e-7
+
+
+@pytest.mark.parametrize("ptsize", [5000, 8000, 14000])
+def test_energy_virial_with_staged_and_global_tables(ptsize):
+    """The energy/virial force variant stages the FP32 force and potential tables in LDS
+    when both fit in 48 KB (5001 points: 40 KB), reads the potential table from global
+    memory when only the force table fits (8001 points), and reads both from global
+    memory beyond that (14001 points).  Same bars in all three cases."""
+    s = W.cu_morse(9, temp_k=600.0, seed=65, ptsize=ptsize)
+    print(f"case ptsize={ptsize} np={s.np}", flush=True)
+    r = A.pair_errors(A.thermal_snapshot(s, 50, W.DT_1FS, 0.5), 0.5)
+    print(r)
+    assert r["force_rms_rel"] <= 1e-5
+    assert r["energy_per_atom_max_abs_ev"] <= 1e-6
+    assert r["pe_rel"] <= 1e-6
+    assert r["pressure_rel"] <= 1e-5
