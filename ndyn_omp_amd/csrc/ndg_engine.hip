@@ -202,6 +202,7 @@ struct ndg_engine {
   // distinct FP32 force tables of all replicas (staged in LDS by the force kernel)
   float *d_tabset = nullptr;
   size_t tabset_cap = 0;
+  std::vector<float> tabset_host;   // host copy of the staged table set
   int tabset_n = 0;      // floats of the F half
   int tabset_p = 0;      // 1: the energy/virial variant stages the P half too
 };
@@ -418,13 +419,26 @@ static int rebuild_tabset(ndg_engine *e) {
   const bool fits = !all.empty() && all.size() * sizeof(float) <= (size_t)kLdsTableBytes;
   const bool fits_p = fits && 2 * all.size() * sizeof(float) <= (size_t)kLdsTableBytesFull;
   all.insert(all.end(), allp.begin(), allp.end());
-  for (auto &s : e->slots) {
-    for (int p = 0; p < 16; ++p) s.P.ltab[p] = -1;
-    if (!s.used || !s.table_key || !fits) continue;
-    auto it = e->tables.find(s.table_key);
-    for (int p = 0; p < s.npairs; ++p) s.P.ltab[p] = off[it->second.phash[p]];
+  // a slot's parameters are uploaded again only when its table offsets move (admitting a
+  // job next to many resident ones must not cost one upload per resident job)
+  std::vector<int> changed;
+  for (size_t r = 0; r < e->slots.size(); ++r) {
+    Slot &s = e->slots[r];
+    int lt[16];
+    for (int p = 0; p < 16; ++p) lt[p] = -1;
+    if (s.used && s.table_key && fits) {
+      auto it = e->tables.find(s.table_key);
+      for (int p = 0; p < s.npairs; ++p) lt[p] = off[it->second.phash[p]];
+    }
+    if (std::memcmp(lt, s.P.ltab, sizeof(lt)) != 0) {
+      std::memcpy(s.P.ltab, lt, sizeof(lt));
+      changed.push_back((int)r);
+    }
   }
-  if (fits) {
+  if (fits && all == e->tabset_host) {
+    // the staged table set is already on the device
+  } else if (fits) {
+    e->tabset_host = all;
     if (all.size() > e->tabset_cap) {
       if (e->d_tabset) {
         HIPCHK(hipStreamSynchronize(e->stream));
@@ -435,9 +449,10 @@ static int rebuild_tabset(ndg_engine *e) {
     }
     HIPCHK(hipMemcpyAsync(e->d_tabset, all.data(), sizeof(float) * all.size(), hipMemcpyHostToDevice, e->stream));
   }
+  if (!fits) e->tabset_host.clear();
   e->tabset_n = fits ? (int)(all.size() / 2) : 0;
   e->tabset_p = fits_p ? 1 : 0;
-  for (size_t r = 0; r < e->slots.size(); ++r)
+  for (int r : changed)
     if (e->slots[r].used && e->slots[r].block)
       HIPCHK(hipMemcpyAsync(e->d_reps + r, &e->slots[r].P, sizeof(RepParams), hipMemcpyHostToDevice, e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
