@@ -286,7 +286,7 @@ def main():
     ap.add_argument("--size-scale", type=float, default=0.25)
     ap.add_argument("--step-scale", type=float, default=0.1)
     # saturated wall-clock throughput of this job sample (arrival rate far above it)
-    ap.add_argument("--capacity", type=float, default=1.15e9)
+    ap.add_argument("--capacity", type=float, default=1.8e9)
     ap.add_argument("--tick-steps", type=int, default=50)
     ap.add_argument("--drift-steps", type=int, default=10000)
     args = ap.parse_args()
