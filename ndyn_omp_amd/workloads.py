@@ -176,6 +176,36 @@ def buck_alloy(n: Sequence[int], a: float = 5.6, temp_k: float = 600.0, seed: in
                   params, 0.25, ptsize, f"buck_alloy_{nx}x{ny}x{nz}")
 
 
+def asym_morse(nx: int, variant: int = 1, a: float = 3.615, temp_k: float = 300.0,
+               seed: int = 12350, jitter_a: float = 0.0, ptsize: int = 5000) -> System:
+    """Two species (50/50, masses 63.5 and 27 amu) on an FCC lattice with directed Morse
+    pairs, as written by the reference's `asym` pair lines (src/read_input.f90:127-133:
+    only PairMat(A,B) is set).  An atom of type t feels a neighbour of type u when
+    PairMat(u,t) > 0 (src/nd.f90:474-483):
+      variant 1: PairMat = [[1, 2], [0, 3]] -- A-A; PairMat(A,B) = 2, PairMat(B,A) = 0, so
+                 B atoms feel A atoms and A atoms do not feel B atoms; B-B;
+      variant 2: PairMat = [[1, 2], [3, 4]] -- A-A; both directions set, with different
+                 ids (the pair uses table 3 both ways); B-B.
+    Each pair has its own Morse parameters, so the table choice (the larger id of the two
+    directions) is visible in the forces."""
+    pos, box = fcc(nx, nx, nx, a)
+    if jitter_a > 0:
+        pos = jitter(pos, box, jitter_a, seed)
+    u = uniform(seed, 13, pos.shape[0])
+    typ = np.where(u < 0.5, 1, 2).astype(np.int32)
+    masses = np.array([63.5, 27.0])
+    vel = thermal_velocities(typ, masses, temp_k, seed)
+    prm = [CU_MORSE, (6.8, 0.25, 1.30, 2.90), (7.0, 0.40, 1.40, 2.85), (6.9, 0.30, 1.35, 2.88)]
+    if variant == 1:
+        pairmat = np.array([[1, 2], [0, 3]], dtype=np.int32)
+        prm = [prm[0], prm[1], prm[3]]
+    else:
+        pairmat = np.array([[1, 2], [3, 4]], dtype=np.int32)
+    n = len(prm)
+    return System(pos, vel, typ, box, masses, pairmat, np.array([MORSE] * n, dtype=np.int32),
+                  np.array(prm, dtype=np.float64), 0.25, ptsize, f"asym_morse{variant}_{nx}")
+
+
 def near_cubic_cells(n_atoms: float) -> tuple:
     """Near-cubic (nx,ny,nz) with 4*nx*ny*nz closest to n_atoms."""
     target = max(1.0, n_atoms / 4.0)

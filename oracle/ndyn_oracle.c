@@ -14,6 +14,8 @@
  *                       entry order included) + brute-force pair sets
  *   ndo_compute0        golden forces/PE/virial from the compiled reference, FCC
  *                       shell sums, the reference logs' PE/pressure, Newton 3
+ *   ndo_compute1        golden forces/PE/virial of asymmetric pair matrices from the
+ *                       compiled reference; equals ndo_compute0 on symmetric matrices
  *   ndo_integrate1/2,   golden multi-step 'dynamics' runs from the compiled reference
  *   ndo_mk_iso_*        (NVE, NVT, NPT, isoenergetic) + the testT NPT log
  *   ndo_check_list_update  compat: golden runs with ReNeighbor T; SKIN mode is not in
@@ -316,6 +318,95 @@ void ndo_compute0(int np, const double *pos, const int *typ, const double box[3]
   free(vdr2);
 }
 
+/* compute1  src/nd.f90:364-530 (asymmetric pair matrices), serial.  For a half-list pair
+ * i<j the table is that of max(PairMat(ti,tj), PairMat(tj,ti)) (:445-447); i takes the
+ * pair force only if PairMat(tj,ti) > 0 (:474-478), j only if PairMat(ti,tj) > 0
+ * (:479-483); energy and virial count only pairs with PairMat(ti,tj) = PairMat(tj,ti)
+ * (:485-498).  The reference leaves k unclamped here (:466-467, the guard is commented out);
+ * pairs below PTrMin do not occur in valid inputs, and the clamp of compute0 is kept so
+ * that such a pair cannot index outside the table.  pene: 1/2 of each counted pair's
+ * energy to both atoms (the reference leaves it at zero). */
+void ndo_compute1(int np, const double *pos, const int *typ, const double box[3],
+                  const int *vnl, int neimax, int ntypes, const int *pairmat,
+                  const double *P, const double *F, const double *rc2, const double *dr2,
+                  double ptrmin, int ptsize, int cv, int cu,
+                  double *force, double *pe, double virial[6], double *pene) {
+  const int stride = neimax + 1;
+  const int tab = ptsize + 1;
+  double vir[6] = {0, 0, 0, 0, 0, 0};
+  double pot = 0.0;
+  int npairs_max = ntypes * ntypes;
+  float *vdr2 = (float *)malloc(sizeof(float) * (npairs_max + 1));
+  for (int p = 0; p < npairs_max + 1; ++p) vdr2[p] = 0.0f;
+  for (int a = 0; a < npairs_max; ++a) {
+    int p = pairmat[a];
+    if (p > 0) vdr2[p] = (float)(1.0 / dr2[p - 1]);
+  }
+  memset(force, 0, sizeof(double) * 3 * (size_t)np);
+  if (cu && pene) memset(pene, 0, sizeof(double) * (size_t)np);
+  for (int i = 0; i < np; ++i) {
+    const double *pi = pos + 3 * i;
+    int ti = typ[i] - 1;
+    int n = vnl[stride * i];
+    for (int jn = 0; jn < n; ++jn) {
+      int j = vnl[stride * i + 1 + jn];
+      int tj = typ[j] - 1;
+      int pij = pairmat[ti + tj * ntypes], pji = pairmat[tj + ti * ntypes];
+      int mx = pij > pji ? pij : pji;
+      double rij[3];
+      double d2 = pair_d2(pi, pos + 3 * j, box, rij);
+      if (d2 > rc2[mx - 1]) continue;
+      float rk = (float)(d2 - ptrmin) * vdr2[mx] + 1.0f;
+      int k = (int)rk;
+      if (k < 1) k = 1;
+      double w = (double)rk - (double)k;
+      const double *Ft = F + (size_t)tab * (mx - 1);
+      double df = w * Ft[k] + (1.0 - w) * Ft[k - 1];
+      df = df / d2;
+      if (pji > 0) {
+        force[3 * i + 0] = force[3 * i + 0] + rij[0] * df;
+        force[3 * i + 1] = force[3 * i + 1] + rij[1] * df;
+        force[3 * i + 2] = force[3 * i + 2] + rij[2] * df;
+      }
+      if (pij > 0) {
+        force[3 * j + 0] = force[3 * j + 0] - rij[0] * df;
+        force[3 * j + 1] = force[3 * j + 1] - rij[1] * df;
+        force[3 * j + 2] = force[3 * j + 2] - rij[2] * df;
+      }
+      if (cu && pij == pji) {
+        const double *Pt = P + (size_t)tab * (mx - 1);
+        double dp = w * Pt[k] + (1.0 - w) * Pt[k - 1];
+        pot = pot + dp;
+        if (pene) {
+          pene[i] += 0.5 * dp;
+          pene[j] += 0.5 * dp;
+        }
+      }
+      if (cv && pij == pji) {
+        vir[0] = vir[0] + rij[0] * rij[0] * df;
+        vir[1] = vir[1] + rij[1] * rij[1] * df;
+        vir[2] = vir[2] + rij[2] * rij[2] * df;
+        vir[3] = vir[3] + rij[0] * rij[1] * df;
+        vir[4] = vir[4] + rij[0] * rij[2] * df;
+        vir[5] = vir[5] + rij[1] * rij[2] * df;
+      }
+    }
+  }
+  if (cv)
+    for (int a = 0; a < 6; ++a) virial[a] = 0.0 + vir[a];
+  if (cu) *pe = 0.0 + pot;
+  free(vdr2);
+}
+
+/* Some PairMat(a,b) != PairMat(b,a): the reference's 'asym' flag (src/read_input.f90:130-133),
+ * which selects compute1 in 'dynamics' (src/nd.f90:155-163). */
+int ndo_asymmetric(int ntypes, const int *pairmat) {
+  for (int a = 0; a < ntypes; ++a)
+    for (int b = 0; b < ntypes; ++b)
+      if (pairmat[a + b * ntypes] != pairmat[b + a * ntypes]) return 1;
+  return 0;
+}
+
 /* ---------------------------------------------------------------- integrator */
 
 void ndo_integrate1(int np, double *pos, const double *vel, const double *acc,
@@ -436,6 +527,7 @@ int ndo_run(const ndo_run_params *rp, int np, int ntypes, int npairs, int ptsize
   int err = ndo_build_half_list(np, pos, typ, box, mcell, ntypes, pairmat, rcp, skin,
                                 ptrmin, neimax, rp->brute, vnl, NULL);
   int nup = 0, nr = 0;
+  const int asym = ndo_asymmetric(ntypes, pairmat);
   double pe = 0.0, ke = 0.0, e0 = 0.0, vir[6] = {0, 0, 0, 0, 0, 0};
   int step = rp->step0;
   for (int tt = 1; tt <= rp->nsteps && !err; ++tt) {
@@ -459,8 +551,12 @@ int ndo_run(const ndo_run_params *rp, int np, int ntypes, int npairs, int ptsize
       }
     }
     ndo_integrate1(np, pos, vel, acc, disp, box, rp->dt);
-    ndo_compute0(np, pos, typ, box, vnl, neimax, ntypes, pairmat, P, F, rc2, dr2, ptrmin,
-                 ptsize, cv, cu, force, &pe, vir, pene);
+    if (asym)
+      ndo_compute1(np, pos, typ, box, vnl, neimax, ntypes, pairmat, P, F, rc2, dr2, ptrmin,
+                   ptsize, cv, cu, force, &pe, vir, pene);
+    else
+      ndo_compute0(np, pos, typ, box, vnl, neimax, ntypes, pairmat, P, F, rc2, dr2, ptrmin,
+                   ptsize, cv, cu, force, &pe, vir, pene);
     ndo_integrate2(np, vel, acc, force, typ, mass, rp->dt, ck, &ke);
     if (tt == 1) e0 = pe + ke;
     if (step % rp->out_steps == 0 || tt == 1) {

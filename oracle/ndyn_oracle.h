@@ -74,6 +74,13 @@ void ndo_compute0(int np, const double *pos, const int *typ, const double box[3]
                   const double *P, const double *F, const double *rc2, const double *dr2,
                   double ptrmin, int ptsize, int cv, int cu,
                   double *force, double *pe, double virial[6], double *pene);
+/* compute1 (asymmetric pair matrices)  src/nd.f90:364-530 */
+void ndo_compute1(int np, const double *pos, const int *typ, const double box[3],
+                  const int *vnl, int neimax, int ntypes, const int *pairmat,
+                  const double *P, const double *F, const double *rc2, const double *dr2,
+                  double ptrmin, int ptsize, int cv, int cu,
+                  double *force, double *pe, double virial[6], double *pene);
+int ndo_asymmetric(int ntypes, const int *pairmat);
 
 /* integrate1 (src/nd.f90:984-1046): drift, scaled displacement accumulation. */
 void ndo_integrate1(int np, double *pos, const double *vel, const double *acc,

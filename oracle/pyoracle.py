@@ -64,6 +64,10 @@ def lib():
                                    C.c_double, C.c_int, C.c_int, C.c_int, dp,
                                    C.POINTER(C.c_double), dp, C.c_void_p]
         L.ndo_compute0.restype = None
+        L.ndo_compute1.argtypes = L.ndo_compute0.argtypes
+        L.ndo_compute1.restype = None
+        L.ndo_asymmetric.argtypes = [C.c_int, ip]
+        L.ndo_asymmetric.restype = C.c_int
         L.ndo_integrate1.argtypes = [C.c_int, dp, dp, dp, dp, dp, C.c_double]
         L.ndo_integrate1.restype = None
         L.ndo_integrate2.argtypes = [C.c_int, dp, dp, dp, ip, dp, C.c_double, C.c_int,
@@ -156,13 +160,34 @@ def neighbour_sets_from_half(vnl):
     return [b[starts[i]:starts[i + 1]] for i in range(npart)]
 
 
+def asymmetric(sys) -> bool:
+    """The reference's 'asym' flag: some PairMat(a,b) != PairMat(b,a)."""
+    pm = np.ascontiguousarray(sys.pairmat.T).ravel().astype(np.int32)
+    return bool(lib().ndo_asymmetric(sys.ntypes, pm))
+
+
+def compute(sys, vnl, neimax, tables, cv=True, cu=True):
+    """The force routine 'dynamics' would call: compute1 for asymmetric pair matrices,
+    compute0 otherwise (src/nd.f90:155-163)."""
+    fn = compute1 if asymmetric(sys) else compute0
+    return fn(sys, vnl, neimax, tables, cv, cu)
+
+
+def compute1(sys, vnl, neimax, tables, cv=True, cu=True):
+    return _compute(lib().ndo_compute1, sys, vnl, neimax, tables, cv, cu)
+
+
 def compute0(sys, vnl, neimax, tables, cv=True, cu=True):
+    return _compute(lib().ndo_compute0, sys, vnl, neimax, tables, cv, cu)
+
+
+def _compute(fn, sys, vnl, neimax, tables, cv, cu):
     P, F, rc2, dr2 = tables
     force = np.zeros((sys.np, 3))
     pe = C.c_double(0.0)
     vir = np.zeros(6)
     pene = np.zeros(sys.np)
-    lib().ndo_compute0(sys.np, np.ascontiguousarray(sys.pos), sys.typ.astype(np.int32),
+    fn(sys.np, np.ascontiguousarray(sys.pos), sys.typ.astype(np.int32),
                        np.ascontiguousarray(sys.box), np.ascontiguousarray(vnl), neimax,
                        sys.ntypes, np.ascontiguousarray(sys.pairmat.T).ravel().astype(np.int32),
                        np.ascontiguousarray(P), np.ascontiguousarray(F), rc2, dr2,

@@ -15,7 +15,8 @@
 !   real64 mass(ntypes); int32 pairmat(ntypes,ntypes); int32 kind(npairs)
 !   real64 pairparams(4,npairs); int32 typ(np); real64 pos(3,np), vel(3,np), acc(3,np)
 ! Modes: 1 tables (buildPairTables), 2 half list (buildCellList),
-!        3 forces (buildCellList + compute0(T,T)), 4 'dynamics' run.
+!        3 forces (buildCellList + compute0(T,T), or compute1(T,T) for an asymmetric
+!        PairMat), 4 'dynamics' run.
 program ref_harness
   use omp_lib
   use mostCommon
@@ -69,7 +70,8 @@ program ref_harness
       case default; stop 'harness: unknown pair kind'
     end select
   enddo
-  asympots = .false.; springpots = .false.; potOuts = .false.
+  ! the deck's 'asym' flag (src/read_input.f90:130-133): some PairMat(a,b) /= PairMat(b,a)
+  asympots = any(PairMat /= transpose(PairMat)); springpots = .false.; potOuts = .false.
 
   allocate(typ(np), pos(3,np), vel(3,np), acc(3,np), force(3,np), pene(np))
   read(20) typ
@@ -110,7 +112,11 @@ program ref_harness
       write(21) MCELL, NeiMax
       write(21) VNL
     else
-      call compute0(.true., .true.)
+      if (asympots) then
+        call compute1(.true., .true.)
+      else
+        call compute0(.true., .true.)
+      endif
       write(21) force, potential, virial
     endif
   case (4)

@@ -111,9 +111,21 @@ def golden_run(fname, s, nsteps, dt, skin, outsteps, **kw):
     np.savez_compressed(os.path.join(OUT, fname), **sys_arrays(s), **meta, **res)
 
 
+def asym_fixtures():
+    """Asymmetric pair matrices (compute1): one-way A->B, and two directed tables."""
+    golden_list("list_asym1.npz", W.asym_morse(7, 1, jitter_a=0.05, seed=121), 0.5)
+    golden_force("force_asym1.npz", W.asym_morse(7, 1, jitter_a=0.05, seed=122), 0.5)
+    golden_force("force_asym2.npz", W.asym_morse(7, 2, jitter_a=0.05, seed=123), 0.5)
+    golden_run("run_asym2_nve.npz", W.asym_morse(7, 2, temp_k=300.0, seed=12351), 60,
+               W.DT_1FS, 0.5, 20)
+
+
 def main():
     if not os.path.exists(HARNESS):
         raise SystemExit("build the reference harness first: make -C oracle ref")
+    if "--asym-only" in sys.argv:
+        asym_fixtures()
+        return
     golden_tables()
     # half lists: Cu 7^3 (MCELL 3), c0-size Cu 10^3 (MCELL 4), Buckingham alloy (3 pairs)
     golden_list("list_cu7.npz", W.cu_morse(7, jitter_a=0.05, seed=101), 0.5)
@@ -134,6 +146,7 @@ def main():
                iso_p=((0.0, 0.0, 0.0), 1000 * W.DT_1FS, 10))
     golden_run("run_alloy5_nve.npz", W.buck_alloy((5, 5, 5), temp_k=900.0, seed=12348), 60,
                W.DT_1FS, 0.5, 20, iso_e=(50 * W.DT_1FS, 5))
+    asym_fixtures()
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f"{f:28s} {os.path.getsize(os.path.join(OUT, f)) / 1024:8.1f} KiB")

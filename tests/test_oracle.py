@@ -54,7 +54,7 @@ def test_tables_closed_form(oracle):
 
 # ----------------------------------------------------------------- lists
 @pytest.mark.parametrize("fname", ["list_cu7.npz", "list_cu10.npz", "list_alloy5.npz",
-                                   "list_ar6.npz"])
+                                   "list_ar6.npz", "list_asym1.npz"])
 def test_half_list_matches_reference_exactly(oracle, fname):
     g = golden(fname)
     s = system_from_golden(g)
@@ -118,6 +118,44 @@ def test_compute0_matches_reference(oracle, fname):
     # invariants: Newton 3 and sum of per-atom energies
     assert np.abs(f.sum(axis=0)).max() < 1e-11 * max(1.0, scale) * math.sqrt(s.np)
     assert pene.sum() == pytest.approx(pe, rel=1e-12)
+
+
+@pytest.mark.parametrize("fname", ["force_asym1.npz", "force_asym2.npz"])
+def test_compute1_matches_reference(oracle, fname):
+    """Asymmetric pair matrices: the reference's compute1 (src/nd.f90:364-530).  compute0
+    would give other forces (both directions feel every pair); the invariants that hold
+    are the per-atom energy sum and, with both directions set (asym2), Newton 3."""
+    g = golden(fname)
+    s = system_from_golden(g)
+    assert oracle.asymmetric(s)
+    skin = float(g["skin"])
+    tables = oracle.pair_tables(s.kinds, s.params, s.ptrmin, s.ptsize)
+    _, vnl, neimax = oracle.half_list(s, skin)
+    f, pe, vir, pene = oracle.compute(s, vnl, neimax, tables)
+    scale = np.abs(g["force"]).max()
+    np.testing.assert_allclose(f, g["force"], rtol=1e-11, atol=1e-13 * scale)
+    assert pe == pytest.approx(float(g["pe"]), rel=1e-13)
+    np.testing.assert_allclose(vir, g["virial"], rtol=1e-12, atol=1e-12 * np.abs(g["virial"]).max())
+    assert pene.sum() == pytest.approx(pe, rel=1e-12)
+    f0, _, _, _ = oracle.compute0(s, vnl, neimax, tables)
+    assert np.abs(f0 - g["force"]).max() > 1e-3 * scale    # the fixture tells the two apart
+    if (s.pairmat > 0).all():
+        assert np.abs(f.sum(axis=0)).max() < 1e-11 * max(1.0, scale) * math.sqrt(s.np)
+    else:
+        # one-way pairs: the net force is the sum of the unanswered pair forces
+        assert np.abs(f.sum(axis=0)).max() > 1e-6
+
+
+def test_compute1_equals_compute0_on_symmetric_matrix(oracle):
+    g = golden("force_alloy5.npz")
+    s = system_from_golden(g)
+    tables = oracle.pair_tables(s.kinds, s.params, s.ptrmin, s.ptsize)
+    _, vnl, neimax = oracle.half_list(s, float(g["skin"]))
+    a = oracle.compute0(s, vnl, neimax, tables)
+    b = oracle.compute1(s, vnl, neimax, tables)
+    np.testing.assert_array_equal(a[0], b[0])
+    assert a[1] == b[1]
+    np.testing.assert_array_equal(a[2], b[2])
 
 
 def _fcc_shells(a, rmax):
@@ -184,7 +222,7 @@ def test_reference_log_known_answers(oracle):
 
 # ----------------------------------------------------------------- dynamics
 @pytest.mark.parametrize("fname", ["run_cu10_nve.npz", "run_ar6_nvt.npz", "run_cu7_npt.npz",
-                                   "run_alloy5_nve.npz"])
+                                   "run_alloy5_nve.npz", "run_asym2_nve.npz"])
 def test_dynamics_matches_reference(oracle, fname):
     g = golden(fname)
     s = system_from_golden(g)
