@@ -129,7 +129,9 @@ int ndg_check_rebuild(ndg_t *h, double skin, int trigger, int *rebuilt);
 /* integrate1 (src/nd.f90:984-1046). */
 int ndg_integrate1(ndg_t *h, double dt);
 
-/* compute0 (src/nd.f90:200-363): forces; potential when cu, virial[6] when cv. */
+/* compute0 (src/nd.f90:200-363), or compute1 (src/nd.f90:364-530) when the PairMat given to
+ * ndg_set_tables is not symmetric (the reference's choice, src/nd.f90:155-163): forces;
+ * potential when cu, virial[6] when cv. */
 int ndg_compute(ndg_t *h, int cv, int cu, double *pe, double virial[6]);
 
 /* integrate2 (src/nd.f90:1081-1121): kick; kinetic energy when ck. */

@@ -667,6 +667,10 @@ static int upload_host(ndg_engine *e, Slot &s, const HostState &h, bool state_to
   for (double r : rcp) rcut = std::max(rcut, r);
   P.rcut = rcut;
   P.ptrmin = h.ptrmin;
+  P.asym = 0;
+  for (int a = 0; a < h.ntypes; ++a)
+    for (int b = 0; b < h.ntypes; ++b)
+      if (h.pairmat[a + b * h.ntypes] != h.pairmat[b + a * h.ntypes]) P.asym = 1;
   hipStream_t st = e->stream;
   HIPCHK(hipMemcpyAsync((void *)P.mass, h.mass.data(), 8 * h.ntypes, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync((void *)P.pairmat, h.pairmat.data(), 4 * h.ntypes * h.ntypes, hipMemcpyHostToDevice, st));
@@ -1230,11 +1234,7 @@ static int check_host(const HostState &h) {
     if (t < 1 || t > h.ntypes) return fail(NDG_ERR_ARG, "atom type out of range");
   for (int p : h.pairmat)
     if (p < 0 || p > h.npairs) return fail(NDG_ERR_ARG, "pair id out of range");
-  // symmetric pairs only (compute0); asymmetric matrices are the reference's compute1
-  for (int a = 0; a < h.ntypes; ++a)
-    for (int b = 0; b < h.ntypes; ++b)
-      if (h.pairmat[a + b * h.ntypes] != h.pairmat[b + a * h.ntypes])
-        return fail(NDG_ERR_ARG, "asymmetric pair matrix is not supported (compute1)");
+  // an asymmetric matrix selects the reference's compute1 rules (src/nd.f90:155-163)
   for (int a = 0; a < 3; ++a)
     if (!(h.box[a] > 0.0)) return fail(NDG_ERR_ARG, "box must be positive");
   return NDG_OK;

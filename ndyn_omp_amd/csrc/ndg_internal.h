@@ -49,7 +49,7 @@ struct RepParams {
   int run_step0, run_nsteps, out_steps;
   int renei, renei_freq, trigger;
   int isokinetic, ke_freq, isobaric, p_freq, isoenergetic, e_freq, box_load;
-  int pad0;
+  int asym;                 // PairMat not symmetric: compute1's rules (src/nd.f90:364-530)
   double dt, skin, rcut, ptrmin;
   double ke_target, ke_tau, p_target[3], p_tau, e_tau, box_vel[3];
   // per-type / per-pair data

@@ -287,7 +287,7 @@ struct ForceShape {
   static constexpr int kNSub = kThreads / kSub;            // tiles per pass
 };
 
-template <int TPA, bool FULL, bool LDS_TAB, bool ONE>
+template <int TPA, bool FULL, bool LDS_TAB, bool ONE, bool ASYM = false>
 __device__ inline void force_atom(const RepParams &R, int i, int sub, const double4 *shift,
                                   const double *rc2, const float *vdr2, const int *pm,
                                   const int *lt, const float *stab, int ptoff, const unsigned *L,
@@ -348,13 +348,21 @@ __device__ inline void force_atom(const RepParams &R, int i, int sub, const doub
       double rc2p = rc2_0;
       float vd = vdr2_0;
       int ltp = lt_0;
+      bool feel = true, bal = true;   // ASYM: the neighbour acts on this atom; balanced pair
       if (!ONE) {
-        pid = pm[(ti - 1) + (int)pj[u].w * nt_] - 1;
+        const int pij = pm[(ti - 1) + (int)pj[u].w * nt_];
+        pid = pij - 1;
+        if (ASYM) {   // compute1 (src/nd.f90:445-498): the larger id of the two directions
+          const int pji = pm[(int)pj[u].w + (ti - 1) * nt_];
+          pid = (pij > pji ? pij : pji) - 1;
+          feel = pji > 0;
+          bal = pij == pji;
+        }
         rc2p = rc2[pid];
         vd = vdr2[pid];
         if (LDS_TAB) ltp = lt[pid];
       }
-      const bool ok = !(d2 > rc2p);
+      const bool ok = !(d2 > rc2p) && feel;
       // the reference's table position, in its single-precision operations
       // (src/nd.f90:299): rk = real(d2 - PTrMin) * vPTdr2 + 1
       const float rk = (float)(d2 - ptrmin) * vd + 1.0f;
@@ -387,10 +395,11 @@ __device__ inline void force_atom(const RepParams &R, int i, int sub, const doub
           P0 = TP[q];
           P1 = TP[q + 1];
         }
-        const double pv = ok ? (double)fmaf(w, P1 - P0, P0) : 0.0;
+        const double pv = ok && bal ? (double)fmaf(w, P1 - P0, P0) : 0.0;
         // energy and virial sums without their factor 1/2, applied once per atom (an
-        // exact scaling)
-        const double gx = dx * df, gy = dy * df, gz = dz * df;
+        // exact scaling); an unbalanced pair of an asymmetric matrix adds to neither
+        const double dv = !ASYM || bal ? df : 0.0;
+        const double gx = dx * dv, gy = dy * dv, gz = dz * dv;
         acc[0] += pv;
         acc[1] = fma(dx, gx, acc[1]);
         acc[2] = fma(dy, gy, acc[2]);
@@ -484,8 +493,15 @@ __global__ __launch_bounds__(ForceShape<TPA>::kThreads) void k_force_mixed(Engin
     if (have && i < R.np) {
       const unsigned *L = R.list + ((size_t)T.ltile * R.list_chunks) * (kTile * TPA) + lt;
       // both choices are the same for every wave of the tile
-      const bool lds = sh_uselds[sg] != 0, one = R.npairs == 1;
-      if (lds && one)
+      const bool asym = R.asym != 0;
+      const bool lds = sh_uselds[sg] != 0, one = R.npairs == 1 && !asym;
+      if (asym && lds)
+        force_atom<TPA, FULL, true, false, true>(R, i, sub, sh_shift[sg], sh_rc2[sg], sh_vdr2[sg],
+                                                 sh_pm[sg], sh_lt[sg], sh_tab, ptoff, L, qunit, fx, fy, fz, acc);
+      else if (asym)
+        force_atom<TPA, FULL, false, false, true>(R, i, sub, sh_shift[sg], sh_rc2[sg], sh_vdr2[sg],
+                                                  sh_pm[sg], sh_lt[sg], sh_tab, ptoff, L, qunit, fx, fy, fz, acc);
+      else if (lds && one)
         force_atom<TPA, FULL, true, true>(R, i, sub, sh_shift[sg], sh_rc2[sg], sh_vdr2[sg],
                                           sh_pm[sg], sh_lt[sg], sh_tab, ptoff, L, qunit, fx, fy, fz, acc);
       else if (lds)
@@ -655,7 +671,18 @@ __global__ __launch_bounds__(kTile) void k_force_fp64(EngineDev E, int t0, int n
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          const int pid = R.pairmat[(ti - 1) + (tj[u] - 1) * R.ntypes] - 1;
+          // compute0: the pair's own id; compute1 (asymmetric matrix, src/nd.f90:445-498):
+          // the larger id of the two directions, the force only if the neighbour's type
+          // acts on this atom's type, energy and virial only for balanced pairs
+          const int pij = R.pairmat[(ti - 1) + (tj[u] - 1) * R.ntypes];
+          int pid = pij - 1;
+          bool feel = true, bal = true;
+          if (R.asym) {
+            const int pji = R.pairmat[(tj[u] - 1) + (ti - 1) * R.ntypes];
+            pid = (pij > pji ? pij : pji) - 1;
+            feel = pji > 0;
+            bal = pij == pji;
+          }
           double s0 = xi - qx[u], s1 = yi - qy[u], s2 = zi - qz[u];
           s0 = s0 - round(s0);
           s1 = s1 - round(s1);
@@ -664,7 +691,7 @@ __global__ __launch_bounds__(kTile) void k_force_fp64(EngineDev E, int t0, int n
           double d2 = r0 * r0;
           d2 = d2 + r1 * r1;
           d2 = d2 + r2 * r2;
-          const bool ok = (c + u < n) && !(d2 > R.rc2d[pid]);
+          const bool ok = (c + u < n) && !(d2 > R.rc2d[pid]) && feel;
           const float rk = (float)(d2 - R.ptrmin) * R.vdr2[pid] + 1.0f;
           int k = (int)rk;
           k = k < 1 ? 1 : (k > kmax ? kmax : k);
@@ -677,14 +704,15 @@ __global__ __launch_bounds__(kTile) void k_force_fp64(EngineDev E, int t0, int n
           fz = fz + r2 * df;
           if (FULL) {
             const double *Pt = R.tabP64 + (size_t)pid * tab;
-            const double dp = ok ? w * Pt[k] + (1.0 - w) * Pt[k - 1] : 0.0;
+            const double dp = ok && bal ? w * Pt[k] + (1.0 - w) * Pt[k - 1] : 0.0;
+            const double dv = bal ? df : 0.0;
             acc[0] += 0.5 * dp;
-            acc[1] += 0.5 * (r0 * r0 * df);
-            acc[2] += 0.5 * (r1 * r1 * df);
-            acc[3] += 0.5 * (r2 * r2 * df);
-            acc[4] += 0.5 * (r0 * r1 * df);
-            acc[5] += 0.5 * (r0 * r2 * df);
-            acc[6] += 0.5 * (r1 * r2 * df);
+            acc[1] += 0.5 * (r0 * r0 * dv);
+            acc[2] += 0.5 * (r1 * r1 * dv);
+            acc[3] += 0.5 * (r2 * r2 * dv);
+            acc[4] += 0.5 * (r0 * r1 * dv);
+            acc[5] += 0.5 * (r0 * r2 * dv);
+            acc[6] += 0.5 * (r1 * r2 * dv);
           }
         }
       }

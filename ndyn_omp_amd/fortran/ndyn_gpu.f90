@@ -10,8 +10,9 @@
 ! src/nd.f90:60-199, through ndg_run_begin / ndg_run_steps / ndg_run_end).
 ! Two directives are added: 'Precision mixed|fp64' selects the engine path and
 ! 'ListTrigger compat|skin' the neighbour-list rebuild criterion.
-! Group/Hold/Disp/ChangeType/InitPos and asymmetric or spring pairs (compute1-3) are not
-! part of this engine; they are reported and ignored.
+! Asymmetric pairs ('asym', the reference's compute1) run on the GPU as well.
+! Group/Hold/Disp/ChangeType/InitPos and spring pairs (compute2-3) are not part of this
+! engine; they are reported and ignored.
 program ndyn_gpu
   use iso_c_binding
   use ndg_iface
@@ -364,11 +365,9 @@ contains
       p2 = type_id(w(2))
       if (p1 == 0 .or. p2 == 0) call die('ERROR: unknown type in Pairs')
       pairmat(p1,p2) = k
-      if (trim(w(3)) == 'sym') then
-        pairmat(p2,p1) = k
-      else
-        call die('ERROR: asymmetric pairs (compute1) are not supported by this engine')
-      endif
+      ! 'asym' leaves PairMat(p2,p1) alone; the engine then applies compute1's rules
+      ! (src/read_input.f90:130-133, src/nd.f90:155-163)
+      if (trim(w(3)) == 'sym') pairmat(p2,p1) = k
       pairname(k) = trim(w(4))
       select case (trim(w(4)))
       case ('morse', 'Morse')

@@ -143,3 +143,62 @@ END
     assert got == [(1, -13581.1, 49836.0), (100, -12798.9, 56637.3), (200, -12798.8, 56637.5)]
     dump = (tmp_path / "out.dump").read_text()
     assert dump.count("ITEM: TIMESTEP") == 3
+
+
+def write_typed_dump(path, s):
+    """A LAMMPS dump of system s whose values are exact in single precision, and the
+    system the driver reads back from it (src/read_input.f90:347-371 reads real(4))."""
+    f32 = np.float32
+    box = np.array([float(f32(L)) for L in s.box])
+    xs = (s.pos + 0.5).astype(f32)
+    with open(path, "w") as f:
+        f.write("ITEM: TIMESTEP\n0\nITEM: NUMBER OF ATOMS\n%d\nITEM: BOX BOUNDS pp pp pp\n" % s.np)
+        for L in box:
+            f.write(f"0 {float(L)!r}\n")
+        f.write("ITEM: ATOMS id type xs ys zs\n")
+        for k in range(s.np):
+            f.write(f"{k + 1} {int(s.typ[k])} {float(xs[k, 0])!r} {float(xs[k, 1])!r} {float(xs[k, 2])!r}\n")
+    t = s.copy()
+    t.box = box
+    t.pos = (xs - f32(0.5)).astype(np.float64)
+    t.vel = np.zeros_like(s.pos)
+    return t
+
+
+@pytest.mark.gpu
+@needs_driver
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_driver_asym_pairs_deck(tmp_path, oracle, precision):
+    """'asym' pair lines (src/read_input.f90:127-133) select compute1: a two-type deck with
+    one directed pair (B atoms feel A atoms, A atoms do not feel B) and its thermo lines
+    against the oracle's run of the same configuration: PE and volume within 1e-5 relative
+    (the lines print 6 significant digits)."""
+    s = W.asym_morse(6, variant=1, jitter_a=0.05, seed=131)
+    t = write_typed_dump(tmp_path / "model.dump", s)
+    t.ptsize = 1000   # the deck's default table size (src/read_input.f90:28)
+    (A, B, C) = [" ".join(repr(float(v)) for v in s.params[k]) for k in range(3)]
+    deck = f"""START
+Model 	model.dump dump
+Skin 0.5
+Types 2
+ A 63.5
+ B 27.0
+Pairs 3
+ A A sym morse {A}
+ A B asym morse {B}
+ B B sym morse {C}
+Precision {precision}
+OutFreq 20
+Run 60 {W.DT_1FS!r}
+END
+"""
+    print(f"case asym deck {precision}", flush=True)
+    p = run_deck(deck, tmp_path)
+    assert p.returncode == 0, p.stderr
+    rows = thermo(p.stdout)
+    _, ref, err = oracle.run(t, 60, W.DT_1FS, skin=0.5, out_steps=20)
+    assert err == 0
+    assert [int(r[0]) for r in rows] == [r["step"] for r in ref] == [1, 20, 40, 60]
+    for r, q in zip(rows, ref):
+        assert r[1] == pytest.approx(q["pe"], rel=1e-5)
+        assert r[5] == pytest.approx(q["volume"], rel=1e-5)
