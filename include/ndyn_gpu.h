@@ -50,6 +50,9 @@ enum {
   NDG_TRIGGER_SKIN = 1    /* d1 + d2 (A) + (rcut+skin)*box strain > skin; re-grid at rebuild */
 };
 
+/* trajectory formats of write_model (src/read_input.f90:403-421), ndg_frame_write */
+enum { NDG_FRAME_XYZ = 1, NDG_FRAME_DUMP = 2 };
+
 /* pair kinds for ndg_pair_table (reference potentials.f90 names) */
 enum { NDG_MORSE = 1, NDG_BUCK = 2, NDG_LJ = 3, NDG_MORSE_PHYS = 4 };
 
@@ -148,6 +151,44 @@ int ndg_scale_velocities(ndg_t *h, double lambda);
 int ndg_run_begin(ndg_t *h, const ndg_run_params *rp, int step0);
 int ndg_run_steps(ndg_t *h, int nsteps, ndg_thermo *rec, int max_rec, int *nrec);
 int ndg_run_end(ndg_t *h, int *nupdates, int *step);
+
+/* The same time loop without the host waiting on it, so that output is formatted while the
+ * device steps (write_thermo / write_model, src/read_input.f90:394-446, called from the loop
+ * at src/nd.f90:169-172).
+ * ndg_run_enqueue: queue nsteps steps and return (for nsteps above 512 it first waits for
+ *   all but the last 512, the thermo ring's half).
+ * ndg_run_wait: wait for the queued steps; records and errors as ndg_run_steps.
+ * ndg_snapshot: queue, behind the steps already queued, a copy of positions (scaled,
+ *   pos(3,np)), velocities, forces and per-atom energies in caller order, with the box and
+ *   step, into pinned host memory.  One snapshot per engine is held; a new one replaces it.
+ * ndg_snapshot_get: wait for that copy and copy it out (any pointer may be NULL). */
+int ndg_run_enqueue(ndg_t *h, int nsteps);
+int ndg_run_wait(ndg_t *h, ndg_thermo *rec, int max_rec, int *nrec);
+int ndg_snapshot(ndg_t *h);
+int ndg_snapshot_get(ndg_t *h, double *pos, double *vel, double *force, double *pene,
+                     double box[3], int *step);
+
+/* write_model (src/read_input.f90:394-431) done by the library.  ndg_frame_write queues a
+ * frame of the current state (a device copy as ndg_snapshot, behind the queued steps) and
+ * returns; a writer thread formats it with the reference's edit descriptors, byte for byte,
+ * on up to 16 host threads and appends it to `path`.
+ *   NDG_FRAME_XYZ:  (i8) np, a blank line, per atom (A4,x,7EN14.4) type name, pos(3) scaled,
+ *                   vel(3), per-atom energy;
+ *   NDG_FRAME_DUMP: LAMMPS dump header (step i7, np i7, box 2f12.4), per atom
+ *                   (i7,x,i2,3f9.5) id, type, real(pos)+0.0.
+ * names: the ntypes type names, name_len characters each, blank padded (a Fortran
+ * character(len=name_len) array; XYZ only).  Two frames may wait; a third call waits for the
+ * oldest.  ndg_frame_flush waits until every queued frame is written and returns the first
+ * write error (ndg_destroy writes what is still queued). */
+int ndg_frame_write(ndg_t *h, const char *path, int format, const char *names, int name_len);
+int ndg_frame_flush(ndg_t *h);
+
+/* The formatter of ndg_frame_write on caller arrays (host only; no device needed): the
+ * frame's text goes to buf (cap bytes; buf may be NULL to query), *len = its length.
+ * vel, pene and names are read for NDG_FRAME_XYZ only, box for NDG_FRAME_DUMP only. */
+int ndg_format_frame(int format, int np, const double *pos, const double *vel,
+                     const double *pene, const int *typ, const char *names, int name_len,
+                     const double box[3], int step, char *buf, size_t cap, size_t *len);
 
 int ndg_destroy(ndg_t *h);
 const char *ndg_last_error(void);

@@ -21,8 +21,10 @@ HIP_FLAGS = [
     # no packed-FP32 instructions anywhere in the device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-value", "-Wno-unused-result",
+    # the frame writer's host threads
+    "-pthread",
 ]
-SOURCES = ["ndg_kernels.hip", "ndg_engine.hip"]
+SOURCES = ["ndg_kernels.hip", "ndg_engine.hip", "ndg_frames.hip"]
 
 
 def _run(cmd, cwd=None):

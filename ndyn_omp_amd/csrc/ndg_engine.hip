@@ -8,13 +8,18 @@
 
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
+#include "ndg_frames.h"
 #include "ndg_internal.h"
 
 namespace ndg {
@@ -26,6 +31,7 @@ void launch_force(const EngineDev &E, int path, int tpa, bool full, int t0, int 
 void launch_reduce(const EngineDev &E, int r0, int nr, int mode, int xflags, int trigger,
                    hipStream_t s);
 void launch_apply_pending(const EngineDev &E, int r0, int nr, hipStream_t s);
+void launch_snapshot(const EngineDev &E, int r, int np, double *out, hipStream_t s);
 void launch_rebuild(const EngineDev &E, int nt, int r0, int nr, hipStream_t s);
 void launch_copy(const float4 *src, float4 *dst, size_t n, hipStream_t s);
 enum : int { RED_FORCE = 1, RED_KICK = 2, RED_FLIP = 4, RED_STEP = 8, RED_CHECK = 16 };
@@ -190,6 +196,11 @@ struct ndg_engine {
   int tpa_override = 0;
   std::vector<std::pair<int, int>> tpa_classes;   // (np_max, tpa), np_max ascending
   int graph_steps = 0;
+  // output snapshot: device staging and pinned host copy of one replica's output state
+  double *snap_d = nullptr, *snap_h = nullptr;
+  size_t snap_cap = 0, snap_hcap = 0;
+  hipEvent_t snap_ev = nullptr;
+  int snap_slot = -1, snap_np = 0;
   // timing of the force kernel
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending;
@@ -207,12 +218,40 @@ struct ndg_engine {
   int tabset_p = 0;      // 1: the energy/virial variant stages the P half too
 };
 
+// Frames queued by ndg_frame_write.  Each waits in one of two pinned buffers (filled by a
+// device copy behind the queued steps) until the writer thread has formatted it and
+// appended it to its file.
+struct FrameWriter {
+  struct Buf {
+    double *h = nullptr;
+    size_t cap = 0;
+    hipEvent_t ev = nullptr;
+    bool busy = false;
+  };
+  struct Job {
+    int buf = 0, format = 0, np = 0, name_len = 0;
+    std::string path;
+    std::vector<int> typ;
+    std::vector<char> names;
+  };
+  Buf bufs[2];
+  std::deque<Job> q;
+  std::mutex m;
+  std::condition_variable cv;
+  std::thread th;
+  bool stop = false;
+  int err = 0;
+  std::string errmsg;
+};
+
 struct ndg_system {
   ndg_engine *e = nullptr;
   int slot = -1;
   HostState hs;
   int neimax_req = 0;
   int path = NDG_PATH_MIXED;
+  bool queued = false;   // steps queued by ndg_run_enqueue and not yet drained
+  std::unique_ptr<FrameWriter> fw;
 };
 
 // ------------------------------------------------------------------ helpers
@@ -1080,7 +1119,7 @@ static int drain(ndg_engine *e) {
 }
 
 // Advance all slots by up to nsteps steps.
-static int engine_steps(ndg_engine *e, int nsteps) {
+static int engine_steps(ndg_engine *e, int nsteps, bool drain_last = true) {
   int rc = rebuild_tiles(e);
   if (rc) return rc;
   int done_steps = 0;
@@ -1123,13 +1162,14 @@ static int engine_steps(ndg_engine *e, int nsteps) {
       HIPCHK(hipGetLastError());
     }
     done_steps += (int)plans.size();
+    if (!drain_last && done_steps >= nsteps) break;   // the caller drains later
     if ((rc = drain(e))) return rc;
   }
   return NDG_OK;
 }
 
 // Plain FP64 path: one system per launch, replica after replica for each step.
-static int engine_steps_plain(ndg_engine *e, int nsteps) {
+static int engine_steps_plain(ndg_engine *e, int nsteps, bool drain_last = true) {
   int rc = rebuild_tiles(e);
   if (rc) return rc;
   int done_steps = 0;
@@ -1153,13 +1193,76 @@ static int engine_steps_plain(ndg_engine *e, int nsteps) {
     HIPCHK(hipGetLastError());
     if (issued == 0) break;
     done_steps += issued;
+    if (!drain_last && done_steps >= nsteps) break;
     if ((rc = drain(e))) return rc;
   }
   return NDG_OK;
 }
 
-static int run_engine(ndg_engine *e, int nsteps) {
-  return e->path == NDG_PATH_FP64 ? engine_steps_plain(e, nsteps) : engine_steps(e, nsteps);
+static int run_engine(ndg_engine *e, int nsteps, bool drain_last = true) {
+  return e->path == NDG_PATH_FP64 ? engine_steps_plain(e, nsteps, drain_last)
+                                  : engine_steps(e, nsteps, drain_last);
+}
+
+static size_t snapshot_doubles(int np) { return 10 * (size_t)np + 4; }
+
+// Queue a caller-order copy of slot r's output state (snapshot_doubles(np) doubles) into the
+// pinned host buffer `host` behind the work already on the stream, and record `ev` after it.
+// The device staging buffer grows to the largest replica copied and is then reused (copies
+// are ordered on the engine's stream).
+static int snapshot_to(ndg_engine *e, int r, double *host, hipEvent_t ev) {
+  Slot &s = e->slots[r];
+  if (!s.block) return fail(NDG_ERR_ARG, "snapshot: the system has no device state yet");
+  const size_t n = snapshot_doubles(s.np);
+  if (n > e->snap_cap) {
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (e->snap_d) HIPCHK(hipFree(e->snap_d));
+    e->snap_d = nullptr;
+    e->snap_cap = 0;
+    HIPCHK(hipMalloc((void **)&e->snap_d, 8 * n));
+    e->snap_cap = n;
+  }
+  launch_snapshot(dev_view(e), r, s.np, e->snap_d, e->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(host, e->snap_d, 8 * n, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipEventRecord(ev, e->stream));
+  return NDG_OK;
+}
+
+// The engine's own snapshot (ndg_snapshot, ndg_engine_fetch): one pinned buffer, read by
+// snapshot_copy.
+static int snapshot_enqueue(ndg_engine *e, int r) {
+  const size_t n = snapshot_doubles(e->slots[r].np);
+  if (n > e->snap_hcap) {
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (e->snap_h) HIPCHK(hipHostFree(e->snap_h));
+    e->snap_h = nullptr;
+    e->snap_hcap = 0;
+    HIPCHK(hipHostMalloc((void **)&e->snap_h, 8 * n, hipHostMallocDefault));
+    e->snap_hcap = n;
+  }
+  if (!e->snap_ev) HIPCHK(hipEventCreateWithFlags(&e->snap_ev, hipEventDisableTiming));
+  int rc = snapshot_to(e, r, e->snap_h, e->snap_ev);
+  if (rc) return rc;
+  e->snap_slot = r;
+  e->snap_np = e->slots[r].np;
+  return NDG_OK;
+}
+
+static int snapshot_copy(ndg_engine *e, int r, double *pos, double *vel, double *force,
+                         double *pene, double box[3], int *step) {
+  if (e->snap_slot != r) return fail(NDG_ERR_ARG, "snapshot: none taken for this system");
+  HIPCHK(hipEventSynchronize(e->snap_ev));
+  const size_t np = (size_t)e->snap_np;
+  const double *h = e->snap_h;
+  if (pos) memcpy(pos, h, 24 * np);
+  if (vel) memcpy(vel, h + 3 * np, 24 * np);
+  if (force) memcpy(force, h + 6 * np, 24 * np);
+  if (pene) memcpy(pene, h + 9 * np, 8 * np);
+  if (box)
+    for (int a = 0; a < 3; ++a) box[a] = h[10 * np + a];
+  if (step) *step = (int)h[10 * np + 3];
+  return NDG_OK;
 }
 
 // ------------------------------------------------------------------ engine lifecycle
@@ -1203,6 +1306,9 @@ extern "C" int ndg_engine_destroy(ndg_engine_t *e) {
   if (e->d_rb) hipFree(e->d_rb);
   if (e->d_tile0) hipFree(e->d_tile0);
   if (e->d_tabset) hipFree(e->d_tabset);
+  if (e->snap_d) hipFree(e->snap_d);
+  if (e->snap_h) hipHostFree(e->snap_h);
+  if (e->snap_ev) hipEventDestroy(e->snap_ev);
   hipStreamDestroy(e->stream);
   delete e;
   return NDG_OK;
@@ -1413,7 +1519,9 @@ extern "C" int ndg_engine_fetch(ndg_engine_t *e, int slot, double *pos, double *
   EngineDev E = dev_view(e);
   launch_apply_pending(E, slot, 1, e->stream);
   HIPCHK(hipGetLastError());
-  return download_state(e, e->slots[slot], pos, vel, nullptr, force, nullptr, box);
+  int rc = snapshot_enqueue(e, slot);
+  if (rc) return rc;
+  return snapshot_copy(e, slot, pos, vel, force, nullptr, box, nullptr);
 }
 
 extern "C" int ndg_engine_release(ndg_engine_t *e, int slot) {
@@ -1519,6 +1627,9 @@ extern "C" int ndg_engine_list_stats(ndg_engine_t *e, double *mean_nbr, long lon
 
 // ------------------------------------------------------------------ single-system API
 
+static int settle(ndg_t *h);
+static void frame_writer_stop(ndg_t *h);
+
 extern "C" const char *ndg_last_error(void) { return g_err.c_str(); }
 
 extern "C" int ndg_create(ndg_t **h, int np, int ntypes, int npairs, int ptsize, int neimax,
@@ -1547,6 +1658,7 @@ extern "C" int ndg_create(ndg_t **h, int np, int ntypes, int npairs, int ptsize,
 
 extern "C" int ndg_destroy(ndg_t *h) {
   if (!h) return NDG_OK;
+  frame_writer_stop(h);   // writes the frames still queued
   ndg_engine_destroy(h->e);
   delete h;
   return NDG_OK;
@@ -1629,6 +1741,8 @@ extern "C" int ndg_build_lists(ndg_t *h, double skin) {
 extern "C" int ndg_get_state(ndg_t *h, double *pos, double *vel, double *acc, double *force,
                              double *pene, double box[3]) {
   if (!h) return fail(NDG_ERR_ARG, "null handle");
+  int rc0 = settle(h);
+  if (rc0) return rc0;
   Slot &s = h->e->slots[h->slot];
   if (!s.block) {
     const HostState &hs = h->hs;
@@ -1805,21 +1919,14 @@ extern "C" int ndg_check_rebuild(ndg_t *h, double skin, int trigger, int *rebuil
   return NDG_OK;
 }
 
-extern "C" int ndg_run_begin(ndg_t *h, const ndg_run_params *rp, int step0) {
-  if (!h || !rp) return fail(NDG_ERR_ARG, "null argument");
-  if (!h->hs.have_tables || !h->hs.have_state)
-    return fail(NDG_ERR_ARG, "tables and state must be set first");
-  int rc = check_host(h->hs);
-  if (rc) return rc;
-  Slot &s = h->e->slots[h->slot];
-  return begin_run(h->e, h->slot, h->hs, *rp, step0, h->neimax_req, s.block == nullptr);
+// Drain the steps ndg_run_enqueue left on the stream (thermo records, errors, step count).
+static int settle(ndg_t *h) {
+  if (!h->queued) return NDG_OK;
+  h->queued = false;
+  return drain(h->e);
 }
 
-extern "C" int ndg_run_steps(ndg_t *h, int nsteps, ndg_thermo *rec, int max_rec, int *nrec) {
-  if (!h) return fail(NDG_ERR_ARG, "null handle");
-  ndg_engine *e = h->e;
-  int rc = run_engine(e, nsteps);
-  Slot &s = e->slots[h->slot];
+static void take_records(Slot &s, ndg_thermo *rec, int max_rec, int *nrec) {
   int n = 0;
   if (rec) {
     n = std::min((int)s.pending_rec.size(), max_rec);
@@ -1829,12 +1936,193 @@ extern "C" int ndg_run_steps(ndg_t *h, int nsteps, ndg_thermo *rec, int max_rec,
     s.pending_rec.clear();
   }
   if (nrec) *nrec = n;
+}
+
+extern "C" int ndg_run_begin(ndg_t *h, const ndg_run_params *rp, int step0) {
+  if (!h || !rp) return fail(NDG_ERR_ARG, "null argument");
+  if (!h->hs.have_tables || !h->hs.have_state)
+    return fail(NDG_ERR_ARG, "tables and state must be set first");
+  int rc = settle(h);
+  if (rc) return rc;
+  if ((rc = check_host(h->hs))) return rc;
+  Slot &s = h->e->slots[h->slot];
+  return begin_run(h->e, h->slot, h->hs, *rp, step0, h->neimax_req, s.block == nullptr);
+}
+
+extern "C" int ndg_run_steps(ndg_t *h, int nsteps, ndg_thermo *rec, int max_rec, int *nrec) {
+  if (!h) return fail(NDG_ERR_ARG, "null handle");
+  ndg_engine *e = h->e;
+  int rc = settle(h);
+  if (!rc) rc = run_engine(e, nsteps);
+  take_records(e->slots[h->slot], rec, max_rec, nrec);
   return rc;
+}
+
+extern "C" int ndg_run_enqueue(ndg_t *h, int nsteps) {
+  if (!h) return fail(NDG_ERR_ARG, "null handle");
+  int rc = settle(h);
+  if (rc) return rc;
+  rc = run_engine(h->e, nsteps, false);
+  h->queued = true;
+  return rc;
+}
+
+extern "C" int ndg_run_wait(ndg_t *h, ndg_thermo *rec, int max_rec, int *nrec) {
+  if (!h) return fail(NDG_ERR_ARG, "null handle");
+  int rc = settle(h);
+  take_records(h->e->slots[h->slot], rec, max_rec, nrec);
+  return rc;
+}
+
+static void frame_writer_loop(FrameWriter *w) {
+  std::string text;
+  const int nthreads = ndg::format_threads();
+  std::unique_lock<std::mutex> lk(w->m);
+  for (;;) {
+    w->cv.wait(lk, [&] { return w->stop || !w->q.empty(); });
+    if (w->q.empty()) return;
+    FrameWriter::Job &j = w->q.front();
+    FrameWriter::Buf &b = w->bufs[j.buf];
+    lk.unlock();
+    int rc = NDG_OK;
+    std::string msg;
+    const hipError_t he = hipEventSynchronize(b.ev);
+    if (he != hipSuccess) {
+      rc = NDG_ERR_HIP;
+      msg = std::string("frame copy: ") + hipGetErrorString(he);
+    } else {
+      const size_t np = (size_t)j.np;
+      ndg::FrameData f;
+      f.format = j.format;
+      f.np = j.np;
+      f.pos = b.h;
+      f.vel = b.h + 3 * np;
+      f.pene = b.h + 9 * np;
+      f.typ = j.typ.data();
+      f.names = j.names.data();
+      f.name_len = j.name_len;
+      for (int a = 0; a < 3; ++a) f.box[a] = b.h[10 * np + a];
+      f.step = (int)b.h[10 * np + 3];
+      ndg::format_frame(f, text, nthreads);
+      FILE *fp = fopen(j.path.c_str(), "ab");
+      if (!fp || fwrite(text.data(), 1, text.size(), fp) != text.size()) {
+        rc = NDG_ERR_ARG;
+        msg = "frame: cannot write " + j.path;
+      }
+      if (fp && fclose(fp) != 0 && !rc) {
+        rc = NDG_ERR_ARG;
+        msg = "frame: cannot write " + j.path;
+      }
+    }
+    lk.lock();
+    b.busy = false;
+    w->q.pop_front();
+    if (rc && !w->err) {
+      w->err = rc;
+      w->errmsg = msg;
+    }
+    w->cv.notify_all();
+  }
+}
+
+static int frame_writer_error(FrameWriter &w) {
+  if (!w.err) return NDG_OK;
+  const int rc = w.err;
+  const std::string msg = w.errmsg;
+  w.err = 0;
+  w.errmsg.clear();
+  return fail(rc, msg.c_str());
+}
+
+static void frame_writer_stop(ndg_t *h) {
+  if (!h->fw) return;
+  FrameWriter &w = *h->fw;
+  {
+    std::lock_guard<std::mutex> lk(w.m);
+    w.stop = true;
+  }
+  w.cv.notify_all();
+  if (w.th.joinable()) w.th.join();
+  for (auto &b : w.bufs) {
+    if (b.h) hipHostFree(b.h);
+    if (b.ev) hipEventDestroy(b.ev);
+  }
+  h->fw.reset();
+}
+
+extern "C" int ndg_frame_write(ndg_t *h, const char *path, int format, const char *names,
+                               int name_len) {
+  if (!h || !path || (format != NDG_FRAME_XYZ && format != NDG_FRAME_DUMP) ||
+      (format == NDG_FRAME_XYZ && (!names || name_len < 1)))
+    return fail(NDG_ERR_ARG, "ndg_frame_write: bad arguments");
+  ndg_engine *e = h->e;
+  Slot &s = e->slots[h->slot];
+  if (!s.block) return fail(NDG_ERR_ARG, "ndg_frame_write: the system has no device state yet");
+  if (!h->fw) {
+    h->fw = std::make_unique<FrameWriter>();
+    h->fw->th = std::thread(frame_writer_loop, h->fw.get());
+  }
+  FrameWriter &w = *h->fw;
+  std::unique_lock<std::mutex> lk(w.m);
+  int rc = frame_writer_error(w);
+  if (rc) return rc;
+  // at most two frames wait: a third waits for the oldest to be written
+  w.cv.wait(lk, [&] { return !w.bufs[0].busy || !w.bufs[1].busy; });
+  const int k = !w.bufs[0].busy ? 0 : 1;
+  FrameWriter::Buf &b = w.bufs[k];
+  lk.unlock();   // a free buffer is touched by this thread only
+  const size_t n = snapshot_doubles(s.np);
+  if (n > b.cap) {
+    if (b.h) HIPCHK(hipHostFree(b.h));
+    b.h = nullptr;
+    b.cap = 0;
+    HIPCHK(hipHostMalloc((void **)&b.h, 8 * n, hipHostMallocDefault));
+    b.cap = n;
+  }
+  if (!b.ev) HIPCHK(hipEventCreateWithFlags(&b.ev, hipEventDisableTiming));
+  if ((rc = snapshot_to(e, h->slot, b.h, b.ev))) return rc;
+  FrameWriter::Job j;
+  j.buf = k;
+  j.format = format;
+  j.np = s.np;
+  j.path = path;
+  j.typ = h->hs.typ;
+  if (format == NDG_FRAME_XYZ) {
+    j.name_len = name_len;
+    j.names.assign(names, names + (size_t)name_len * h->hs.ntypes);
+  }
+  lk.lock();
+  b.busy = true;
+  w.q.push_back(std::move(j));
+  w.cv.notify_all();
+  return NDG_OK;
+}
+
+extern "C" int ndg_frame_flush(ndg_t *h) {
+  if (!h) return fail(NDG_ERR_ARG, "null handle");
+  if (!h->fw) return NDG_OK;
+  FrameWriter &w = *h->fw;
+  std::unique_lock<std::mutex> lk(w.m);
+  w.cv.wait(lk, [&] { return w.q.empty(); });
+  return frame_writer_error(w);
+}
+
+extern "C" int ndg_snapshot(ndg_t *h) {
+  if (!h) return fail(NDG_ERR_ARG, "null handle");
+  return snapshot_enqueue(h->e, h->slot);
+}
+
+extern "C" int ndg_snapshot_get(ndg_t *h, double *pos, double *vel, double *force, double *pene,
+                                double box[3], int *step) {
+  if (!h) return fail(NDG_ERR_ARG, "null handle");
+  return snapshot_copy(h->e, h->slot, pos, vel, force, pene, box, step);
 }
 
 extern "C" int ndg_run_end(ndg_t *h, int *nupdates, int *step) {
   if (!h) return fail(NDG_ERR_ARG, "null handle");
   ndg_engine *e = h->e;
+  int rc0 = settle(h);
+  if (rc0) return rc0;
   launch_apply_pending(dev_view(e), h->slot, 1, e->stream);
   HIPCHK(hipGetLastError());
   RepState S;

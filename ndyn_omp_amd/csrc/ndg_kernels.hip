@@ -954,6 +954,47 @@ __global__ __launch_bounds__(256) void k_apply_pending(EngineDev E, int r0, int 
   }
 }
 
+// Caller-order copy of one replica's output state, for a single copy to the host:
+// pos(3,np) scaled, vel(3,np), force(3,np), per-atom energy(np), then box(3) and the step.
+// Velocities are as the last step left them (a pending thermostat scale is applied by the
+// next drift, as the reference writes its frame before mkIsoKE, src/nd.f90:172-176).
+__global__ __launch_bounds__(256) void k_snapshot(EngineDev E, int r, double *out) {
+  const RepParams &R = E.reps[r];
+  const RepState &S = E.st[r];
+  const int np = R.np;
+  const bool mixed = R.path == NDG_PATH_MIXED;
+  // latest forces: the buffer a force evaluation wrote if no kick has consumed it yet
+  const float4 *fl = (S.parity ^ S.fnew) ? R.f32b : R.f32a;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < np; i += gridDim.x * blockDim.x) {
+    const int o = R.oid[i];
+    double *p = out + 3 * (size_t)o;
+    p[0] = R.px[i];
+    p[1] = R.py[i];
+    p[2] = R.pz[i];
+    double *v = out + 3 * ((size_t)np + o);
+    v[0] = R.vx[i];
+    v[1] = R.vy[i];
+    v[2] = R.vz[i];
+    double *f = out + 3 * (2 * (size_t)np + o);
+    if (mixed) {
+      const float4 q = fl[i];
+      f[0] = q.x;
+      f[1] = q.y;
+      f[2] = q.z;
+      out[9 * (size_t)np + o] = q.w;
+    } else {
+      f[0] = R.fx[i];
+      f[1] = R.fy[i];
+      f[2] = R.fz[i];
+      out[9 * (size_t)np + o] = R.pene[i];
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 4) {
+    const int a = threadIdx.x;
+    out[10 * (size_t)np + a] = a < 3 ? S.box[a] : (double)S.step;
+  }
+}
+
 // ------------------------------------------------------------------ list rebuild
 // Counting sort of atoms by reference cell (canonical order: cell, then caller index),
 // then a full Verlet list from the 27 surrounding cells in a fixed stencil order.
@@ -1810,6 +1851,12 @@ void launch_reduce(const EngineDev &E, int r0, int nr, int mode, int xflags, int
 void launch_apply_pending(const EngineDev &E, int r0, int nr, hipStream_t s) {
   if (nr <= 0) return;
   hipLaunchKernelGGL(k_apply_pending, dim3(nr < 1024 ? nr : 1024), dim3(256), 0, s, E, r0, nr);
+}
+
+void launch_snapshot(const EngineDev &E, int r, int np, double *out, hipStream_t s) {
+  int g = (np + 255) / 256;
+  if (g > 2048) g = 2048;
+  hipLaunchKernelGGL(k_snapshot, dim3(g), dim3(256), 0, s, E, r, out);
 }
 
 void launch_rebuild(const EngineDev &E, int nt, int r0, int nr, hipStream_t s) {

@@ -88,6 +88,10 @@ def load_library():
         "ndg_run_begin": [V, P(RunParams), C.c_int],
         "ndg_run_steps": [V, C.c_int, P(Thermo), C.c_int, P(C.c_int)],
         "ndg_run_end": [V, P(C.c_int), P(C.c_int)],
+        "ndg_run_enqueue": [V, C.c_int],
+        "ndg_run_wait": [V, P(Thermo), C.c_int, P(C.c_int)],
+        "ndg_snapshot": [V],
+        "ndg_snapshot_get": [V, V, V, V, V, V, P(C.c_int)],
         "ndg_destroy": [V],
         "ndg_engine_create": [P(V), C.c_int, C.c_size_t],
         "ndg_engine_destroy": [V],
@@ -256,6 +260,17 @@ class Simulation:
     def set_box(self, box):
         _check(load_library().ndg_set_box(self.h, np.ascontiguousarray(box, dtype=np.float64)))
 
+    def snapshot(self):
+        """ndg_snapshot + ndg_snapshot_get: the output state in caller order."""
+        L = load_library()
+        n = self.sys.np
+        pos, vel, force = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+        pene, box, st = np.zeros(n), np.zeros(3), C.c_int(0)
+        _check(L.ndg_snapshot(self.h))
+        _check(L.ndg_snapshot_get(self.h, _ptr(pos), _ptr(vel), _ptr(force), _ptr(pene),
+                                  _ptr(box), C.byref(st)))
+        return dict(step=st.value, pos=pos, vel=vel, force=force, pene=pene, box=box)
+
     # --- the dynamics loop ----------------------------------------------------------------
     def run(self, rp: RunParams, step0=0, chunk=None):
         L = load_library()
@@ -274,6 +289,41 @@ class Simulation:
         nup, step = C.c_int(0), C.c_int(0)
         _check(L.ndg_run_end(self.h, C.byref(nup), C.byref(step)))
         return recs, nup.value, step.value
+
+    def run_frames(self, rp: RunParams, frame_every, step0=0):
+        """The Fortran driver's output loop: chunks that end at multiples of frame_every,
+        each frame copied on the device behind its chunk (ndg_snapshot) and read after the
+        next chunk is queued (ndg_run_enqueue / ndg_run_wait / ndg_snapshot_get)."""
+        L = load_library()
+        _check(L.ndg_run_begin(self.h, C.byref(rp), int(step0)))
+        recs, frames = [], []
+        cap = rp.nsteps // max(1, rp.out_steps) + 2
+        buf = (Thermo * cap)()
+        left, cur = rp.nsteps, step0
+        chunk = min(left, frame_every - cur % frame_every)
+        _check(L.ndg_run_enqueue(self.h, chunk))
+        while left > 0:
+            frame = (cur + chunk) % frame_every == 0
+            if frame:
+                _check(L.ndg_snapshot(self.h))
+            got = C.c_int(0)
+            _check(L.ndg_run_wait(self.h, buf, cap, C.byref(got)))
+            recs.extend(buf[k].as_dict() for k in range(got.value))
+            cur += chunk
+            left -= chunk
+            if left > 0:
+                chunk = min(left, frame_every - cur % frame_every)
+                _check(L.ndg_run_enqueue(self.h, chunk))
+            if frame:
+                n = self.sys.np
+                pos, vel, force = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+                pene, box, st = np.zeros(n), np.zeros(3), C.c_int(0)
+                _check(L.ndg_snapshot_get(self.h, _ptr(pos), _ptr(vel), _ptr(force), _ptr(pene),
+                                          _ptr(box), C.byref(st)))
+                frames.append(dict(step=st.value, pos=pos, vel=vel, force=force, pene=pene, box=box))
+        nup, step = C.c_int(0), C.c_int(0)
+        _check(L.ndg_run_end(self.h, C.byref(nup), C.byref(step)))
+        return recs, frames, nup.value, step.value
 
 
 class BatchEngine:

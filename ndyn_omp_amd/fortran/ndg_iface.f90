@@ -13,6 +13,7 @@ module ndg_iface
   integer(c_int), parameter :: NDG_PATH_FP64 = 0, NDG_PATH_MIXED = 1
   integer(c_int), parameter :: NDG_TRIGGER_COMPAT = 0, NDG_TRIGGER_SKIN = 1
   integer(c_int), parameter :: NDG_MORSE = 1, NDG_BUCK = 2, NDG_LJ = 3, NDG_MORSE_PHYS = 4
+  integer(c_int), parameter :: NDG_FRAME_XYZ = 1, NDG_FRAME_DUMP = 2
 
   type, bind(C) :: ndg_run_params
     integer(c_int) :: nsteps
@@ -139,6 +140,58 @@ module ndg_iface
       integer(c_int), value :: nsteps, max_rec
       type(ndg_thermo), intent(out) :: rec(*)
       integer(c_int), intent(out) :: nrec
+    end function
+
+    integer(c_int) function ndg_run_enqueue(h, nsteps) bind(C, name='ndg_run_enqueue')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: nsteps
+    end function
+
+    integer(c_int) function ndg_run_wait(h, rec, max_rec, nrec) bind(C, name='ndg_run_wait')
+      import :: c_int, c_ptr, ndg_thermo
+      type(c_ptr), value :: h
+      integer(c_int), value :: max_rec
+      type(ndg_thermo), intent(out) :: rec(*)
+      integer(c_int), intent(out) :: nrec
+    end function
+
+    integer(c_int) function ndg_snapshot(h) bind(C, name='ndg_snapshot')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+
+    integer(c_int) function ndg_snapshot_get(h, pos, vel, force, pene, box, step) &
+        bind(C, name='ndg_snapshot_get')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: pos(*), vel(*), force(*), pene(*), box(3)
+      integer(c_int), intent(out) :: step
+    end function
+
+    integer(c_int) function ndg_frame_write(h, path, format, names, name_len) &
+        bind(C, name='ndg_frame_write')
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: h
+      character(kind=c_char), intent(in) :: path(*), names(*)
+      integer(c_int), value :: format, name_len
+    end function
+
+    integer(c_int) function ndg_frame_flush(h) bind(C, name='ndg_frame_flush')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+
+    integer(c_int) function ndg_format_frame(format, np, pos, vel, pene, typ, names, name_len, &
+        box, step, buf, cap, len) bind(C, name='ndg_format_frame')
+      import :: c_int, c_double, c_char, c_size_t
+      integer(c_int), value :: format, np, name_len, step
+      real(c_double), intent(in) :: pos(*), vel(*), pene(*), box(3)
+      integer(c_int), intent(in) :: typ(*)
+      character(kind=c_char), intent(in) :: names(*)
+      character(kind=c_char), intent(out) :: buf(*)
+      integer(c_size_t), value :: cap
+      integer(c_size_t), intent(out) :: len
     end function
 
     integer(c_int) function ndg_run_end(h, nupdates, step) bind(C, name='ndg_run_end')

@@ -8,8 +8,10 @@
 ! thermo lines (src/read_input.f90:433-446) and trajectory files out.xyz / out.dump
 ! (src/read_input.f90:394-431).  The MD step runs on the GPU ('dynamics',
 ! src/nd.f90:60-199, through ndg_run_begin / ndg_run_steps / ndg_run_end).
-! Two directives are added: 'Precision mixed|fp64' selects the engine path and
-! 'ListTrigger compat|skin' the neighbour-list rebuild criterion.
+! Three directives are added: 'Precision mixed|fp64' selects the engine path,
+! 'ListTrigger compat|skin' the neighbour-list rebuild criterion, and
+! 'FrameWriter engine|fortran' who formats the trajectory frames: the library's writer
+! thread (default; same bytes, formatted while the device keeps stepping) or write_model.
 ! Asymmetric pairs ('asym', the reference's compute1) run on the GPU as well.
 ! Group/Hold/Disp/ChangeType/InitPos and spring pairs (compute2-3) are not part of this
 ! engine; they are reported and ignored.
@@ -35,6 +37,8 @@ program ndyn_gpu
   character(len=10) :: outtype = 'NA'
   logical :: potouts = .false.
   integer :: path = NDG_PATH_MIXED, trigger = NDG_TRIGGER_COMPAT
+  ! frames formatted by the library's writer thread (engine) or by write_model (fortran)
+  logical :: engine_frames = .true.
   ! engine handle
   type(c_ptr) :: h = c_null_ptr
   logical :: tables_dirty = .true., state_dirty = .true., have_model = .false.
@@ -156,6 +160,8 @@ program ndyn_gpu
       else
         path = NDG_PATH_MIXED
       endif
+    case ('FrameWriter')
+      engine_frames = trim(w(2)) /= 'fortran'
     case ('ListTrigger')
       if (trim(w(2)) == 'skin') then
         trigger = NDG_TRIGGER_SKIN
@@ -450,9 +456,12 @@ contains
   subroutine dynamics
     type(ndg_run_params) :: rp
     type(ndg_thermo), allocatable :: rec(:)
-    integer(c_int) :: nrec, nup, stepdev
+    integer(c_int) :: nrec, nup, stepdev, stepsnap, fmt
     integer :: u, left, chunk, k, cur, mcell(3), ne
-    real(8) :: rcut, sk, t0, t1
+    integer(8) :: c0, c1, crate
+    logical :: frame
+    real(8) :: rcut, sk
+    character(len=:), allocatable :: fname, names
     if (.not. have_model) call random_model
     if (npairs == 0) call die('ERROR: no Pairs defined')
     if (.not. c_associated(h)) then
@@ -501,16 +510,46 @@ contains
     rp%isobaric = merge(1, 0, isobar); rp%p_target = ptarget; rp%p_tau = pparm; rp%p_freq = pfreq
     rp%isoenergetic = merge(1, 0, isoen); rp%e_tau = eparm; rp%e_freq = efreq
     rp%box_load = merge(1, 0, boxload); rp%box_vel = boxvel
-    call cpu_time(t0)
-    open(newunit=u, file='out.'//trim(outtype), position='append')
-    call write_model(u)
+    call system_clock(c0, crate)
+    select case (trim(outtype))
+    case ('XYZ', 'xyz')
+      fmt = NDG_FRAME_XYZ
+    case ('dump', 'trj')
+      fmt = NDG_FRAME_DUMP
+    case default
+      call die('ERROR: model out type: '//trim(outtype)//' not supported')
+    end select
+    fname = 'out.'//trim(outtype)
+    names = ''
+    do k = 1, ntypes
+      names = names//typename(k)
+    enddo
+    if (.not. engine_frames) open(newunit=u, file=fname, position='append')
     call check(ndg_run_begin(h, rp, int(step, c_int)))
+    ! the frame before the time loop (src/nd.f90:126-127)
+    if (engine_frames) then
+      call check(ndg_frame_write(h, fname//c_null_char, fmt, names, int(len(typename), c_int)))
+    else
+      call check(ndg_snapshot(h))
+      call check(ndg_snapshot_get(h, pos, vel, frc, pene, box, stepsnap))
+      call write_model(u)
+    endif
     left = step_num
     cur = step
+    ! Steps run in chunks that end at the ConfOutFreq frames.  A frame is copied on the
+    ! device behind its chunk, and the next chunk is queued before the frame is formatted,
+    ! so the device keeps stepping while the host writes (src/nd.f90:169-172).
+    chunk = min(left, confouts - mod(cur, confouts))
+    call check(ndg_run_enqueue(h, int(chunk, c_int)))
     do while (left > 0)
-      chunk = min(left, confouts - mod(cur, confouts))
+      frame = mod(cur + chunk, confouts) == 0
+      if (frame .and. engine_frames) then
+        call check(ndg_frame_write(h, fname//c_null_char, fmt, names, int(len(typename), c_int)))
+      else if (frame) then
+        call check(ndg_snapshot(h))
+      endif
       allocate(rec(chunk / max(1, outsteps) + 2))
-      call check(ndg_run_steps(h, int(chunk, c_int), rec, int(size(rec), c_int), nrec))
+      call check(ndg_run_wait(h, rec, int(size(rec), c_int), nrec))
       do k = 1, nrec
         write(6, '(x,i8,x,g14.6,x,g14.6,x,g14.6,x,g14.6,x,g14.6)') rec(k)%step, rec(k)%pe, &
           rec(k)%ke, rec(k)%eerr, rec(k)%pressure, rec(k)%volume
@@ -520,20 +559,28 @@ contains
       cur = cur + chunk
       left = left - chunk
       step = cur
-      if (mod(cur, confouts) == 0) then
-        call pull_state
+      if (left > 0) then
+        chunk = min(left, confouts - mod(cur, confouts))
+        call check(ndg_run_enqueue(h, int(chunk, c_int)))
+      endif
+      if (frame .and. .not. engine_frames) then
+        call check(ndg_snapshot_get(h, pos, vel, frc, pene, box, stepsnap))
         call write_model(u)
       endif
     enddo
     call check(ndg_run_end(h, nup, stepdev))
     step = stepdev
     call pull_state
-    close(u)
-    call cpu_time(t1)
+    if (engine_frames) then
+      call check(ndg_frame_flush(h))
+    else
+      close(u)
+    endif
+    call system_clock(c1)
     write(6, '(a,i8)') '# Number of neighbour list updates:', nup
     write(6, '(a)') ''
     write(6, '(a)') '#  Elapsed time for main dynamics:'
-    write(6, '(a,g14.6,a)') '#  ', t1 - t0, ' seconds'
+    write(6, '(a,g14.6,a)') '#  ', dble(c1 - c0) / dble(crate), ' seconds'
     call flush(6)
   end subroutine dynamics
 
@@ -542,7 +589,8 @@ contains
     write(6,'(A)') ' Skin NeiMax Types Pairs WritePotentialTable Isokinetic Anisokinetic'
     write(6,'(A)') ' Isoenergetic Anisoenergetic Isobaric Anisobaric BoxVel StopBoxVel Exec Print'
     write(6,'(A)') ' OutFreq StepNumber ConfOutFreq Run END; engine: Precision mixed|fp64,'
-    write(6,'(A)') ' ListTrigger compat|skin.  Pair kinds: morse morse_phys buck LJ (sym only).'
+    write(6,'(A)') ' ListTrigger compat|skin, FrameWriter engine|fortran.'
+    write(6,'(A)') ' Pair kinds: morse morse_phys buck LJ (sym or asym).'
     call flush(6)
   end subroutine help
 

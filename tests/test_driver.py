@@ -172,7 +172,8 @@ def test_driver_asym_pairs_deck(tmp_path, oracle, precision):
     """'asym' pair lines (src/read_input.f90:127-133) select compute1: a two-type deck with
     one directed pair (B atoms feel A atoms, A atoms do not feel B) and its thermo lines
     against the oracle's run of the same configuration: PE and volume within 1e-5 relative
-    (the lines print 6 significant digits)."""
+    (the lines print 6 significant digits); the dump frames (ConfOutFreq) at their steps, the
+    last one equal to the oracle's final positions to the printed 5 decimals."""
     s = W.asym_morse(6, variant=1, jitter_a=0.05, seed=131)
     t = write_typed_dump(tmp_path / "model.dump", s)
     t.ptsize = 1000   # the deck's default table size (src/read_input.f90:28)
@@ -189,6 +190,7 @@ Pairs 3
  B B sym morse {C}
 Precision {precision}
 OutFreq 20
+ConfOutFreq 20 dump
 Run 60 {W.DT_1FS!r}
 END
 """
@@ -196,9 +198,63 @@ END
     p = run_deck(deck, tmp_path)
     assert p.returncode == 0, p.stderr
     rows = thermo(p.stdout)
-    _, ref, err = oracle.run(t, 60, W.DT_1FS, skin=0.5, out_steps=20)
+    st, ref, err = oracle.run(t, 60, W.DT_1FS, skin=0.5, out_steps=20)
     assert err == 0
     assert [int(r[0]) for r in rows] == [r["step"] for r in ref] == [1, 20, 40, 60]
     for r, q in zip(rows, ref):
         assert r[1] == pytest.approx(q["pe"], rel=1e-5)
         assert r[5] == pytest.approx(q["volume"], rel=1e-5)
+    # frames at steps 0, 20, 40, 60 (copied on the device behind each chunk); the last one
+    # holds the oracle's final positions to the format's 5 decimals
+    lines = (tmp_path / "out.dump").read_text().splitlines()
+    heads = [k for k, l in enumerate(lines) if l.startswith("ITEM: TIMESTEP")]
+    assert [int(lines[k + 1]) for k in heads] == [0, 20, 40, 60]
+    last = np.array([[float(x) for x in l.split()[2:5]] for l in lines[heads[-1] + 9:]])
+    assert last.shape == st["pos"].shape
+    assert np.abs(last - st["pos"]).max() < 1e-5
+
+
+@pytest.mark.gpu
+@needs_driver
+@pytest.mark.parametrize("fmt", ["xyz", "dump"])
+def test_driver_engine_frames_equal_write_model(tmp_path, fmt):
+    """'FrameWriter engine' (the library's writer thread, formatting while the device steps)
+    writes the same bytes as 'FrameWriter fortran' (write_model's edit descriptors) over two
+    Runs with frames every 10 steps, the second Run appending to the same file."""
+    s = W.asym_morse(6, variant=2, temp_k=600.0, seed=151)
+    t = write_typed_dump(tmp_path / "model.dump", s)
+    prm = [" ".join(repr(float(v)) for v in s.params[k]) for k in range(4)]
+    out = {}
+    for writer in ("fortran", "engine"):
+        d = tmp_path / writer
+        d.mkdir()
+        (d / "model.dump").write_bytes((tmp_path / "model.dump").read_bytes())
+        deck = f"""START
+Model 	model.dump dump
+Seed 1234567890
+Skin 0.5
+Types 2
+ Cu 63.5
+ Al-x 27.0
+Pairs 4
+ Cu Cu sym morse {prm[0]}
+ Cu Al-x asym morse {prm[1]}
+ Al-x Cu asym morse {prm[2]}
+ Al-x Al-x sym morse {prm[3]}
+Velocity 0.05
+OutFreq 10
+ConfOutFreq 10 {fmt}
+FrameWriter {writer}
+Run 30 {W.DT_1FS!r}
+Run 25 {W.DT_1FS!r}
+END
+"""
+        print(f"case frames {fmt} {writer}", flush=True)
+        p = run_deck(deck, d)
+        assert p.returncode == 0, p.stderr
+        out[writer] = (d / f"out.{fmt}").read_bytes()
+    assert t.np == s.np
+    frames = out["fortran"].count(b"ITEM: TIMESTEP") if fmt == "dump" else \
+        sum(1 for l in out["fortran"].splitlines() if l.strip() == str(s.np).encode())
+    assert frames == 4 + 3      # Run 1: steps 0, 10, 20, 30; Run 2: 30, 40, 50
+    assert out["engine"] == out["fortran"]
