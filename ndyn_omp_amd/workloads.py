@@ -20,9 +20,11 @@ DT_1FS = 0.0982269            # 1 fs in internal units (deck value)
 DT_2FS = 0.196454             # 2 fs
 
 # pair kinds (same ids as the C ABI / oracle)
-MORSE, BUCK, LJ, MORSE_PHYS = 1, 2, 3, 4
+MORSE, BUCK, LJ, MORSE_PHYS, SPRING = 1, 2, 3, 4, 5
 
 CU_MORSE = (7.0, 0.3429, 1.3588, 2.866)        # rc, D, alpha, r0
+# peridynamic bond of examples/testPDspringTension/input.sh: rc, C, max strain s0, Vi
+PD_SPRING = (6.0, 2.152295e-3, 0.1, 12.459008)
 AR_LJ = (8.5125, 0.0104, 3.405, 0.0)           # rc, eps, sigma
 ALLOY_BUCK = (8.0, 1000.0, 0.3, 10.0)          # rc, A, rho, C (code order, src/potentials.f90:115-117)
 
@@ -73,6 +75,9 @@ class System:
     ptrmin: float = 0.25
     ptsize: int = 5000
     name: str = ""
+    # initial positions (scaled, (np,3)) and box of initPositionSet, for 'spring' pairs
+    ipos: Optional[np.ndarray] = None
+    init_box: Optional[np.ndarray] = None
 
     @property
     def np(self) -> int:
@@ -86,9 +91,15 @@ class System:
     def npairs(self) -> int:
         return int(self.params.shape[0])
 
+    @property
+    def springs(self) -> bool:
+        return bool(np.any(np.asarray(self.kinds) == SPRING))
+
     def copy(self) -> "System":
         return dataclasses.replace(self, pos=self.pos.copy(), vel=self.vel.copy(),
-                                   typ=self.typ.copy(), box=self.box.copy())
+                                   typ=self.typ.copy(), box=self.box.copy(),
+                                   ipos=None if self.ipos is None else self.ipos.copy(),
+                                   init_box=None if self.init_box is None else self.init_box.copy())
 
 
 def fcc(nx: int, ny: int, nz: int, a: float, fp32_like_dump: bool = False):
@@ -204,6 +215,57 @@ def asym_morse(nx: int, variant: int = 1, a: float = 3.615, temp_k: float = 300.
     n = len(prm)
     return System(pos, vel, typ, box, masses, pairmat, np.array([MORSE] * n, dtype=np.int32),
                   np.array(prm, dtype=np.float64), 0.25, ptsize, f"asym_morse{variant}_{nx}")
+
+
+def spring_cu(nx: int, a: float = 3.68, stretch=(0.0, 0.0, 0.0), jitter_a: float = 0.0,
+              temp_k: float = 0.0, seed: int = 12360, ptsize: int = 1000) -> System:
+    """A peridynamic FCC block: one 'spring' pair (PD_SPRING, as in
+    examples/testPDspringTension/input.sh), initial positions = the perfect lattice in its
+    own box, current state = the lattice stretched by `stretch` (relative, per axis) and
+    jittered by jitter_a (A).  Bonds whose strain exceeds PairParams(3) carry no force."""
+    ipos, box0 = fcc(nx, nx, nx, a)
+    box = box0 * (1.0 + np.asarray(stretch, dtype=np.float64))
+    pos = jitter(ipos, box, jitter_a, seed) if jitter_a > 0 else ipos.copy()
+    typ = np.ones(pos.shape[0], dtype=np.int32)
+    masses = np.array([63.546])
+    vel = thermal_velocities(typ, masses, temp_k, seed)
+    return System(pos, vel, typ, box, masses, np.array([[1]], dtype=np.int32),
+                  np.array([SPRING], dtype=np.int32), np.array([PD_SPRING], dtype=np.float64),
+                  0.25, ptsize, f"spring_cu_{nx}", ipos=ipos, init_box=box0.copy())
+
+
+def spring_mix(nx: int, variant: int = 0, a: float = 3.615, stretch=(0.02, 0.0, -0.01),
+               jitter_a: float = 0.1, temp_k: float = 0.0, seed: int = 12361,
+               ptsize: int = 2000) -> System:
+    """Two species (50/50, 63.5 / 27 amu) on an FCC lattice, Morse within each species and
+    springs between them, against the unstretched lattice as initial positions:
+      variant 0: PairMat = [[1, 2], [2, 3]] -- symmetric, A-B is a spring (compute2);
+      variant 1: PairMat = [[1, 2], [0, 3]] -- A-B spring one way only: B atoms feel A atoms
+                 through it, A atoms do not feel B (compute3, as the GP deck's
+                 'Cu1 Cu-2 asym spring', examples/testGP/GP/input.sh:27);
+      variant 2: PairMat = [[1, 2], [3, 4]] -- springs both ways with different ids, the
+                 pair uses spring 3 (the larger id) both ways (compute3)."""
+    ipos, box0 = fcc(nx, nx, nx, a)
+    box = box0 * (1.0 + np.asarray(stretch, dtype=np.float64))
+    pos = jitter(ipos, box, jitter_a, seed) if jitter_a > 0 else ipos.copy()
+    u = uniform(seed, 13, pos.shape[0])
+    typ = np.where(u < 0.5, 1, 2).astype(np.int32)
+    masses = np.array([63.5, 27.0])
+    vel = thermal_velocities(typ, masses, temp_k, seed)
+    mA, mB = CU_MORSE, (6.8, 0.25, 1.30, 2.90)
+    sp1, sp2 = (6.0, 2.152295e-3, 0.06, 12.459008), (6.5, 3.0e-3, 0.08, 10.0)
+    if variant == 0:
+        pm = [[1, 2], [2, 3]]
+        kinds, prm = [MORSE, SPRING, MORSE], [mA, sp1, mB]
+    elif variant == 1:
+        pm = [[1, 2], [0, 3]]
+        kinds, prm = [MORSE, SPRING, MORSE], [mA, sp1, mB]
+    else:
+        pm = [[1, 2], [3, 4]]
+        kinds, prm = [MORSE, SPRING, SPRING, MORSE], [mA, sp1, sp2, mB]
+    return System(pos, vel, typ, box, masses, np.array(pm, dtype=np.int32),
+                  np.array(kinds, dtype=np.int32), np.array(prm, dtype=np.float64), 0.25,
+                  ptsize, f"spring_mix{variant}_{nx}", ipos=ipos, init_box=box0.copy())
 
 
 def near_cubic_cells(n_atoms: float) -> tuple:

@@ -32,8 +32,14 @@
 
 /* ---------------------------------------------------------------- tables */
 
-/* pot()  src/potentials.f90:59-152 (Morse :94-110, Buck :112-122, LJ :124-133) */
+/* pot()  src/potentials.f90:59-152 (Morse :94-110, Buck :112-122, LJ :124-133,
+ * spring :88-92) */
 static double pot_fn(int kind, double r, const double *p) {
+  if (kind == NDO_SPRING) {
+    double rc = p[0], D = p[1], C = p[2];
+    double r0 = r / rc * C;
+    return D * (r0 * r0 - C * C);
+  }
   if (kind == NDO_MORSE || kind == NDO_MORSE_PHYS) {
     double D = p[1], alfa = p[2], r0 = p[3];
     double tmp = r - r0;
@@ -61,6 +67,11 @@ static double pot_fn(int kind, double r, const double *p) {
  * Buck :206-216, LJ :218-227).  Returns -dV/dr except for the reference Morse,
  * which returns -dV/dr / r. */
 static double dpot_fn(int kind, double r, const double *p) {
+  if (kind == NDO_SPRING) {   /* src/potentials.f90:178-183 */
+    double rc = p[0], D = p[1], C = p[2];
+    double r0 = r / rc * C;
+    return -2.0 * D * r0;
+  }
   if (kind == NDO_MORSE || kind == NDO_MORSE_PHYS) {
     double D = p[1], alfa = p[2], r0 = p[3];
     double tmp = r - r0;
@@ -88,7 +99,7 @@ static double dpot_fn(int kind, double r, const double *p) {
 /* buildPairTables  src/potentials.f90:2-56 */
 int ndo_pair_table(int kind, const double params[4], double ptrmin, int ptsize,
                    double *P, double *F, double *rc2, double *dr2) {
-  if (kind < NDO_MORSE || kind > NDO_MORSE_PHYS || ptsize < 2) return NDO_ERR_ARG;
+  if (kind < NDO_MORSE || kind > NDO_SPRING || ptsize < 2) return NDO_ERR_ARG;
   double rcut = params[0];
   double c2 = rcut * rcut;
   double d2 = (c2 - ptrmin) / (double)(ptsize - 1);
@@ -398,6 +409,103 @@ void ndo_compute1(int np, const double *pos, const int *typ, const double box[3]
   free(vdr2);
 }
 
+/* compute2 / compute3 (src/nd.f90:531-933), serial.  The pair id is PairMat(ti,tj) in
+ * compute2 (:610) and the larger of both directions in compute3 (:810-812).  A spring pair
+ * (:622-660, :827-862): strain against the initial positions, cut at PairParams(3) (no
+ * distance cutoff); other pairs: the table as compute0/compute1 (:661-677, :863-882, the
+ * index clamped at 1 as in compute0).  Force gating and energy/virial accounting for
+ * asymmetric matrices as compute1 (:884-903). */
+void ndo_compute23(int asym, int np, const double *pos, const int *typ, const double box[3],
+                   const int *vnl, int neimax, int ntypes, const int *pairmat,
+                   const double *P, const double *F, const double *rc2, const double *dr2,
+                   double ptrmin, int ptsize, const int *spring, const double *pairparams,
+                   const double *ipos, const double init_box[3], int cv, int cu,
+                   double *force, double *pe, double virial[6], double *pene) {
+  const int stride = neimax + 1;
+  const int tab = ptsize + 1;
+  double vir[6] = {0, 0, 0, 0, 0, 0};
+  double pot = 0.0;
+  int npairs_max = ntypes * ntypes;
+  float *vdr2 = (float *)malloc(sizeof(float) * (npairs_max + 1));
+  for (int p = 0; p < npairs_max + 1; ++p) vdr2[p] = 0.0f;
+  for (int a = 0; a < npairs_max; ++a) {
+    int p = pairmat[a];
+    if (p > 0) vdr2[p] = (float)(1.0 / dr2[p - 1]);
+  }
+  memset(force, 0, sizeof(double) * 3 * (size_t)np);
+  if (cu && pene) memset(pene, 0, sizeof(double) * (size_t)np);
+  for (int i = 0; i < np; ++i) {
+    const double *pi = pos + 3 * i;
+    int ti = typ[i] - 1;
+    int n = vnl[stride * i];
+    for (int jn = 0; jn < n; ++jn) {
+      int j = vnl[stride * i + 1 + jn];
+      int tj = typ[j] - 1;
+      int pij = pairmat[ti + tj * ntypes], pji = pairmat[tj + ti * ntypes];
+      int p = asym ? (pij > pji ? pij : pji) : pij;
+      const int bal = !asym || pij == pji;
+      double rij[3];
+      double d2 = pair_d2(pi, pos + 3 * j, box, rij);
+      double df, dp = 0.0;
+      if (spring[p - 1]) {
+        const double *prm = pairparams + 4 * (p - 1);
+        double rij0[3];
+        double d20 = pair_d2(ipos + 3 * i, ipos + 3 * j, init_box, rij0);
+        double r = sqrt(d2);
+        double r0 = sqrt(d20);
+        double sij = r / r0 - 1.0;
+        if (sij > prm[2]) continue;
+        double Vi = prm[3];
+        df = prm[1] * sij * Vi * Vi;
+        if (cu && bal) dp = 0.5 * df * sij * r0;
+        df = -df / r;
+      } else {
+        if (d2 > rc2[p - 1]) continue;
+        float rk = (float)(d2 - ptrmin) * vdr2[p] + 1.0f;
+        int k = (int)rk;
+        if (k < 1) k = 1;
+        double w = (double)rk - (double)k;
+        const double *Ft = F + (size_t)tab * (p - 1);
+        df = w * Ft[k] + (1.0 - w) * Ft[k - 1];
+        if (cu && bal) {
+          const double *Pt = P + (size_t)tab * (p - 1);
+          dp = w * Pt[k] + (1.0 - w) * Pt[k - 1];
+        }
+        df = df / d2;
+      }
+      if (!asym || pji > 0) {
+        force[3 * i + 0] = force[3 * i + 0] + rij[0] * df;
+        force[3 * i + 1] = force[3 * i + 1] + rij[1] * df;
+        force[3 * i + 2] = force[3 * i + 2] + rij[2] * df;
+      }
+      if (!asym || pij > 0) {
+        force[3 * j + 0] = force[3 * j + 0] - rij[0] * df;
+        force[3 * j + 1] = force[3 * j + 1] - rij[1] * df;
+        force[3 * j + 2] = force[3 * j + 2] - rij[2] * df;
+      }
+      if (cu && bal) {
+        pot = pot + dp;
+        if (pene) {
+          pene[i] += 0.5 * dp;
+          pene[j] += 0.5 * dp;
+        }
+      }
+      if (cv && bal) {
+        vir[0] = vir[0] + rij[0] * rij[0] * df;
+        vir[1] = vir[1] + rij[1] * rij[1] * df;
+        vir[2] = vir[2] + rij[2] * rij[2] * df;
+        vir[3] = vir[3] + rij[0] * rij[1] * df;
+        vir[4] = vir[4] + rij[0] * rij[2] * df;
+        vir[5] = vir[5] + rij[1] * rij[2] * df;
+      }
+    }
+  }
+  if (cv)
+    for (int a = 0; a < 6; ++a) virial[a] = 0.0 + vir[a];
+  if (cu) *pe = 0.0 + pot;
+  free(vdr2);
+}
+
 /* Some PairMat(a,b) != PairMat(b,a): the reference's 'asym' flag (src/read_input.f90:130-133),
  * which selects compute1 in 'dynamics' (src/nd.f90:155-163). */
 int ndo_asymmetric(int ntypes, const int *pairmat) {
@@ -528,6 +636,15 @@ int ndo_run(const ndo_run_params *rp, int np, int ntypes, int npairs, int ptsize
                                 ptrmin, neimax, rp->brute, vnl, NULL);
   int nup = 0, nr = 0;
   const int asym = ndo_asymmetric(ntypes, pairmat);
+  int *spring = (int *)calloc((size_t)npairs, sizeof(int));
+  for (int p = 0; p < npairs && p < 31; ++p) spring[p] = (rp->springs >> p) & 1;
+  ndo_run_params *rpw = (ndo_run_params *)rp;   /* ipos / init_box / ipos_set are in-out */
+  if (rp->springs && !rp->ipos_set) {
+    /* initPositionSet at the start of dynamics (src/nd.f90:103, :1731-1763) */
+    memcpy(rpw->ipos, pos, sizeof(double) * 3 * (size_t)np);
+    for (int a = 0; a < 3; ++a) rpw->init_box[a] = box[a];
+    rpw->ipos_set = 1;
+  }
   double pe = 0.0, ke = 0.0, e0 = 0.0, vir[6] = {0, 0, 0, 0, 0, 0};
   int step = rp->step0;
   for (int tt = 1; tt <= rp->nsteps && !err; ++tt) {
@@ -551,7 +668,11 @@ int ndo_run(const ndo_run_params *rp, int np, int ntypes, int npairs, int ptsize
       }
     }
     ndo_integrate1(np, pos, vel, acc, disp, box, rp->dt);
-    if (asym)
+    if (rp->springs)
+      ndo_compute23(asym, np, pos, typ, box, vnl, neimax, ntypes, pairmat, P, F, rc2, dr2,
+                    ptrmin, ptsize, spring, pairparams, rp->ipos, rp->init_box, cv, cu, force,
+                    &pe, vir, pene);
+    else if (asym)
       ndo_compute1(np, pos, typ, box, vnl, neimax, ntypes, pairmat, P, F, rc2, dr2, ptrmin,
                    ptsize, cv, cu, force, &pe, vir, pene);
     else
@@ -587,5 +708,6 @@ int ndo_run(const ndo_run_params *rp, int np, int ntypes, int npairs, int ptsize
   free(rcp);
   free(vnl);
   free(disp);
+  free(spring);
   return err;
 }

@@ -27,7 +27,9 @@ enum {
   NDO_MORSE = 1,      /* reference Morse: table force = -V'(r)   (src/potentials.f90:184-204) */
   NDO_BUCK = 2,       /* Buckingham A exp(-r/rho) - C/r^6        (src/potentials.f90:112-122,206-216) */
   NDO_LJ = 3,         /* Lennard-Jones 4 eps((s/r)^12-(s/r)^6)   (src/potentials.f90:124-133,218-227) */
-  NDO_MORSE_PHYS = 4  /* energy-consistent Morse: table force = -r V'(r) (not in the reference) */
+  NDO_MORSE_PHYS = 4, /* energy-consistent Morse: table force = -r V'(r) (not in the reference) */
+  NDO_SPRING = 5      /* 'spring' (src/potentials.f90:88-92,178-183): a table for output only;
+                         compute2/3 take the force from the strain against ipos */
 };
 
 /* list-rebuild trigger */
@@ -81,6 +83,18 @@ void ndo_compute1(int np, const double *pos, const int *typ, const double box[3]
                   double ptrmin, int ptsize, int cv, int cu,
                   double *force, double *pe, double virial[6], double *pene);
 int ndo_asymmetric(int ntypes, const int *pairmat);
+/* compute2 (src/nd.f90:531-727, asym = 0) and compute3 (src/nd.f90:728-933, asym = 1):
+ * compute0 / compute1 where the pairs flagged in spring[] (npairs entries) take their force
+ * from the strain s = r/r0 - 1 against the initial positions ipos (3*np, scaled) in the
+ * initial box init_box: no force nor energy when s > PairParams(3), otherwise
+ * f = C s Vi^2 along the pair (C = PairParams(2), Vi = PairParams(4)) and e = f s r0 / 2.
+ * Spring pairs have no cutoff test: every listed pair interacts. */
+void ndo_compute23(int asym, int np, const double *pos, const int *typ, const double box[3],
+                   const int *vnl, int neimax, int ntypes, const int *pairmat,
+                   const double *P, const double *F, const double *rc2, const double *dr2,
+                   double ptrmin, int ptsize, const int *spring, const double *pairparams,
+                   const double *ipos, const double init_box[3], int cv, int cu,
+                   double *force, double *pe, double virial[6], double *pene);
 
 /* integrate1 (src/nd.f90:984-1046): drift, scaled displacement accumulation. */
 void ndo_integrate1(int np, double *pos, const double *vel, const double *acc,
@@ -121,6 +135,14 @@ typedef struct {
   int isobaric; double p_target[3], p_tau; int p_freq;
   int isoenergetic; double e_tau; int e_freq;
   int box_load; double box_vel[3];
+  /* spring pairs (compute2/3): bit p set when pair id p+1 is a 'spring'; ipos (3*np) and
+   * init_box are the initial positions and box of initPositionSet (src/nd.f90:1731-1763),
+   * filled from pos/box at the start of the Run when ipos_set is 0, as dynamics does
+   * (src/nd.f90:103) */
+  int springs;
+  double *ipos;
+  double init_box[3];
+  int ipos_set;
 } ndo_run_params;
 
 /* One thermo line (src/read_input.f90:433-446) plus the full virial and box. */

@@ -20,7 +20,7 @@ _lib = None
 dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 
-MORSE, BUCK, LJ, MORSE_PHYS = 1, 2, 3, 4
+MORSE, BUCK, LJ, MORSE_PHYS, SPRING = 1, 2, 3, 4, 5
 TRIGGER_COMPAT, TRIGGER_SKIN = 0, 1
 
 
@@ -34,7 +34,9 @@ class RunParams(C.Structure):
                 ("isobaric", C.c_int), ("p_target", C.c_double * 3), ("p_tau", C.c_double),
                 ("p_freq", C.c_int),
                 ("isoenergetic", C.c_int), ("e_tau", C.c_double), ("e_freq", C.c_int),
-                ("box_load", C.c_int), ("box_vel", C.c_double * 3)]
+                ("box_load", C.c_int), ("box_vel", C.c_double * 3),
+                ("springs", C.c_int), ("ipos", C.c_void_p), ("init_box", C.c_double * 3),
+                ("ipos_set", C.c_int)]
 
 
 class Thermo(C.Structure):
@@ -66,6 +68,10 @@ def lib():
         L.ndo_compute0.restype = None
         L.ndo_compute1.argtypes = L.ndo_compute0.argtypes
         L.ndo_compute1.restype = None
+        L.ndo_compute23.argtypes = [C.c_int, C.c_int, dp, ip, dp, ip, C.c_int, C.c_int, ip, dp,
+                                    dp, dp, dp, C.c_double, C.c_int, ip, dp, dp, dp, C.c_int,
+                                    C.c_int, dp, C.POINTER(C.c_double), dp, C.c_void_p]
+        L.ndo_compute23.restype = None
         L.ndo_asymmetric.argtypes = [C.c_int, ip]
         L.ndo_asymmetric.restype = C.c_int
         L.ndo_integrate1.argtypes = [C.c_int, dp, dp, dp, dp, dp, C.c_double]
@@ -166,11 +172,39 @@ def asymmetric(sys) -> bool:
     return bool(lib().ndo_asymmetric(sys.ntypes, pm))
 
 
-def compute(sys, vnl, neimax, tables, cv=True, cu=True):
-    """The force routine 'dynamics' would call: compute1 for asymmetric pair matrices,
-    compute0 otherwise (src/nd.f90:155-163)."""
+def springs(sys) -> int:
+    """Bit mask of the 'spring' pair ids (bit p: pair p+1), the reference's springpots."""
+    return int(sum(1 << p for p, k in enumerate(sys.kinds) if int(k) == SPRING))
+
+
+def compute(sys, vnl, neimax, tables, cv=True, cu=True, ipos=None, init_box=None):
+    """The force routine 'dynamics' would call (src/nd.f90:155-163): compute2/compute3
+    when spring pairs exist, else compute1 for asymmetric pair matrices, else compute0."""
+    if springs(sys):
+        return compute23(sys, vnl, neimax, tables, ipos, init_box, cv, cu)
     fn = compute1 if asymmetric(sys) else compute0
     return fn(sys, vnl, neimax, tables, cv, cu)
+
+
+def compute23(sys, vnl, neimax, tables, ipos, init_box, cv=True, cu=True, asym=None):
+    """compute2 (symmetric PairMat) or compute3 (asymmetric) with the initial positions
+    ipos (np,3 scaled) and box init_box of initPositionSet."""
+    P, F, rc2, dr2 = tables
+    asym = asymmetric(sys) if asym is None else asym
+    spring = np.array([1 if int(k) == SPRING else 0 for k in sys.kinds], np.int32)
+    force = np.zeros((sys.np, 3))
+    pe = C.c_double(0.0)
+    vir = np.zeros(6)
+    pene = np.zeros(sys.np)
+    lib().ndo_compute23(int(asym), sys.np, np.ascontiguousarray(sys.pos), sys.typ.astype(np.int32),
+                        np.ascontiguousarray(sys.box), np.ascontiguousarray(vnl), neimax,
+                        sys.ntypes, np.ascontiguousarray(sys.pairmat.T).ravel().astype(np.int32),
+                        np.ascontiguousarray(P), np.ascontiguousarray(F), rc2, dr2, sys.ptrmin,
+                        sys.ptsize, spring, np.ascontiguousarray(sys.params, dtype=np.float64).ravel(),
+                        np.ascontiguousarray(ipos, dtype=np.float64),
+                        np.ascontiguousarray(init_box, dtype=np.float64), int(cv), int(cu),
+                        force, C.byref(pe), vir, pene.ctypes.data_as(C.c_void_p))
+    return force, pe.value, vir, pene
 
 
 def compute1(sys, vnl, neimax, tables, cv=True, cu=True):
@@ -198,9 +232,12 @@ def _compute(fn, sys, vnl, neimax, tables, cv, cu):
 
 def run(sys, nsteps, dt, skin=-1.0, out_steps=None, step0=0, neimax=0, renei=True,
         renei_freq=1, trigger=TRIGGER_COMPAT, brute=False, acc=None,
-        isokinetic=None, isobaric=None, isoenergetic=None, box_vel=None, tables=None):
+        isokinetic=None, isobaric=None, isoenergetic=None, box_vel=None, tables=None,
+        ipos=None, init_box=None):
     """ndo_run on a copy of sys. isokinetic=(ke_target, tau, freq); isobaric=((px,py,pz),
-    tau, freq); isoenergetic=(tau, freq).  Returns (state dict, thermo list, err)."""
+    tau, freq); isoenergetic=(tau, freq).  With spring pairs, ipos/init_box are the initial
+    positions and box (default: pos and box at the start of the Run, as dynamics sets them).
+    Returns (state dict, thermo list, err); the state holds ipos/init_box."""
     if tables is None:
         tables = pair_tables(sys.kinds, sys.params, sys.ptrmin, sys.ptsize)
     P, F, rc2, dr2 = tables
@@ -228,6 +265,14 @@ def run(sys, nsteps, dt, skin=-1.0, out_steps=None, step0=0, neimax=0, renei=Tru
         rp.box_load = 1
         for a in range(3):
             rp.box_vel[a] = box_vel[a]
+    rp.springs = springs(sys)
+    ipos_buf = np.zeros((sys.np, 3))
+    if ipos is not None:
+        ipos_buf[:] = ipos
+        for a in range(3):
+            rp.init_box[a] = init_box[a]
+        rp.ipos_set = 1
+    rp.ipos = ipos_buf.ctypes.data
     pos = np.ascontiguousarray(sys.pos, dtype=np.float64).copy()
     vel = np.ascontiguousarray(sys.vel, dtype=np.float64).copy()
     accv = np.zeros_like(pos) if acc is None else np.ascontiguousarray(acc, dtype=np.float64).copy()
@@ -250,5 +295,5 @@ def run(sys, nsteps, dt, skin=-1.0, out_steps=None, step0=0, neimax=0, renei=Tru
                    volume=r.volume, virial=list(r.virial), box=list(r.box))
               for r in recs[:nrec.value]]
     state = dict(pos=pos, vel=vel, acc=accv, box=box, force=force, pene=pene,
-                 nupdates=nup.value)
+                 nupdates=nup.value, ipos=ipos_buf, init_box=np.array(rp.init_box[:]))
     return state, thermo, err

@@ -14,9 +14,13 @@
 !   real64 box(3), dt, skin, ptrmin, ketarget, keparm, ptarget(3), pparm, eparm
 !   real64 mass(ntypes); int32 pairmat(ntypes,ntypes); int32 kind(npairs)
 !   real64 pairparams(4,npairs); int32 typ(np); real64 pos(3,np), vel(3,np), acc(3,np)
+!   int32  boxload, have_ipos; real64 boxvel(3)
+!   (have_ipos /= 0) real64 ipos(3,np), initbox(3)
+! Pair kinds: 1 morse, 2 buck, 3 LJ, 5 spring.
 ! Modes: 1 tables (buildPairTables), 2 half list (buildCellList),
-!        3 forces (buildCellList + compute0(T,T), or compute1(T,T) for an asymmetric
-!        PairMat), 4 'dynamics' run.
+!        3 forces (buildCellList + the force routine dynamics would call: compute0,
+!        compute1 for an asymmetric PairMat, compute2/compute3 with spring pairs),
+!        4 'dynamics' run.
 program ref_harness
   use omp_lib
   use mostCommon
@@ -27,7 +31,7 @@ program ref_harness
   implicit none
   character(len=1024) :: fin, fout
   integer :: mode, nthr, i
-  integer :: iv(18)
+  integer :: iv(18), iv2(2)
   real(8) :: dv(13)
   integer, allocatable :: kinds(:)
 
@@ -67,24 +71,33 @@ program ref_harness
       case (1); PairName(i) = 'morse'
       case (2); PairName(i) = 'buck'
       case (3); PairName(i) = 'LJ'
+      case (5); PairName(i) = 'spring'
       case default; stop 'harness: unknown pair kind'
     end select
   enddo
   ! the deck's 'asym' flag (src/read_input.f90:130-133): some PairMat(a,b) /= PairMat(b,a)
-  asympots = any(PairMat /= transpose(PairMat)); springpots = .false.; potOuts = .false.
+  asympots = any(PairMat /= transpose(PairMat)); springpots = any(kinds == 5)
+  potOuts = .false.
 
   allocate(typ(np), pos(3,np), vel(3,np), acc(3,np), force(3,np), pene(np))
   read(20) typ
   read(20) pos
   read(20) vel
   read(20) acc
+  read(20) iv2
+  read(20) boxVel
+  if (iv2(2) /= 0) then
+    allocate(ipos(3,np))
+    read(20) ipos
+    read(20) initBox
+  endif
   close(20)
   force = 0.d0; pene = 0.d0
   grpBuff = 100; holdBuff = 100; Ngrp = 0; Nhold = 0
   allocate(grp(np), groupName(grpBuff), hold(np), holdName(holdBuff), &
            bcTyp(holdBuff), bcParm(3,holdBuff), bcDir(3,holdBuff))
   grp = 0; hold = 0; bcTyp = 'X'; bcParm = 0.d0; bcDir = .false.
-  boxLoad = .false.; boxVel = 0.d0
+  boxLoad = iv2(1) /= 0
   Model = 'NA'; ModelInType = 'NA'; ModelOutType = 'dump'; ConfOuts = 0
   potential = 0.d0; kinetic = 0.d0; virial = 0.d0; e0 = 0.d0
 
@@ -112,7 +125,11 @@ program ref_harness
       write(21) MCELL, NeiMax
       write(21) VNL
     else
-      if (asympots) then
+      if (asympots .and. springpots) then
+        call compute3(.true., .true.)
+      else if (springpots) then
+        call compute2(.true., .true.)
+      else if (asympots) then
         call compute1(.true., .true.)
       else
         call compute0(.true., .true.)

@@ -24,7 +24,7 @@ def available() -> bool:
 
 def write_input(path, mode, s, nsteps=0, outsteps=1, step0=0, neimax=0, skin=-1.0, dt=0.0,
                 renei=True, renei_freq=1, iso_ke=None, iso_p=None, iso_e=None, acc=None,
-                nthreads=1):
+                nthreads=1, box_vel=None, ipos=None, init_box=None):
     """Binary input of ref_harness.f90 (its header documents the layout)."""
     iv = np.array([mode, s.np, s.ntypes, s.npairs, s.ptsize, neimax, nsteps, outsteps, step0,
                    int(renei), renei_freq,
@@ -55,6 +55,11 @@ def write_input(path, mode, s, nsteps=0, outsteps=1, step0=0, neimax=0, skin=-1.
         f.write(np.asarray(s.vel, "<f8").tobytes())
         a = np.zeros_like(s.pos) if acc is None else acc
         f.write(np.asarray(a, "<f8").tobytes())
+        f.write(np.array([0 if box_vel is None else 1, 0 if ipos is None else 1], "<i4").tobytes())
+        f.write(np.asarray([0.0, 0.0, 0.0] if box_vel is None else box_vel, "<f8").tobytes())
+        if ipos is not None:
+            f.write(np.asarray(ipos, "<f8").tobytes())
+            f.write(np.asarray(init_box, "<f8").tobytes())
 
 
 def run_harness(mode, s, timeout=3600, **kw):

@@ -22,11 +22,13 @@ def golden(name):
 def system_from_golden(g):
     import numpy as np
     from ndyn_omp_amd.workloads import System
+    ipos = np.array(g["ipos"]) if "ipos" in g.files else None
+    init_box = np.array(g["init_box"]) if "init_box" in g.files else None
     return System(pos=np.array(g["pos"]), vel=np.array(g["vel"]), typ=np.array(g["typ"], np.int32),
                   box=np.array(g["box"]), masses=np.array(g["masses"]),
                   pairmat=np.array(g["pairmat"], np.int32), kinds=np.array(g["kinds"], np.int32),
                   params=np.array(g["params"]), ptrmin=float(g["ptrmin"]),
-                  ptsize=int(g["ptsize"]))
+                  ptsize=int(g["ptsize"]), ipos=ipos, init_box=init_box)
 
 
 @pytest.fixture(scope="session")

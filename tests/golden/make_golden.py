@@ -29,9 +29,16 @@ run_harness = refrun.run_harness
 
 
 def sys_arrays(s):
-    return dict(pos=s.pos, vel=s.vel, typ=s.typ, box=s.box, masses=s.masses,
-                pairmat=s.pairmat, kinds=s.kinds, params=s.params,
-                ptrmin=np.float64(s.ptrmin), ptsize=np.int64(s.ptsize))
+    d = dict(pos=s.pos, vel=s.vel, typ=s.typ, box=s.box, masses=s.masses,
+             pairmat=s.pairmat, kinds=s.kinds, params=s.params,
+             ptrmin=np.float64(s.ptrmin), ptsize=np.int64(s.ptsize))
+    if s.ipos is not None:
+        d.update(ipos=s.ipos, init_box=s.init_box)
+    return d
+
+
+def ipos_kw(s):
+    return {} if s.ipos is None else dict(ipos=s.ipos, init_box=s.init_box)
 
 
 def thermo_lines(stdout):
@@ -77,7 +84,7 @@ def golden_list(fname, s, skin):
 
 
 def golden_force(fname, s, skin):
-    raw, _ = run_harness(3, s, skin=skin)
+    raw, _ = run_harness(3, s, skin=skin, **ipos_kw(s))
     a = np.frombuffer(raw, "<f8")
     n = s.np
     np.savez_compressed(os.path.join(OUT, fname), **sys_arrays(s), skin=np.float64(skin),
@@ -85,7 +92,8 @@ def golden_force(fname, s, skin):
 
 
 def golden_run(fname, s, nsteps, dt, skin, outsteps, **kw):
-    raw, stdout = run_harness(4, s, nsteps=nsteps, dt=dt, skin=skin, outsteps=outsteps, **kw)
+    raw, stdout = run_harness(4, s, nsteps=nsteps, dt=dt, skin=skin, outsteps=outsteps,
+                              **ipos_kw(s), **kw)
     a = np.frombuffer(raw, "<f8")
     n = s.np
     o = 0
@@ -120,11 +128,32 @@ def asym_fixtures():
                W.DT_1FS, 0.5, 20)
 
 
+def spring_fixtures():
+    """Spring pairs (compute2 / compute3) against initial positions, and the peridynamic
+    tension deck examples/testPDspringTension/input.sh (4000 atoms, BoxVel, ReNeighbor F,
+    initial positions set by dynamics) for its first 300 steps."""
+    golden_force("force_spring_cu7.npz", W.spring_cu(7, stretch=(0.03, 0.0, 0.0), jitter_a=0.15,
+                                                     seed=131), 0.0)
+    golden_force("force_spring_mix0.npz", W.spring_mix(7, 0, seed=132), 0.5)
+    golden_force("force_spring_mix1.npz", W.spring_mix(7, 1, seed=133), 0.5)
+    golden_force("force_spring_mix2.npz", W.spring_mix(7, 2, seed=134), 0.5)
+    pd = W.spring_cu(10, a=3.68)
+    pd.ipos = pd.init_box = None
+    golden_run("run_spring_pd.npz", pd, 300, 0.1, 0.0, 100, renei=False,
+               box_vel=(1.0e-2, 0.0, 0.0))
+    golden_run("run_spring_mix1_nve.npz", W.spring_mix(7, 1, temp_k=300.0, jitter_a=0.0,
+                                                       stretch=(0.0, 0.0, 0.0), seed=12362),
+               60, W.DT_1FS, 0.5, 20)
+
+
 def main():
     if not os.path.exists(HARNESS):
         raise SystemExit("build the reference harness first: make -C oracle ref")
     if "--asym-only" in sys.argv:
         asym_fixtures()
+        return
+    if "--spring-only" in sys.argv:
+        spring_fixtures()
         return
     golden_tables()
     # half lists: Cu 7^3 (MCELL 3), c0-size Cu 10^3 (MCELL 4), Buckingham alloy (3 pairs)
@@ -147,6 +176,7 @@ def main():
     golden_run("run_alloy5_nve.npz", W.buck_alloy((5, 5, 5), temp_k=900.0, seed=12348), 60,
                W.DT_1FS, 0.5, 20, iso_e=(50 * W.DT_1FS, 5))
     asym_fixtures()
+    spring_fixtures()
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f"{f:28s} {os.path.getsize(os.path.join(OUT, f)) / 1024:8.1f} KiB")

@@ -146,6 +146,105 @@ def test_compute1_matches_reference(oracle, fname):
         assert np.abs(f.sum(axis=0)).max() > 1e-6
 
 
+SPRING_FORCES = ["force_spring_cu7.npz", "force_spring_mix0.npz", "force_spring_mix1.npz",
+                 "force_spring_mix2.npz"]
+
+
+def _spring_strains(s, vnl):
+    """Strain r/r0 - 1 of every listed spring pair, from the definitions (numpy)."""
+    stride = vnl.shape[1]
+    ii = np.repeat(np.arange(s.np), vnl[:, 0])
+    jj = np.concatenate([vnl[i, 1:1 + vnl[i, 0]] for i in range(s.np)])
+    ti, tj = s.typ[ii] - 1, s.typ[jj] - 1
+    pm = np.maximum(s.pairmat[ti, tj], s.pairmat[tj, ti])
+    sp = s.kinds[pm - 1] == W.SPRING
+    def dist(p, box):
+        d = p[ii] - p[jj]
+        d -= np.round(d)
+        return np.sqrt(((d * box) ** 2).sum(axis=1))
+    st = dist(s.pos, s.box) / dist(s.ipos, s.init_box) - 1.0
+    return st[sp], s.params[pm[sp] - 1, 2]
+
+
+@pytest.mark.parametrize("fname", SPRING_FORCES)
+def test_compute23_matches_reference(oracle, fname):
+    """Spring pairs: the reference's compute2 (symmetric PairMat, src/nd.f90:531-727) and
+    compute3 (asymmetric, src/nd.f90:728-933) against initial positions, bit for bit.
+    Every fixture has listed springs on both sides of the strain cut PairParams(3)."""
+    g = golden(fname)
+    s = system_from_golden(g)
+    assert s.springs and s.ipos is not None
+    skin = float(g["skin"])
+    tables = oracle.pair_tables(s.kinds, s.params, s.ptrmin, s.ptsize)
+    _, vnl, neimax = oracle.half_list(s, skin)
+    f, pe, vir, pene = oracle.compute(s, vnl, neimax, tables, ipos=s.ipos, init_box=s.init_box)
+    scale = np.abs(g["force"]).max()
+    np.testing.assert_allclose(f, g["force"], rtol=1e-11, atol=1e-13 * scale)
+    assert pe == pytest.approx(float(g["pe"]), rel=1e-13)
+    np.testing.assert_allclose(vir, g["virial"], rtol=1e-12, atol=1e-12 * np.abs(g["virial"]).max())
+    assert pene.sum() == pytest.approx(pe, rel=1e-12)
+    strain, s0 = _spring_strains(s, vnl)
+    assert (strain > s0).sum() > 10 and (strain <= s0).sum() > 100
+    if not oracle.asymmetric(s):
+        assert np.abs(f.sum(axis=0)).max() < 1e-11 * max(1.0, scale) * math.sqrt(s.np)
+
+
+def test_spring_energy_and_virial_closed_form(oracle):
+    """A perfect lattice stretched affinely (no jitter): every pair within the list cutoff
+    (rc + skin = 6 A, current distances) carries e = C Vi^2 s^2 r0 / 2 and the virial
+    r (x) r f/r with f = -C s Vi^2, s = r/r0 - 1, unless s > s0.  Brute force over all
+    minimum-image pairs in numpy."""
+    s = W.spring_cu(5, stretch=(0.07, -0.02, 0.11))
+    rc, C, s0, Vi = W.PD_SPRING
+    tables = oracle.pair_tables(s.kinds, s.params, s.ptrmin, s.ptsize)
+    _, vnl, neimax = oracle.half_list(s, 0.0)
+    _, pe, vir, _ = oracle.compute(s, vnl, neimax, tables, ipos=s.ipos, init_box=s.init_box)
+    i, j = np.triu_indices(s.np, 1)
+    d = s.pos[i] - s.pos[j]
+    d -= np.round(d)
+    rv = d * s.box
+    r = np.sqrt((rv ** 2).sum(axis=1))
+    d0 = s.ipos[i] - s.ipos[j]
+    d0 -= np.round(d0)
+    r0 = np.sqrt(((d0 * s.init_box) ** 2).sum(axis=1))
+    keep = r <= rc
+    assert np.abs(r[keep] - rc).min() > 1e-6 and np.abs(r[~keep] - rc).min() > 1e-6
+    st = r / r0 - 1.0
+    on = keep & (st <= s0)
+    assert (keep & (st > s0)).sum() > 0     # some listed bonds are past the strain cut
+    e_ref = (0.5 * C * Vi * Vi * st[on] ** 2 * r0[on]).sum()
+    fr = -C * st[on] * Vi * Vi / r[on]
+    v = rv[on]
+    vir_ref = np.array([(v[:, 0] ** 2 * fr).sum(), (v[:, 1] ** 2 * fr).sum(),
+                        (v[:, 2] ** 2 * fr).sum(), (v[:, 0] * v[:, 1] * fr).sum(),
+                        (v[:, 0] * v[:, 2] * fr).sum(), (v[:, 1] * v[:, 2] * fr).sum()])
+    assert pe == pytest.approx(e_ref, rel=1e-12)
+    np.testing.assert_allclose(vir, vir_ref, rtol=1e-11, atol=1e-11 * np.abs(vir_ref).max())
+
+
+def test_spring_tension_log(oracle):
+    """examples/testPDspringTension/input.sh (4000-atom lattice at rest, one spring pair,
+    Skin 0, ReNeighbor F, BoxVel 1e-2 0 0, Run 40000 0.1; initial positions set by dynamics)
+    for 300 steps against examples/testPDspringTension/out:39-41: PE 0.191554, pressure
+    -0.145643E-02, V 49970.1 at step 100; 0.774476, -0.292621E-02, 50105.5 at 200;
+    1.74955, -0.439464E-02, 50240.9 at 300 (6 significant digits)."""
+    s = W.spring_cu(10, a=3.68)
+    s.ipos = s.init_box = None
+    st, thermo, err = oracle.run(s, 300, 0.1, skin=0.0, out_steps=100, renei=False,
+                                 box_vel=(1.0e-2, 0.0, 0.0))
+    assert err == 0
+    log = {100: (0.191554, -0.145643e-02, 49970.1), 200: (0.774476, -0.292621e-02, 50105.5),
+           300: (1.74955, -0.439464e-02, 50240.9)}
+    got = {r["step"]: (r["pe"], r["pressure"], r["volume"]) for r in thermo}
+    for step, ref in log.items():
+        for a, b in zip(got[step], ref):
+            unit = 10.0 ** (math.floor(math.log10(abs(b))) - 5)
+            assert abs(a - b) <= unit, (step, a, b)
+    # dynamics set the initial positions from the Run's first state
+    np.testing.assert_array_equal(st["ipos"], s.pos)
+    np.testing.assert_array_equal(st["init_box"], s.box)
+
+
 def test_compute1_equals_compute0_on_symmetric_matrix(oracle):
     g = golden("force_alloy5.npz")
     s = system_from_golden(g)
@@ -222,7 +321,8 @@ def test_reference_log_known_answers(oracle):
 
 # ----------------------------------------------------------------- dynamics
 @pytest.mark.parametrize("fname", ["run_cu10_nve.npz", "run_ar6_nvt.npz", "run_cu7_npt.npz",
-                                   "run_alloy5_nve.npz", "run_asym2_nve.npz"])
+                                   "run_alloy5_nve.npz", "run_asym2_nve.npz",
+                                   "run_spring_pd.npz", "run_spring_mix1_nve.npz"])
 def test_dynamics_matches_reference(oracle, fname):
     g = golden(fname)
     s = system_from_golden(g)
@@ -236,6 +336,12 @@ def test_dynamics_matches_reference(oracle, fname):
     if "iso_e" in g.files:
         e = g["iso_e"]
         kw["isoenergetic"] = (float(e[0]), int(e[1]))
+    if "box_vel" in g.files:
+        kw["box_vel"] = tuple(float(v) for v in g["box_vel"])
+    if "renei" in g.files:
+        kw["renei"] = bool(int(g["renei"]))
+    if s.ipos is not None:
+        kw.update(ipos=s.ipos, init_box=s.init_box)
     st, thermo, err = oracle.run(s, int(g["nsteps"]), float(g["dt"]), skin=float(g["skin"]),
                                  out_steps=int(g["outsteps"]), **kw)
     assert err == 0
