@@ -54,7 +54,7 @@ enum {
 enum { NDG_FRAME_XYZ = 1, NDG_FRAME_DUMP = 2 };
 
 /* pair kinds for ndg_pair_table (reference potentials.f90 names) */
-enum { NDG_MORSE = 1, NDG_BUCK = 2, NDG_LJ = 3, NDG_MORSE_PHYS = 4 };
+enum { NDG_MORSE = 1, NDG_BUCK = 2, NDG_LJ = 3, NDG_MORSE_PHYS = 4, NDG_SPRING = 5 };
 
 typedef struct ndg_system ndg_t;
 typedef struct ndg_engine ndg_engine_t;
@@ -103,6 +103,21 @@ int ndg_create(ndg_t **h, int np, int ntypes, int npairs, int ptsize, int neimax
 int ndg_set_tables(ndg_t *h, const double *P, const double *F, const double *rc2,
                    const double *dr2, double ptrmin, const int *pairmat,
                    const double *pairparams, const double *mass);
+
+/* PairName (src/read_input.f90:134-135) as NDG_* kinds, npairs entries.  NDG_SPRING pairs
+ * select the reference's compute2 (symmetric PairMat) or compute3 (asymmetric) rules
+ * (src/nd.f90:155-163, :531-933): force from the strain against the initial positions, no
+ * distance cutoff, none past the strain limit PairParams(3).  Set before the first list
+ * build or Run; the default is no spring pairs. */
+int ndg_set_pair_kinds(ndg_t *h, const int *kinds);
+
+/* initPositionSet (src/nd.f90:1731-1763, the deck's InitPos): the current positions and
+ * box become the initial ones of the spring pairs.  Without it, the first Run with spring
+ * pairs sets them from its first state, as dynamics does (src/nd.f90:103). */
+int ndg_init_positions(ndg_t *h);
+
+/* The module state ipos(3,np) (scaled) and initBox(3) (src/mods.f90:5,7) set directly. */
+int ndg_set_initial_positions(ndg_t *h, const double *ipos, const double ibox[3]);
 
 /* Upload state: pos (scaled), vel, acc (may be NULL = 0, as read_model sets,
  * src/read_input.f90:354-355), typ (1-based), box (A). */

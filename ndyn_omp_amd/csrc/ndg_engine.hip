@@ -66,7 +66,9 @@ static const char *err_text(int code) {
 // ------------------------------------------------------------------ pair tables (host)
 // buildPairTables (src/potentials.f90:2-56) with pot/dpot (src/potentials.f90:59-247):
 // Morse (:94-110, :184-204, the /r of :198 kept), Buckingham (:112-122, :206-216),
-// Lennard-Jones (:124-133, :218-227); NDG_MORSE_PHYS tabulates -r V'(r) for Morse.
+// Lennard-Jones (:124-133, :218-227), spring (:88-92, :178-183: a table written for output
+// only, the force kernels take spring forces from the strain); NDG_MORSE_PHYS tabulates
+// -r V'(r) for Morse.
 
 static double pair_v(int kind, double r, const double *p) {
   switch (kind) {
@@ -83,6 +85,10 @@ static double pair_v(int kind, double r, const double *p) {
     case NDG_LJ: {
       double q = p[2] / r, q6 = q * q * q * q * q * q;
       return 4.0 * p[1] * (q6 * q6 - q6);
+    }
+    case NDG_SPRING: {   // D (r/rc C)^2 - D C^2 with rc, D, C = PairParams(1:3)
+      double x = r / p[0] * p[2];
+      return p[1] * (x * x - p[2] * p[2]);
     }
   }
   return 0.0;
@@ -106,13 +112,17 @@ static double pair_dv(int kind, double r, const double *p) {  // -dV/dr (referen
       double q = p[2] / r, q6 = q * q * q * q * q * q;
       return 24.0 * p[1] * (2.0 * q6 * q6 / r - q6 / r);
     }
+    case NDG_SPRING: {
+      double x = r / p[0] * p[2];
+      return -2.0 * p[1] * x;
+    }
   }
   return 0.0;
 }
 
 extern "C" int ndg_pair_table(int kind, const double params[4], double ptrmin, int ptsize,
                               double *P, double *F, double *rc2, double *dr2) {
-  if (kind < NDG_MORSE || kind > NDG_MORSE_PHYS || ptsize < 2 || !P || !F)
+  if (kind < NDG_MORSE || kind > NDG_SPRING || ptsize < 2 || !P || !F)
     return fail(NDG_ERR_ARG, "ndg_pair_table: bad arguments");
   const double rc = params[0];
   const double c2 = rc * rc;
@@ -151,10 +161,23 @@ struct HostState {      // caller-provided data for one replica (caller order)
   std::vector<double> pos, vel, acc;
   double box[3] = {0, 0, 0};
   bool have_tables = false, have_state = false;
+  // pair kinds (NDG_MORSE..NDG_SPRING; empty: no spring pairs) and, for spring pairs, the
+  // initial positions (caller order, scaled) and box of initPositionSet
+  std::vector<int> kinds;
+  std::vector<double> ipos;
+  double ibox[3] = {0, 0, 0};
+  bool have_ipos = false;
+  bool springs() const {
+    for (int k : kinds)
+      if (k == NDG_SPRING) return true;
+    return false;
+  }
 };
 
 struct Slot {
   bool used = false;
+  bool springs = false;     // the block holds initial positions (spring pairs)
+  bool ipos_dirty = false;  // host initial positions newer than the device copy
   int np = 0, ntypes = 0, npairs = 0, ptsize = 0, neimax = 0, tpa = 1, ncell_cap = 0;
   int ntiles = 0, tile_base = 0;
   void *block = nullptr;
@@ -311,6 +334,9 @@ static size_t carve(Slot &s, int path, char *block) {
   size_t o_vdr2 = L.add(sizeof(float) * s.npairs);
   size_t o_d[16];
   for (int q = 0; q < 16; ++q) o_d[q] = L.add(sizeof(double) * np);  // px..pbz (16 arrays)
+  const size_t nip = s.springs ? (size_t)np : 0;
+  size_t o_ip[3];
+  for (int q = 0; q < 3; ++q) o_ip[q] = L.add(sizeof(double) * nip);  // ipx..ipz (springs)
   size_t o_f32a = L.add(sizeof(float4) * np);
   size_t o_f32b = L.add(sizeof(float4) * np);
   size_t o_x64a = L.add(sizeof(double4) * np);
@@ -326,7 +352,7 @@ static size_t carve(Slot &s, int path, char *block) {
   size_t o_cf = L.add(sizeof(int) * s.ncell_cap);
   size_t o_key = L.add(sizeof(int) * np);
   size_t o_src = L.add(sizeof(int) * np);
-  size_t o_tmpd = L.add(sizeof(double) * 9 * (size_t)np);
+  size_t o_tmpd = L.add(sizeof(double) * (s.springs ? 12 : 9) * (size_t)np);
   size_t o_tmpf = L.add(sizeof(float4) * 2 * (size_t)np);
   size_t o_tmpi = L.add(sizeof(int) * 2 * (size_t)np);
   size_t o_part = L.add(sizeof(double) * kPartials * ntiles);
@@ -342,6 +368,9 @@ static size_t carve(Slot &s, int path, char *block) {
     double **dp[16] = {&P.px, &P.py, &P.pz, &P.vx, &P.vy, &P.vz, &P.ax, &P.ay,
                        &P.az, &P.fx, &P.fy, &P.fz, &P.pene, &P.pbx, &P.pby, &P.pbz};
     for (int q = 0; q < 16; ++q) *dp[q] = (double *)(block + o_d[q]);
+    P.ipx = s.springs ? (double *)(block + o_ip[0]) : nullptr;
+    P.ipy = s.springs ? (double *)(block + o_ip[1]) : nullptr;
+    P.ipz = s.springs ? (double *)(block + o_ip[2]) : nullptr;
     P.f32a = (float4 *)(block + o_f32a);
     P.f32b = (float4 *)(block + o_f32b);
     P.x64a = (double4 *)(block + o_x64a);
@@ -571,6 +600,9 @@ static EngineDev dev_view(ndg_engine *e) {
   E.tabset = e->d_tabset;
   E.tabset_n = e->tabset_n;
   E.tabset_p = e->tabset_p;
+  E.springs = 0;
+  for (const Slot &sl : e->slots)
+    if (sl.used && sl.P.spring_mask) E.springs = 1;
   return E;
 }
 
@@ -710,6 +742,15 @@ static int upload_host(ndg_engine *e, Slot &s, const HostState &h, bool state_to
   for (int a = 0; a < h.ntypes; ++a)
     for (int b = 0; b < h.ntypes; ++b)
       if (h.pairmat[a + b * h.ntypes] != h.pairmat[b + a * h.ntypes]) P.asym = 1;
+  // spring pairs (compute2/3): C, strain limit and Vi from PairParams(2:4)
+  P.spring_mask = 0;
+  for (int p = 0; p < (int)h.kinds.size() && p < 16; ++p) {
+    if (h.kinds[p] != NDG_SPRING || !P.ipx) continue;
+    P.spring_mask |= 1 << p;
+    P.spr_c[p] = h.pairparams[4 * p + 1];
+    P.spr_s0[p] = h.pairparams[4 * p + 2];
+    P.spr_vi[p] = h.pairparams[4 * p + 3];
+  }
   hipStream_t st = e->stream;
   HIPCHK(hipMemcpyAsync((void *)P.mass, h.mass.data(), 8 * h.ntypes, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync((void *)P.pairmat, h.pairmat.data(), 4 * h.ntypes * h.ntypes, hipMemcpyHostToDevice, st));
@@ -734,6 +775,11 @@ static int upload_host(ndg_engine *e, Slot &s, const HostState &h, bool state_to
     int rc;
     if ((rc = put3(h.pos, P.px, P.py, P.pz))) return rc;
     if ((rc = put3(h.vel, P.vx, P.vy, P.vz))) return rc;
+    // the state goes up in caller order, so the initial positions go up as given
+    if (P.ipx && h.have_ipos) {
+      if ((rc = put3(h.ipos, P.ipx, P.ipy, P.ipz))) return rc;
+      s.ipos_dirty = false;
+    }
     if ((rc = put3(h.acc, P.ax, P.ay, P.az))) return rc;
     // mixed path keeps the latest force (F = m a) in FP32
     std::vector<float4> f4(np);
@@ -841,6 +887,11 @@ static int migrate_block(ndg_engine *e, const RepParams &o, const RepParams &n, 
   void *dst[] = {n.px, n.py, n.pz, n.vx, n.vy, n.vz, n.ax, n.ay, n.az, n.fx, n.fy, n.fz,
                  n.pene, n.pbx, n.pby, n.pbz};
   for (int q = 0; q < 16; ++q) HIPCHK(hipMemcpyAsync(dst[q], src[q], d, hipMemcpyDeviceToDevice, st));
+  if (o.ipx && n.ipx) {
+    HIPCHK(hipMemcpyAsync(n.ipx, o.ipx, d, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(n.ipy, o.ipy, d, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(n.ipz, o.ipz, d, hipMemcpyDeviceToDevice, st));
+  }
   HIPCHK(hipMemcpyAsync(n.f32a, o.f32a, 16 * (size_t)np, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(n.f32b, o.f32b, 16 * (size_t)np, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(n.x64a, o.x64a, 32 * (size_t)np, hipMemcpyDeviceToDevice, st));
@@ -1366,6 +1417,26 @@ static int fill_host(HostState &h, int np, int ntypes, int npairs, int ptsize, c
   return NDG_OK;
 }
 
+// Initial positions (caller order in h.ipos) to the device in the slot's current sorted
+// order (atom i of the block is caller atom oid[i]).
+static int upload_ipos_sorted(ndg_engine *e, Slot &s, const HostState &h) {
+  const int np = s.np;
+  hipStream_t st = e->stream;
+  std::vector<int> oid(np);
+  HIPCHK(hipMemcpyAsync(oid.data(), s.P.oid, 4 * (size_t)np, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  std::vector<double> soa(3 * (size_t)np);
+  for (int i = 0; i < np; ++i)
+    for (int a = 0; a < 3; ++a) soa[(size_t)a * np + i] = h.ipos[3 * (size_t)oid[i] + a];
+  double *dst[3] = {s.P.ipx, s.P.ipy, s.P.ipz};
+  for (int a = 0; a < 3; ++a)
+    HIPCHK(hipMemcpyAsync(dst[a], soa.data() + (size_t)a * np, 8 * (size_t)np,
+                          hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  s.ipos_dirty = false;
+  return NDG_OK;
+}
+
 // Make slot r resident for a Run: allocate for the list capacity, upload, build lists.
 static int begin_run(ndg_engine *e, int r, HostState &h, const ndg_run_params &rp, int step0,
                      int neimax_req, bool fresh_state) {
@@ -1394,7 +1465,23 @@ static int begin_run(ndg_engine *e, int r, HostState &h, const ndg_run_params &r
       }
   }
   const int ncap = ncell_capacity(box, rcut, skin);
-  const bool need_alloc = !s.block || neimax > s.neimax || ncap > s.ncell_cap || tpa != s.tpa;
+  const bool springs = h.springs();
+  if (springs && !h.have_ipos) {
+    // initPositionSet at the start of dynamics when no initial positions exist yet
+    // (src/nd.f90:103, :1731-1763): the Run's first positions and box
+    if (fresh_state || !s.block) {
+      h.ipos = h.pos;
+    } else {
+      h.ipos.assign(3 * (size_t)h.np, 0.0);
+      int rc0 = download_state(e, s, h.ipos.data(), nullptr, nullptr, nullptr, nullptr, nullptr);
+      if (rc0) return rc0;
+    }
+    for (int a = 0; a < 3; ++a) h.ibox[a] = box[a];
+    h.have_ipos = true;
+    s.ipos_dirty = true;
+  }
+  const bool need_alloc = !s.block || neimax > s.neimax || ncap > s.ncell_cap || tpa != s.tpa ||
+                          springs != s.springs;
   int rc;
   if (need_alloc) {
     // keep the state across the reallocation when it is already on the device
@@ -1414,6 +1501,7 @@ static int begin_run(ndg_engine *e, int r, HostState &h, const ndg_run_params &r
     s.ntypes = h.ntypes;
     s.npairs = h.npairs;
     s.ptsize = h.ptsize;
+    s.springs = springs;
     if ((rc = alloc_slot(e, r, neimax, ncap, tpa))) return rc;
     if ((rc = ensure_capacity(e))) return rc;
   }
@@ -1424,6 +1512,7 @@ static int begin_run(ndg_engine *e, int r, HostState &h, const ndg_run_params &r
   set_run_params(s, rp, step0);
   if ((rc = upload_host(e, s, h, fresh_state))) return rc;
   if ((rc = upload_params(e, r))) return rc;
+  if (springs && s.ipos_dirty && (rc = upload_ipos_sorted(e, s, h))) return rc;
   RepState S;
   if (fresh_state) {
     memset(&S, 0, sizeof(S));
@@ -1432,6 +1521,7 @@ static int begin_run(ndg_engine *e, int r, HostState &h, const ndg_run_params &r
   } else if ((rc = get_state_dev(e, r, &S))) {
     return rc;
   }
+  for (int a = 0; a < 3; ++a) S.ibox[a] = springs ? h.ibox[a] : 0.0;
   S.step = step0;
   S.done = rp.nsteps <= 0 ? 1 : 0;
   S.err = 0;
@@ -1685,6 +1775,61 @@ extern "C" int ndg_set_tables(ndg_t *h, const double *P, const double *F, const 
     }
   }
   return NDG_OK;
+}
+
+extern "C" int ndg_set_pair_kinds(ndg_t *h, const int *kinds) {
+  if (!h || !kinds) return fail(NDG_ERR_ARG, "ndg_set_pair_kinds: null argument");
+  HostState &hs = h->hs;
+  std::vector<int> k(kinds, kinds + hs.npairs);
+  for (int v : k)
+    if (v < NDG_MORSE || v > NDG_SPRING) return fail(NDG_ERR_ARG, "ndg_set_pair_kinds: unknown kind");
+  if (k == hs.kinds) return NDG_OK;
+  Slot &s = h->e->slots[h->slot];
+  if (s.block)
+    return fail(NDG_ERR_ARG, "ndg_set_pair_kinds: set the kinds before the first list build or Run");
+  hs.kinds = k;
+  return NDG_OK;
+}
+
+// Initial positions (caller order) and box to the host copy and, when resident with room
+// for them, to the device in its sorted order.
+static int put_initial_positions(ndg_t *h, const double *ipos, const double ibox[3]) {
+  HostState &hs = h->hs;
+  hs.ipos.assign(ipos, ipos + 3 * (size_t)hs.np);
+  for (int a = 0; a < 3; ++a) hs.ibox[a] = ibox[a];
+  hs.have_ipos = true;
+  Slot &s = h->e->slots[h->slot];
+  s.ipos_dirty = true;
+  if (!s.block || !s.P.ipx) return NDG_OK;   // uploaded when the next Run begins
+  int rc = upload_ipos_sorted(h->e, s, hs);
+  if (rc) return rc;
+  RepState S;
+  if ((rc = get_state_dev(h->e, h->slot, &S))) return rc;
+  for (int a = 0; a < 3; ++a) S.ibox[a] = ibox[a];
+  return put_state_dev(h->e, h->slot, &S);
+}
+
+extern "C" int ndg_set_initial_positions(ndg_t *h, const double *ipos, const double ibox[3]) {
+  if (!h || !ipos || !ibox) return fail(NDG_ERR_ARG, "ndg_set_initial_positions: null argument");
+  int rc = settle(h);
+  if (rc) return rc;
+  return put_initial_positions(h, ipos, ibox);
+}
+
+extern "C" int ndg_init_positions(ndg_t *h) {
+  if (!h) return fail(NDG_ERR_ARG, "null handle");
+  int rc = settle(h);
+  if (rc) return rc;
+  HostState &hs = h->hs;
+  Slot &s = h->e->slots[h->slot];
+  if (!s.block) {
+    if (!hs.have_state) return fail(NDG_ERR_ARG, "ndg_init_positions: no state set");
+    return put_initial_positions(h, hs.pos.data(), hs.box);
+  }
+  std::vector<double> pos(3 * (size_t)hs.np);
+  double box[3];
+  if ((rc = download_state(h->e, s, pos.data(), nullptr, nullptr, nullptr, nullptr, box))) return rc;
+  return put_initial_positions(h, pos.data(), box);
 }
 
 extern "C" int ndg_set_state(ndg_t *h, const double *pos, const double *vel, const double *acc,

@@ -50,6 +50,7 @@ struct RepParams {
   int renei, renei_freq, trigger;
   int isokinetic, ke_freq, isobaric, p_freq, isoenergetic, e_freq, box_load;
   int asym;                 // PairMat not symmetric: compute1's rules (src/nd.f90:364-530)
+  int spring_mask;          // bit p: pair id p+1 is a 'spring' (compute2/3, src/nd.f90:531-933)
   double dt, skin, rcut, ptrmin;
   double ke_target, ke_tau, p_target[3], p_tau, e_tau, box_vel[3];
   // per-type / per-pair data
@@ -64,6 +65,7 @@ struct RepParams {
   const float2 *tabF2;      // [npairs*(ptsize+1)] {F_k, F_k+1} (mixed path)
   const float2 *tabP2;      // same for the potential
   int ltab[16];             // offset of each pair's FP32 F table in the engine's LDS set, -1: none
+  double spr_c[16], spr_s0[16], spr_vi[16];  // spring pairs: PairParams(2), (3), (4)
   // per-atom state (sorted order)
   double *px, *py, *pz, *vx, *vy, *vz;
   double *ax, *ay, *az;     // FP64 path acceleration
@@ -72,6 +74,7 @@ struct RepParams {
   float4 *f32a, *f32b;      // mixed path force (xyz) + per-atom energy (w), two parities
   double4 *x64a;            // mixed path cell-anchored coordinates (xyz, A) + type bits (w)
   float4 *x32b;             // FP32 copy of x64a at the last build (displacement check)
+  double *ipx, *ipy, *ipz;  // spring pairs: initial positions (scaled) of initPositionSet
   uint4 *xq;                // 16-byte copy of x64a gathered by the mixed force kernel
   int4 *anchor;             // integer anchor per axis (image*M + cell) and linear cell (w)
   int *typ, *oid, *nnb;
@@ -79,7 +82,7 @@ struct RepParams {
   // rebuild scratch
   int *cell_count, *cell_start, *cell_fill;  // cell_start has ncell_cap+1 entries
   int *key, *slot_src;
-  double *tmpd;             // 9 * np
+  double *tmpd;             // 9 * np (12 * np with spring pairs)
   float4 *tmpf;             // 2 * np (both force buffers)
   int *tmpi;                // 2 * np
   double *partials;         // kPartials * ntiles, quantity-major
@@ -103,6 +106,7 @@ struct RepState {
   double box_build[3];
   double e0, pe, ke, vir[6];
   double lam1, lam2; // pending velocity scales (isokinetic, isoenergetic)
+  double ibox[3];    // spring pairs: box of initPositionSet (initBox)
 };
 
 struct EngineDev {
@@ -117,6 +121,7 @@ struct EngineDev {
   const float *tabset;  // distinct FP32 F tables of all replicas, then their P tables
   int tabset_n;         // floats of the F half (0: tables are read from global memory)
   int tabset_p;         // 1: the energy/virial force variant also stages the P half
+  int springs;          // some replica has spring pairs: launch the force variants with them
 };
 
 // LDS budgets of the staged tables: with the force kernel's own LDS (at most ~15 KB) a

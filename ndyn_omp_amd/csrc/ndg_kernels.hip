@@ -287,12 +287,12 @@ struct ForceShape {
   static constexpr int kNSub = kThreads / kSub;            // tiles per pass
 };
 
-template <int TPA, bool FULL, bool LDS_TAB, bool ONE, bool ASYM = false>
+template <int TPA, bool FULL, bool LDS_TAB, bool ONE, bool ASYM = false, bool SPRING = false>
 __device__ inline void force_atom(const RepParams &R, int i, int sub, const double4 *shift,
                                   const double *rc2, const float *vdr2, const int *pm,
                                   const int *lt, const float *stab, int ptoff, const unsigned *L,
                                   double qunit, double &fx, double &fy, double &fz,
-                                  double (&acc)[7]) {
+                                  double (&acc)[7], const double *ibox = nullptr) {
   // the energy/virial variant holds seven more sums: an unroll of 2 keeps it at four
   // waves per SIMD (1.37 vs 1.52 ms per launch on 2M atoms)
   constexpr int U = FULL ? 2 : kUnroll;
@@ -358,11 +358,33 @@ __device__ inline void force_atom(const RepParams &R, int i, int sub, const doub
           feel = pji > 0;
           bal = pij == pji;
         }
+        // padding entries (code 63) read the atom itself, whose type may have no pair with
+        // itself (id 0): keep the parameter and table reads in bounds (d2 masks the entry)
+        pid = pid < 0 ? 0 : pid;
         rc2p = rc2[pid];
         vd = vdr2[pid];
         if (LDS_TAB) ltp = lt[pid];
       }
-      const bool ok = !(d2 > rc2p) && feel;
+      bool ok = !(d2 > rc2p) && feel;
+      double dsp = 0.0, psp = 0.0;   // SPRING: force over r and pair energy of a spring pair
+      const bool spr = SPRING && ((R.spring_mask >> pid) & 1);
+      if (spr) {
+        // compute2/3 spring (src/nd.f90:622-660): strain against the initial positions in
+        // FP64, no distance cutoff, nothing past the strain limit PairParams(3)
+        const int j = (int)(e[u] & kIndexMask);
+        double u0 = R.ipx[i] - R.ipx[j], u1 = R.ipy[i] - R.ipy[j], u2 = R.ipz[i] - R.ipz[j];
+        u0 = u0 - round(u0);
+        u1 = u1 - round(u1);
+        u2 = u2 - round(u2);
+        const double q0 = ibox[0] * u0, q1 = ibox[1] * u1, q2 = ibox[2] * u2;
+        const double rr = sqrt(d2), rr0 = sqrt(fma(q0, q0, fma(q1, q1, q2 * q2)));
+        const double sij = rr / rr0 - 1.0;
+        ok = feel && (e[u] >> kCodeShift) != 63u && !(sij > R.spr_s0[pid]);
+        const double Vi = R.spr_vi[pid];
+        const double fs = R.spr_c[pid] * sij * Vi * Vi;
+        dsp = ok ? -fs / rr : 0.0;
+        psp = 0.5 * fs * sij * rr0;
+      }
       // the reference's table position, in its single-precision operations
       // (src/nd.f90:299): rk = real(d2 - PTrMin) * vPTdr2 + 1
       const float rk = (float)(d2 - ptrmin) * vd + 1.0f;
@@ -380,7 +402,7 @@ __device__ inline void force_atom(const RepParams &R, int i, int sub, const doub
         const float F0 = TF[q], F1 = TF[q + 1];
         fv = fmaf(w, F1 - F0, F0);
       }
-      const double df = (double)(ok ? fv * __builtin_amdgcn_rcpf((float)d2) : 0.0f);
+      const double df = spr ? dsp : (double)(ok ? fv * __builtin_amdgcn_rcpf((float)d2) : 0.0f);
       fx = fma(dx, df, fx);
       fy = fma(dy, df, fy);
       fz = fma(dz, df, fz);
@@ -395,7 +417,7 @@ __device__ inline void force_atom(const RepParams &R, int i, int sub, const doub
           P0 = TP[q];
           P1 = TP[q + 1];
         }
-        const double pv = ok && bal ? (double)fmaf(w, P1 - P0, P0) : 0.0;
+        const double pv = ok && bal ? (spr ? psp : (double)fmaf(w, P1 - P0, P0)) : 0.0;
         // energy and virial sums without their factor 1/2, applied once per atom (an
         // exact scaling); an unbalanced pair of an asymmetric matrix adds to neither
         const double dv = !ASYM || bal ? df : 0.0;
@@ -414,7 +436,7 @@ __device__ inline void force_atom(const RepParams &R, int i, int sub, const doub
 
 // kick != 0: integrate2 follows in the same pass (v += dt/2 (F_new + F_old)/m, kinetic
 // energy partials), with the arithmetic and the summation order of k_kick_mixed.
-template <int TPA, bool FULL>
+template <int TPA, bool FULL, bool SPR>
 __global__ __launch_bounds__(ForceShape<TPA>::kThreads) void k_force_mixed(EngineDev E, int t0,
                                                                            int nt, int xflags,
                                                                            int kick) {
@@ -494,8 +516,15 @@ __global__ __launch_bounds__(ForceShape<TPA>::kThreads) void k_force_mixed(Engin
       const unsigned *L = R.list + ((size_t)T.ltile * R.list_chunks) * (kTile * TPA) + lt;
       // both choices are the same for every wave of the tile
       const bool asym = R.asym != 0;
-      const bool lds = sh_uselds[sg] != 0, one = R.npairs == 1 && !asym;
-      if (asym && lds)
+      const bool lds = sh_uselds[sg] != 0, one = R.npairs == 1 && !asym && !R.spring_mask;
+      // SPR: the instance launched when some replica has spring pairs (the spring
+      // variant's registers stay out of the common instance)
+      if (SPR && R.spring_mask)
+        force_atom<TPA, FULL, false, false, true, true>(R, i, sub, sh_shift[sg], sh_rc2[sg],
+                                                        sh_vdr2[sg], sh_pm[sg], sh_lt[sg], sh_tab,
+                                                        ptoff, L, qunit, fx, fy, fz, acc,
+                                                        E.st[T.rep].ibox);
+      else if (asym && lds)
         force_atom<TPA, FULL, true, false, true>(R, i, sub, sh_shift[sg], sh_rc2[sg], sh_vdr2[sg],
                                                  sh_pm[sg], sh_lt[sg], sh_tab, ptoff, L, qunit, fx, fy, fz, acc);
       else if (asym)
@@ -631,8 +660,10 @@ __global__ __launch_bounds__(kTile) void k_drift_fp64(EngineDev E, int t0, int n
 // compute0 in FP64 on the full list: the reference's minimum image, d2 order, single-
 // precision table index and FP64 interpolation (src/nd.f90:283-330).  Same walk as the
 // mixed kernel (U entries in flight per lane, cutoff as a select), one atom per lane.
-template <bool FULL>
+template <bool FULL, bool SPR>
 __global__ __launch_bounds__(kTile) void k_force_fp64(EngineDev E, int t0, int nt, int xflags) {
+  // SPR: the instance launched when some replica has spring pairs (its registers stay out
+  // of the common instance)
   __shared__ double sh_red[8 * 2];
   constexpr int U = kUnroll;
   int tb, te, ts;
@@ -683,6 +714,9 @@ __global__ __launch_bounds__(kTile) void k_force_fp64(EngineDev E, int t0, int n
             feel = pji > 0;
             bal = pij == pji;
           }
+          // list entries past the end read the atom itself, whose type may have no pair
+          // with itself (id 0): keep the table reads in bounds (the entry is masked anyway)
+          pid = pid < 0 ? 0 : pid;
           double s0 = xi - qx[u], s1 = yi - qy[u], s2 = zi - qz[u];
           s0 = s0 - round(s0);
           s1 = s1 - round(s1);
@@ -691,20 +725,44 @@ __global__ __launch_bounds__(kTile) void k_force_fp64(EngineDev E, int t0, int n
           double d2 = r0 * r0;
           d2 = d2 + r1 * r1;
           d2 = d2 + r2 * r2;
-          const bool ok = (c + u < n) && !(d2 > R.rc2d[pid]) && feel;
-          const float rk = (float)(d2 - R.ptrmin) * R.vdr2[pid] + 1.0f;
-          int k = (int)rk;
-          k = k < 1 ? 1 : (k > kmax ? kmax : k);
-          const double w = (double)rk - (double)k;
-          const double *Ft = R.tabF64 + (size_t)pid * tab;
-          double df = w * Ft[k] + (1.0 - w) * Ft[k - 1];
-          df = ok ? df / d2 : 0.0;
+          double df, dp = 0.0;
+          if (SPR && ((R.spring_mask >> pid) & 1)) {
+            // compute2/3 spring (src/nd.f90:622-660): strain against the initial positions,
+            // no distance cutoff, nothing past the strain limit PairParams(3)
+            const int jj = j[u];
+            double u0 = R.ipx[i] - R.ipx[jj], u1 = R.ipy[i] - R.ipy[jj], u2 = R.ipz[i] - R.ipz[jj];
+            u0 = u0 - round(u0);
+            u1 = u1 - round(u1);
+            u2 = u2 - round(u2);
+            const double q0 = S.ibox[0] * u0, q1 = S.ibox[1] * u1, q2 = S.ibox[2] * u2;
+            double d20 = q0 * q0;
+            d20 = d20 + q1 * q1;
+            d20 = d20 + q2 * q2;
+            const double rr = sqrt(d2), rr0 = sqrt(d20);
+            const double sij = rr / rr0 - 1.0;
+            const bool on = (c + u < n) && feel && !(sij > R.spr_s0[pid]);
+            const double Vi = R.spr_vi[pid];
+            const double fs = R.spr_c[pid] * sij * Vi * Vi;
+            if (FULL) dp = on && bal ? 0.5 * fs * sij * rr0 : 0.0;
+            df = on ? -fs / rr : 0.0;
+          } else {
+            const bool ok = (c + u < n) && !(d2 > R.rc2d[pid]) && feel;
+            const float rk = (float)(d2 - R.ptrmin) * R.vdr2[pid] + 1.0f;
+            int k = (int)rk;
+            k = k < 1 ? 1 : (k > kmax ? kmax : k);
+            const double w = (double)rk - (double)k;
+            const double *Ft = R.tabF64 + (size_t)pid * tab;
+            df = w * Ft[k] + (1.0 - w) * Ft[k - 1];
+            df = ok ? df / d2 : 0.0;
+            if (FULL) {
+              const double *Pt = R.tabP64 + (size_t)pid * tab;
+              dp = ok && bal ? w * Pt[k] + (1.0 - w) * Pt[k - 1] : 0.0;
+            }
+          }
           fx = fx + r0 * df;
           fy = fy + r1 * df;
           fz = fz + r2 * df;
           if (FULL) {
-            const double *Pt = R.tabP64 + (size_t)pid * tab;
-            const double dp = ok && bal ? w * Pt[k] + (1.0 - w) * Pt[k - 1] : 0.0;
             const double dv = bal ? df : 0.0;
             acc[0] += 0.5 * dp;
             acc[1] += 0.5 * (r0 * r0 * dv);
@@ -1014,6 +1072,11 @@ __device__ inline void permute_atom(const RepParams &R, int src, int dst, int pa
     R.tmpf[dst] = R.f32a[src];
     R.tmpf[R.np + dst] = R.f32b[src];
   }
+  if (R.spring_mask) {
+    R.tmpd[9 * R.np + dst] = R.ipx[src];
+    R.tmpd[10 * R.np + dst] = R.ipy[src];
+    R.tmpd[11 * R.np + dst] = R.ipz[src];
+  }
   R.tmpi[dst] = R.typ[src];
   R.tmpi[R.np + dst] = R.oid[src];
 }
@@ -1027,6 +1090,11 @@ __device__ inline void place_atom(const RepParams &R, const RepState &S, int p) 
   const int ty = R.tmpi[p];
   R.typ[p] = ty;
   R.oid[p] = R.tmpi[R.np + p];
+  if (R.spring_mask) {
+    R.ipx[p] = R.tmpd[9 * R.np + p];
+    R.ipy[p] = R.tmpd[10 * R.np + p];
+    R.ipz[p] = R.tmpd[11 * R.np + p];
+  }
   if (R.path == NDG_PATH_FP64) {
     R.ax[p] = R.tmpd[6 * R.np + p];
     R.ay[p] = R.tmpd[7 * R.np + p];
@@ -1816,20 +1884,29 @@ static void launch_force_tpa(const EngineDev &E, bool full, int t0, int nt, int 
   const int nblk = (nt + SH::kNSub - 1) / SH::kNSub;
   const int g = grid_for(nblk, 2);
   const size_t lds = sizeof(float) * (size_t)E.tabset_n * (full && E.tabset_p ? 2 : 1);
-  if (full)
-    hipLaunchKernelGGL((k_force_mixed<TPA, true>), dim3(g), dim3(SH::kThreads), lds, s, E, t0, nt, xflags, kick);
+  if (full && E.springs)
+    hipLaunchKernelGGL((k_force_mixed<TPA, true, true>), dim3(g), dim3(SH::kThreads), lds, s, E, t0, nt, xflags, kick);
+  else if (full)
+    hipLaunchKernelGGL((k_force_mixed<TPA, true, false>), dim3(g), dim3(SH::kThreads), lds, s, E, t0, nt, xflags, kick);
+  else if (E.springs)
+    hipLaunchKernelGGL((k_force_mixed<TPA, false, true>), dim3(g), dim3(SH::kThreads), lds, s, E, t0, nt, xflags, kick);
   else
-    hipLaunchKernelGGL((k_force_mixed<TPA, false>), dim3(g), dim3(SH::kThreads), lds, s, E, t0, nt, xflags, kick);
+    hipLaunchKernelGGL((k_force_mixed<TPA, false, false>), dim3(g), dim3(SH::kThreads), lds, s, E, t0, nt, xflags, kick);
 }
 
 void launch_force(const EngineDev &E, int path, int tpa, bool full, int t0, int nt, int xflags,
                   hipStream_t s, int kick) {
   if (nt <= 0) return;
   if (path == NDG_PATH_FP64) {
-    if (full)
-      hipLaunchKernelGGL(k_force_fp64<true>, dim3(grid_for(nt, 8)), dim3(kTile), 0, s, E, t0, nt, xflags);
+    const dim3 g(grid_for(nt, 8));
+    if (full && E.springs)
+      hipLaunchKernelGGL((k_force_fp64<true, true>), g, dim3(kTile), 0, s, E, t0, nt, xflags);
+    else if (full)
+      hipLaunchKernelGGL((k_force_fp64<true, false>), g, dim3(kTile), 0, s, E, t0, nt, xflags);
+    else if (E.springs)
+      hipLaunchKernelGGL((k_force_fp64<false, true>), g, dim3(kTile), 0, s, E, t0, nt, xflags);
     else
-      hipLaunchKernelGGL(k_force_fp64<false>, dim3(grid_for(nt, 8)), dim3(kTile), 0, s, E, t0, nt, xflags);
+      hipLaunchKernelGGL((k_force_fp64<false, false>), g, dim3(kTile), 0, s, E, t0, nt, xflags);
     return;
   }
   switch (tpa) {

@@ -75,6 +75,9 @@ def load_library():
         "ndg_create": [P(V), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
         "ndg_set_tables": [V, dp, dp, dp, dp, C.c_double, ip, dp, dp],
         "ndg_set_state": [V, dp, V, V, ip, dp],
+        "ndg_set_pair_kinds": [V, ip],
+        "ndg_init_positions": [V],
+        "ndg_set_initial_positions": [V, dp, dp],
         "ndg_get_state": [V, V, V, V, V, V, V],
         "ndg_set_box": [V, dp],
         "ndg_get_box": [V, dp],
@@ -196,12 +199,26 @@ class Simulation:
                                 np.ascontiguousarray(system.pairmat.T).ravel().astype(np.int32),
                                 np.ascontiguousarray(system.params, dtype=np.float64).ravel(),
                                 np.ascontiguousarray(system.masses, dtype=np.float64)))
+        if system.springs:
+            _check(L.ndg_set_pair_kinds(h, np.ascontiguousarray(system.kinds, dtype=np.int32)))
         pos = np.ascontiguousarray(system.pos, dtype=np.float64)
         vel = np.ascontiguousarray(system.vel, dtype=np.float64)
         a = None if acc is None else np.ascontiguousarray(acc, dtype=np.float64)
         self._keep = (pos, vel, a)
         _check(L.ndg_set_state(h, pos, _ptr(vel), _ptr(a), system.typ.astype(np.int32),
                                np.ascontiguousarray(system.box, dtype=np.float64)))
+        if system.springs and system.ipos is not None:
+            self.set_initial_positions(system.ipos, system.init_box)
+
+    def set_initial_positions(self, ipos, init_box):
+        """ipos (np,3 scaled) and initBox of the spring pairs (ndg_set_initial_positions)."""
+        _check(load_library().ndg_set_initial_positions(
+            self.h, np.ascontiguousarray(ipos, dtype=np.float64),
+            np.ascontiguousarray(init_box, dtype=np.float64)))
+
+    def init_positions(self):
+        """initPositionSet: the current positions and box become the initial ones."""
+        _check(load_library().ndg_init_positions(self.h))
 
     def close(self):
         if getattr(self, "h", None):

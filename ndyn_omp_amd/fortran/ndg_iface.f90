@@ -12,7 +12,8 @@ module ndg_iface
                                NDG_ERR_HIP = 3, NDG_ERR_ARG = 4, NDG_ERR_NOMEM = 5
   integer(c_int), parameter :: NDG_PATH_FP64 = 0, NDG_PATH_MIXED = 1
   integer(c_int), parameter :: NDG_TRIGGER_COMPAT = 0, NDG_TRIGGER_SKIN = 1
-  integer(c_int), parameter :: NDG_MORSE = 1, NDG_BUCK = 2, NDG_LJ = 3, NDG_MORSE_PHYS = 4
+  integer(c_int), parameter :: NDG_MORSE = 1, NDG_BUCK = 2, NDG_LJ = 3, NDG_MORSE_PHYS = 4, &
+                                NDG_SPRING = 5
   integer(c_int), parameter :: NDG_FRAME_XYZ = 1, NDG_FRAME_DUMP = 2
 
   type, bind(C) :: ndg_run_params
@@ -67,6 +68,24 @@ module ndg_iface
       real(c_double), intent(in) :: P(*), F(*), rc2(*), dr2(*), pairparams(*), mass(*)
       real(c_double), value :: ptrmin
       integer(c_int), intent(in) :: pairmat(*)
+    end function
+
+    integer(c_int) function ndg_set_pair_kinds(h, kinds) bind(C, name='ndg_set_pair_kinds')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), intent(in) :: kinds(*)
+    end function
+
+    integer(c_int) function ndg_init_positions(h) bind(C, name='ndg_init_positions')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+
+    integer(c_int) function ndg_set_initial_positions(h, ipos, ibox) &
+        bind(C, name='ndg_set_initial_positions')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: ipos(*), ibox(3)
     end function
 
     integer(c_int) function ndg_set_state(h, pos, vel, acc, typ, box) &

@@ -12,9 +12,9 @@
 ! 'ListTrigger compat|skin' the neighbour-list rebuild criterion, and
 ! 'FrameWriter engine|fortran' who formats the trajectory frames: the library's writer
 ! thread (default; same bytes, formatted while the device keeps stepping) or write_model.
-! Asymmetric pairs ('asym', the reference's compute1) run on the GPU as well.
-! Group/Hold/Disp/ChangeType/InitPos and spring pairs (compute2-3) are not part of this
-! engine; they are reported and ignored.
+! Asymmetric pairs ('asym', the reference's compute1), spring pairs (compute2/3) and
+! InitPos run on the GPU as well.  Group/Hold/Disp/ChangeType are not part of this engine;
+! they are reported and ignored.
 program ndyn_gpu
   use iso_c_binding
   use ndg_iface
@@ -39,6 +39,7 @@ program ndyn_gpu
   integer :: path = NDG_PATH_MIXED, trigger = NDG_TRIGGER_COMPAT
   ! frames formatted by the library's writer thread (engine) or by write_model (fortran)
   logical :: engine_frames = .true.
+  logical :: initpos_pending = .false.
   ! engine handle
   type(c_ptr) :: h = c_null_ptr
   logical :: tables_dirty = .true., state_dirty = .true., have_model = .false.
@@ -172,7 +173,13 @@ program ndyn_gpu
       read(w(2),*) step_num
       if (nw >= 3) read(w(3),*) dt
       call dynamics
-    case ('Group', 'ClearGroups', 'Hold', 'ClearHolds', 'ChangeType', 'Disp', 'InitPos', &
+    case ('InitPos')
+      ! initPositionSet (src/read_input.f90:165, src/nd.f90:1731-1763), applied to the
+      ! state the next Run starts from
+      if (allocated(pairkind)) then
+        if (any(pairkind == NDG_SPRING)) initpos_pending = .true.
+      endif
+    case ('Group', 'ClearGroups', 'Hold', 'ClearHolds', 'ChangeType', 'Disp', &
           'GroupBufferSize', 'HoldBufferSize')
       write(0,'(A)') '#  read_input: WARNING: directive not supported by this engine: '//trim(w(1))
     case ('END', 'DONE', 'end', 'done')
@@ -384,6 +391,8 @@ contains
         pairkind(k) = NDG_BUCK
       case ('LJ')
         pairkind(k) = NDG_LJ
+      case ('spring')
+        pairkind(k) = NDG_SPRING
       case default
         call die('ERROR: No potential available for Potential Type: '//trim(w(4)))
       end select
@@ -472,11 +481,16 @@ contains
     endif
     if (tables_dirty) then
       call check(ndg_set_tables(h, ptab, ftab, rc2, dr2, ptrmin, pairmat, pairparams, mass))
+      call check(ndg_set_pair_kinds(h, pairkind))
       tables_dirty = .false.
     endif
     if (state_dirty) then
       call check(ndg_set_state(h, pos, vel, acc, typ, box))
       state_dirty = .false.
+    endif
+    if (initpos_pending) then
+      call check(ndg_init_positions(h))
+      initpos_pending = .false.
     endif
     rcut = maxval(pairparams(1,:))
     sk = skin
@@ -590,7 +604,7 @@ contains
     write(6,'(A)') ' Isoenergetic Anisoenergetic Isobaric Anisobaric BoxVel StopBoxVel Exec Print'
     write(6,'(A)') ' OutFreq StepNumber ConfOutFreq Run END; engine: Precision mixed|fp64,'
     write(6,'(A)') ' ListTrigger compat|skin, FrameWriter engine|fortran.'
-    write(6,'(A)') ' Pair kinds: morse morse_phys buck LJ (sym or asym).'
+    write(6,'(A)') ' Pair kinds: morse morse_phys buck LJ spring (sym or asym).'
     call flush(6)
   end subroutine help
 
