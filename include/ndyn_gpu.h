@@ -119,6 +119,16 @@ int ndg_init_positions(ndg_t *h);
 /* The module state ipos(3,np) (scaled) and initBox(3) (src/mods.f90:5,7) set directly. */
 int ndg_set_initial_positions(ndg_t *h, const double *ipos, const double ibox[3]);
 
+/* The hold groups of the Hold/Disp directives (src/read_input.f90:177-190, module
+ * groupStuff src/mods.f90:11-17): hold[np] = hold group of each atom (0: none, caller
+ * order); for groups g = 1..nhold, bc_disp[g-1] != 0 when bcTyp is 'd', bc_dir[3(g-1)+a]
+ * != 0 when axis a is constrained, bc_parm[3(g-1)+a] its scaled displacement per step.
+ * integrate1 then moves those atoms by bc_parm instead of integrating them
+ * (src/nd.f90:1048-1075); their velocities still follow the forces.  At most 16 groups
+ * with a displacement condition.  nhold = 0 clears them.  Set before the Run begins. */
+int ndg_set_holds(ndg_t *h, const int *hold, int nhold, const int *bc_disp, const int *bc_dir,
+                  const double *bc_parm);
+
 /* Upload state: pos (scaled), vel, acc (may be NULL = 0, as read_model sets,
  * src/read_input.f90:354-355), typ (1-based), box (A). */
 int ndg_set_state(ndg_t *h, const double *pos, const double *vel, const double *acc,

@@ -167,6 +167,11 @@ struct HostState {      // caller-provided data for one replica (caller order)
   std::vector<double> ipos;
   double ibox[3] = {0, 0, 0};
   bool have_ipos = false;
+  // displacement boundary conditions: per atom (caller order) the condition index + 1
+  // (0: none) into bc_dir (bit a: axis a constrained) / bc_du (3 per condition)
+  std::vector<int> hbc, bc_dir;
+  std::vector<double> bc_du;
+  bool holds() const { return !bc_dir.empty(); }
   bool springs() const {
     for (int k : kinds)
       if (k == NDG_SPRING) return true;
@@ -178,6 +183,8 @@ struct Slot {
   bool used = false;
   bool springs = false;     // the block holds initial positions (spring pairs)
   bool ipos_dirty = false;  // host initial positions newer than the device copy
+  bool holds = false;       // the block holds per-atom boundary conditions
+  bool hbc_dirty = false;   // host boundary conditions newer than the device copy
   int np = 0, ntypes = 0, npairs = 0, ptsize = 0, neimax = 0, tpa = 1, ncell_cap = 0;
   int ntiles = 0, tile_base = 0;
   void *block = nullptr;
@@ -337,6 +344,7 @@ static size_t carve(Slot &s, int path, char *block) {
   const size_t nip = s.springs ? (size_t)np : 0;
   size_t o_ip[3];
   for (int q = 0; q < 3; ++q) o_ip[q] = L.add(sizeof(double) * nip);  // ipx..ipz (springs)
+  size_t o_hbc = L.add(sizeof(int) * (s.holds ? (size_t)np : 0));
   size_t o_f32a = L.add(sizeof(float4) * np);
   size_t o_f32b = L.add(sizeof(float4) * np);
   size_t o_x64a = L.add(sizeof(double4) * np);
@@ -354,7 +362,7 @@ static size_t carve(Slot &s, int path, char *block) {
   size_t o_src = L.add(sizeof(int) * np);
   size_t o_tmpd = L.add(sizeof(double) * (s.springs ? 12 : 9) * (size_t)np);
   size_t o_tmpf = L.add(sizeof(float4) * 2 * (size_t)np);
-  size_t o_tmpi = L.add(sizeof(int) * 2 * (size_t)np);
+  size_t o_tmpi = L.add(sizeof(int) * (s.holds ? 3 : 2) * (size_t)np);
   size_t o_part = L.add(sizeof(double) * kPartials * ntiles);
   size_t o_ring = L.add(sizeof(ndg_thermo) * kRing);
   if (block) {
@@ -371,6 +379,7 @@ static size_t carve(Slot &s, int path, char *block) {
     P.ipx = s.springs ? (double *)(block + o_ip[0]) : nullptr;
     P.ipy = s.springs ? (double *)(block + o_ip[1]) : nullptr;
     P.ipz = s.springs ? (double *)(block + o_ip[2]) : nullptr;
+    P.hbc = s.holds ? (int *)(block + o_hbc) : nullptr;
     P.f32a = (float4 *)(block + o_f32a);
     P.f32b = (float4 *)(block + o_f32b);
     P.x64a = (double4 *)(block + o_x64a);
@@ -742,6 +751,12 @@ static int upload_host(ndg_engine *e, Slot &s, const HostState &h, bool state_to
   for (int a = 0; a < h.ntypes; ++a)
     for (int b = 0; b < h.ntypes; ++b)
       if (h.pairmat[a + b * h.ntypes] != h.pairmat[b + a * h.ntypes]) P.asym = 1;
+  // displacement boundary conditions
+  P.nbc = P.hbc ? (int)h.bc_dir.size() : 0;
+  for (int b = 0; b < P.nbc; ++b) {
+    P.bc_dir[b] = h.bc_dir[b];
+    for (int a = 0; a < 3; ++a) P.bc_du[b][a] = h.bc_du[3 * b + a];
+  }
   // spring pairs (compute2/3): C, strain limit and Vi from PairParams(2:4)
   P.spring_mask = 0;
   for (int p = 0; p < (int)h.kinds.size() && p < 16; ++p) {
@@ -775,10 +790,16 @@ static int upload_host(ndg_engine *e, Slot &s, const HostState &h, bool state_to
     int rc;
     if ((rc = put3(h.pos, P.px, P.py, P.pz))) return rc;
     if ((rc = put3(h.vel, P.vx, P.vy, P.vz))) return rc;
-    // the state goes up in caller order, so the initial positions go up as given
+    // the state goes up in caller order, so the initial positions and boundary conditions
+    // go up as given
     if (P.ipx && h.have_ipos) {
       if ((rc = put3(h.ipos, P.ipx, P.ipy, P.ipz))) return rc;
       s.ipos_dirty = false;
+    }
+    if (P.hbc) {
+      HIPCHK(hipMemcpyAsync(P.hbc, h.hbc.data(), 4 * (size_t)np, hipMemcpyHostToDevice, st));
+      HIPCHK(hipStreamSynchronize(st));
+      s.hbc_dirty = false;
     }
     if ((rc = put3(h.acc, P.ax, P.ay, P.az))) return rc;
     // mixed path keeps the latest force (F = m a) in FP32
@@ -887,6 +908,7 @@ static int migrate_block(ndg_engine *e, const RepParams &o, const RepParams &n, 
   void *dst[] = {n.px, n.py, n.pz, n.vx, n.vy, n.vz, n.ax, n.ay, n.az, n.fx, n.fy, n.fz,
                  n.pene, n.pbx, n.pby, n.pbz};
   for (int q = 0; q < 16; ++q) HIPCHK(hipMemcpyAsync(dst[q], src[q], d, hipMemcpyDeviceToDevice, st));
+  if (o.hbc && n.hbc) HIPCHK(hipMemcpyAsync(n.hbc, o.hbc, 4 * (size_t)np, hipMemcpyDeviceToDevice, st));
   if (o.ipx && n.ipx) {
     HIPCHK(hipMemcpyAsync(n.ipx, o.ipx, d, hipMemcpyDeviceToDevice, st));
     HIPCHK(hipMemcpyAsync(n.ipy, o.ipy, d, hipMemcpyDeviceToDevice, st));
@@ -1437,6 +1459,20 @@ static int upload_ipos_sorted(ndg_engine *e, Slot &s, const HostState &h) {
   return NDG_OK;
 }
 
+// Boundary conditions (caller order in h.hbc) to the device in the slot's sorted order.
+static int upload_hbc_sorted(ndg_engine *e, Slot &s, const HostState &h) {
+  const int np = s.np;
+  hipStream_t st = e->stream;
+  std::vector<int> oid(np), v(np);
+  HIPCHK(hipMemcpyAsync(oid.data(), s.P.oid, 4 * (size_t)np, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int i = 0; i < np; ++i) v[i] = h.hbc[oid[i]];
+  HIPCHK(hipMemcpyAsync(s.P.hbc, v.data(), 4 * (size_t)np, hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  s.hbc_dirty = false;
+  return NDG_OK;
+}
+
 // Make slot r resident for a Run: allocate for the list capacity, upload, build lists.
 static int begin_run(ndg_engine *e, int r, HostState &h, const ndg_run_params &rp, int step0,
                      int neimax_req, bool fresh_state) {
@@ -1480,8 +1516,9 @@ static int begin_run(ndg_engine *e, int r, HostState &h, const ndg_run_params &r
     h.have_ipos = true;
     s.ipos_dirty = true;
   }
+  const bool holds = h.holds();
   const bool need_alloc = !s.block || neimax > s.neimax || ncap > s.ncell_cap || tpa != s.tpa ||
-                          springs != s.springs;
+                          springs != s.springs || holds != s.holds;
   int rc;
   if (need_alloc) {
     // keep the state across the reallocation when it is already on the device
@@ -1502,6 +1539,7 @@ static int begin_run(ndg_engine *e, int r, HostState &h, const ndg_run_params &r
     s.npairs = h.npairs;
     s.ptsize = h.ptsize;
     s.springs = springs;
+    s.holds = holds;
     if ((rc = alloc_slot(e, r, neimax, ncap, tpa))) return rc;
     if ((rc = ensure_capacity(e))) return rc;
   }
@@ -1513,6 +1551,7 @@ static int begin_run(ndg_engine *e, int r, HostState &h, const ndg_run_params &r
   if ((rc = upload_host(e, s, h, fresh_state))) return rc;
   if ((rc = upload_params(e, r))) return rc;
   if (springs && s.ipos_dirty && (rc = upload_ipos_sorted(e, s, h))) return rc;
+  if (holds && s.hbc_dirty && (rc = upload_hbc_sorted(e, s, h))) return rc;
   RepState S;
   if (fresh_state) {
     memset(&S, 0, sizeof(S));
@@ -1719,6 +1758,8 @@ extern "C" int ndg_engine_list_stats(ndg_engine_t *e, double *mean_nbr, long lon
 
 static int settle(ndg_t *h);
 static void frame_writer_stop(ndg_t *h);
+static int upload_host(ndg_engine *e, Slot &s, const HostState &h, bool state_too);
+static int upload_hbc_sorted(ndg_engine *e, Slot &s, const HostState &h);
 
 extern "C" const char *ndg_last_error(void) { return g_err.c_str(); }
 
@@ -1807,6 +1848,48 @@ static int put_initial_positions(ndg_t *h, const double *ipos, const double ibox
   if ((rc = get_state_dev(h->e, h->slot, &S))) return rc;
   for (int a = 0; a < 3; ++a) S.ibox[a] = ibox[a];
   return put_state_dev(h->e, h->slot, &S);
+}
+
+extern "C" int ndg_set_holds(ndg_t *h, const int *hold, int nhold, const int *bc_disp,
+                             const int *bc_dir, const double *bc_parm) {
+  if (!h || nhold < 0 || (nhold > 0 && (!hold || !bc_disp || !bc_dir || !bc_parm)))
+    return fail(NDG_ERR_ARG, "ndg_set_holds: bad arguments");
+  int rc = settle(h);
+  if (rc) return rc;
+  HostState &hs = h->hs;
+  // compact the hold groups with a displacement condition on some axis into the
+  // device table; atoms of other groups integrate freely (src/nd.f90:1058-1064)
+  std::vector<int> index(nhold + 1, 0), dir;
+  std::vector<double> du;
+  for (int g = 1; g <= nhold; ++g) {
+    const int d = (bc_dir[3 * (g - 1)] ? 1 : 0) | (bc_dir[3 * (g - 1) + 1] ? 2 : 0) |
+                  (bc_dir[3 * (g - 1) + 2] ? 4 : 0);
+    if (!bc_disp[g - 1] || !d) continue;
+    if ((int)dir.size() == 16)
+      return fail(NDG_ERR_ARG, "ndg_set_holds: at most 16 hold groups with Disp");
+    dir.push_back(d);
+    for (int a = 0; a < 3; ++a) du.push_back(bc_parm[3 * (g - 1) + a]);
+    index[g] = (int)dir.size();
+  }
+  std::vector<int> hbc(hs.np, 0);
+  if (!dir.empty())
+    for (int i = 0; i < hs.np; ++i) {
+      const int g = hold[i];
+      if (g < 0 || g > nhold) return fail(NDG_ERR_ARG, "ndg_set_holds: hold id out of range");
+      hbc[i] = index[g];
+    }
+  hs.hbc.swap(hbc);
+  hs.bc_dir.swap(dir);
+  hs.bc_du.swap(du);
+  if (!hs.holds()) hs.hbc.clear();
+  Slot &s = h->e->slots[h->slot];
+  s.hbc_dirty = true;
+  // resident with room for them (or cleared): apply now; a resident block without room
+  // is reallocated when the next Run begins
+  if (!s.block || (hs.holds() && !s.holds)) return NDG_OK;
+  if (hs.holds() && (rc = upload_hbc_sorted(h->e, s, hs))) return rc;
+  if ((rc = upload_host(h->e, s, hs, false))) return rc;
+  return upload_params(h->e, h->slot);
 }
 
 extern "C" int ndg_set_initial_positions(ndg_t *h, const double *ipos, const double ibox[3]) {

@@ -66,6 +66,12 @@ struct RepParams {
   const float2 *tabP2;      // same for the potential
   int ltab[16];             // offset of each pair's FP32 F table in the engine's LDS set, -1: none
   double spr_c[16], spr_s0[16], spr_vi[16];  // spring pairs: PairParams(2), (3), (4)
+  // displacement boundary conditions (Hold + Disp, src/nd.f90:1048-1075): atom i with
+  // hbc[i] = b > 0 moves by bc_du[b-1][a] (scaled, per step) along each axis a set in
+  // bc_dir[b-1] (bit a) instead of integrating
+  int nbc;
+  int bc_dir[16];
+  double bc_du[16][3];
   // per-atom state (sorted order)
   double *px, *py, *pz, *vx, *vy, *vz;
   double *ax, *ay, *az;     // FP64 path acceleration
@@ -75,6 +81,7 @@ struct RepParams {
   double4 *x64a;            // mixed path cell-anchored coordinates (xyz, A) + type bits (w)
   float4 *x32b;             // FP32 copy of x64a at the last build (displacement check)
   double *ipx, *ipy, *ipz;  // spring pairs: initial positions (scaled) of initPositionSet
+  int *hbc;                 // displacement boundary condition of each atom (0: none)
   uint4 *xq;                // 16-byte copy of x64a gathered by the mixed force kernel
   int4 *anchor;             // integer anchor per axis (image*M + cell) and linear cell (w)
   int *typ, *oid, *nnb;
@@ -84,7 +91,7 @@ struct RepParams {
   int *key, *slot_src;
   double *tmpd;             // 9 * np (12 * np with spring pairs)
   float4 *tmpf;             // 2 * np (both force buffers)
-  int *tmpi;                // 2 * np
+  int *tmpi;                // 2 * np (3 * np with boundary conditions)
   double *partials;         // kPartials * ntiles, quantity-major
   ndg_thermo *ring;         // kRing
 };

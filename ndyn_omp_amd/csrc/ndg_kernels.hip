@@ -169,6 +169,19 @@ __device__ inline void bin_axis(double p, int m, int brute, int &k, int &c) {
 
 // integrate1 (src/nd.f90:1029-1046) with the pending thermostat scale of the previous
 // step folded in; writes the packed cell-anchored copy (xq) and the displacement top-2.
+// integrate1's displacement conditions (src/nd.f90:1055-1064): a held atom whose hold
+// group has Disp moves by the prescribed scaled displacement along the constrained axes
+__device__ inline void hold_displacement(const RepParams &R, int i, double &du0, double &du1,
+                                         double &du2) {
+  const int b = R.hbc[i];
+  if (b > 0) {
+    const int d = R.bc_dir[b - 1];
+    if (d & 1) du0 = R.bc_du[b - 1][0];
+    if (d & 2) du1 = R.bc_du[b - 1][1];
+    if (d & 4) du2 = R.bc_du[b - 1][2];
+  }
+}
+
 __global__ __launch_bounds__(kTile) void k_drift_mixed(EngineDev E, int t0, int nt, int xflags) {
   __shared__ double sh[8];
   int tb, te, ts;
@@ -195,9 +208,11 @@ __global__ __launch_bounds__(kTile) void k_drift_mixed(EngineDev E, int t0, int 
       const double rm = 1.0 / R.mass[ty - 1];
       const double ax = (double)Fo.x * rm, ay = (double)Fo.y * rm, az = (double)Fo.z * rm;
       const double li0 = 1.0 / S.box[0], li1 = 1.0 / S.box[1], li2 = 1.0 / S.box[2];
-      double px = R.px[i] + li0 * (vx * dt + 0.5 * ax * dt * dt);
-      double py = R.py[i] + li1 * (vy * dt + 0.5 * ay * dt * dt);
-      double pz = R.pz[i] + li2 * (vz * dt + 0.5 * az * dt * dt);
+      double du0 = li0 * (vx * dt + 0.5 * ax * dt * dt);
+      double du1 = li1 * (vy * dt + 0.5 * ay * dt * dt);
+      double du2 = li2 * (vz * dt + 0.5 * az * dt * dt);
+      if (R.nbc) hold_displacement(R, i, du0, du1, du2);
+      const double px = R.px[i] + du0, py = R.py[i] + du1, pz = R.pz[i] + du2;
       R.px[i] = px; R.py[i] = py; R.pz[i] = pz;
       const int4 A = R.anchor[i];
       const int m0 = S.mcell[0], m1 = S.mcell[1], m2 = S.mcell[2];
@@ -635,9 +650,11 @@ __global__ __launch_bounds__(kTile) void k_drift_fp64(EngineDev E, int t0, int n
         R.vx[i] = vx; R.vy[i] = vy; R.vz[i] = vz;
       }
       const double li0 = 1.0 / S.box[0], li1 = 1.0 / S.box[1], li2 = 1.0 / S.box[2];
-      const double px = R.px[i] + li0 * (vx * dt + 0.5 * R.ax[i] * dt * dt);
-      const double py = R.py[i] + li1 * (vy * dt + 0.5 * R.ay[i] * dt * dt);
-      const double pz = R.pz[i] + li2 * (vz * dt + 0.5 * R.az[i] * dt * dt);
+      double du0 = li0 * (vx * dt + 0.5 * R.ax[i] * dt * dt);
+      double du1 = li1 * (vy * dt + 0.5 * R.ay[i] * dt * dt);
+      double du2 = li2 * (vz * dt + 0.5 * R.az[i] * dt * dt);
+      if (R.nbc) hold_displacement(R, i, du0, du1, du2);
+      const double px = R.px[i] + du0, py = R.py[i] + du1, pz = R.pz[i] + du2;
       R.px[i] = px; R.py[i] = py; R.pz[i] = pz;
       const double dx = px - R.pbx[i], dy = py - R.pby[i], dz = pz - R.pbz[i];
       if (R.trigger == NDG_TRIGGER_COMPAT) {
@@ -1077,6 +1094,7 @@ __device__ inline void permute_atom(const RepParams &R, int src, int dst, int pa
     R.tmpd[10 * R.np + dst] = R.ipy[src];
     R.tmpd[11 * R.np + dst] = R.ipz[src];
   }
+  if (R.nbc) R.tmpi[2 * R.np + dst] = R.hbc[src];
   R.tmpi[dst] = R.typ[src];
   R.tmpi[R.np + dst] = R.oid[src];
 }
@@ -1090,6 +1108,7 @@ __device__ inline void place_atom(const RepParams &R, const RepState &S, int p) 
   const int ty = R.tmpi[p];
   R.typ[p] = ty;
   R.oid[p] = R.tmpi[R.np + p];
+  if (R.nbc) R.hbc[p] = R.tmpi[2 * R.np + p];
   if (R.spring_mask) {
     R.ipx[p] = R.tmpd[9 * R.np + p];
     R.ipy[p] = R.tmpd[10 * R.np + p];

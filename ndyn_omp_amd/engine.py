@@ -78,6 +78,7 @@ def load_library():
         "ndg_set_pair_kinds": [V, ip],
         "ndg_init_positions": [V],
         "ndg_set_initial_positions": [V, dp, dp],
+        "ndg_set_holds": [V, V, C.c_int, V, V, V],
         "ndg_get_state": [V, V, V, V, V, V, V],
         "ndg_set_box": [V, dp],
         "ndg_get_box": [V, dp],
@@ -215,6 +216,17 @@ class Simulation:
         _check(load_library().ndg_set_initial_positions(
             self.h, np.ascontiguousarray(ipos, dtype=np.float64),
             np.ascontiguousarray(init_box, dtype=np.float64)))
+
+    def set_holds(self, hold, bc_disp, bc_dir, bc_parm):
+        """Hold/Disp boundary conditions (ndg_set_holds): hold group per atom (0: none) and,
+        per group, bcTyp 'd' flag, constrained axes and scaled displacement per step."""
+        hk = [np.ascontiguousarray(hold, dtype=np.int32),
+              np.ascontiguousarray(bc_disp, dtype=np.int32),
+              np.ascontiguousarray(bc_dir, dtype=np.int32).ravel(),
+              np.ascontiguousarray(bc_parm, dtype=np.float64).ravel()]
+        self._holds = hk
+        _check(load_library().ndg_set_holds(self.h, _ptr(hk[0]), len(hk[1]), _ptr(hk[1]),
+                                            _ptr(hk[2]), _ptr(hk[3])))
 
     def init_positions(self):
         """initPositionSet: the current positions and box become the initial ones."""

@@ -88,6 +88,15 @@ module ndg_iface
       real(c_double), intent(in) :: ipos(*), ibox(3)
     end function
 
+    integer(c_int) function ndg_set_holds(h, hold, nhold, bc_disp, bc_dir, bc_parm) &
+        bind(C, name='ndg_set_holds')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: nhold
+      integer(c_int), intent(in) :: hold(*), bc_disp(*), bc_dir(*)
+      real(c_double), intent(in) :: bc_parm(*)
+    end function
+
     integer(c_int) function ndg_set_state(h, pos, vel, acc, typ, box) &
         bind(C, name='ndg_set_state')
       import :: c_int, c_ptr, c_double

@@ -517,6 +517,25 @@ int ndo_asymmetric(int ntypes, const int *pairmat) {
 
 /* ---------------------------------------------------------------- integrator */
 
+void ndo_integrate1_hold(int np, double *pos, const double *vel, const double *acc,
+                         double *disp, const double box[3], double dt, const int *hold,
+                         const int *bc_disp, const int *bc_dir, const double *bc_parm) {
+  double li[3] = {1.0 / box[0], 1.0 / box[1], 1.0 / box[2]};
+  for (int j = 0; j < np; ++j) {
+    double du[3];
+    for (int a = 0; a < 3; ++a)
+      du[a] = li[a] * (vel[3 * j + a] * dt + 0.5 * acc[3 * j + a] * dt * dt);
+    const int g = hold[j];
+    if (g != 0 && bc_disp[g - 1])
+      for (int a = 0; a < 3; ++a)
+        if (bc_dir[3 * (g - 1) + a]) du[a] = bc_parm[3 * (g - 1) + a];
+    for (int a = 0; a < 3; ++a) {
+      disp[3 * j + a] = disp[3 * j + a] + du[a];
+      pos[3 * j + a] = pos[3 * j + a] + du[a];
+    }
+  }
+}
+
 void ndo_integrate1(int np, double *pos, const double *vel, const double *acc,
                     double *disp, const double box[3], double dt) {
   double li[3] = {1.0 / box[0], 1.0 / box[1], 1.0 / box[2]};
@@ -667,7 +686,11 @@ int ndo_run(const ndo_run_params *rp, int np, int ntypes, int npairs, int ptsize
         if (err) break;
       }
     }
-    ndo_integrate1(np, pos, vel, acc, disp, box, rp->dt);
+    if (rp->nhold > 0)
+      ndo_integrate1_hold(np, pos, vel, acc, disp, box, rp->dt, rp->hold, rp->bc_disp,
+                          rp->bc_dir, rp->bc_parm);
+    else
+      ndo_integrate1(np, pos, vel, acc, disp, box, rp->dt);
     if (rp->springs)
       ndo_compute23(asym, np, pos, typ, box, vnl, neimax, ntypes, pairmat, P, F, rc2, dr2,
                     ptrmin, ptsize, spring, pairparams, rp->ipos, rp->init_box, cv, cu, force,

@@ -83,6 +83,12 @@ void ndo_compute1(int np, const double *pos, const int *typ, const double box[3]
                   double ptrmin, int ptsize, int cv, int cu,
                   double *force, double *pe, double virial[6], double *pene);
 int ndo_asymmetric(int ntypes, const int *pairmat);
+/* integrate1 with held atoms (src/nd.f90:1051-1075): an atom of a hold group with a
+ * displacement condition moves by bc_parm along the constrained axes. */
+void ndo_integrate1_hold(int np, double *pos, const double *vel, const double *acc,
+                         double *disp, const double box[3], double dt, const int *hold,
+                         const int *bc_disp, const int *bc_dir, const double *bc_parm);
+
 /* compute2 (src/nd.f90:531-727, asym = 0) and compute3 (src/nd.f90:728-933, asym = 1):
  * compute0 / compute1 where the pairs flagged in spring[] (npairs entries) take their force
  * from the strain s = r/r0 - 1 against the initial positions ipos (3*np, scaled) in the
@@ -143,6 +149,12 @@ typedef struct {
   double *ipos;
   double init_box[3];
   int ipos_set;
+  /* Hold/Disp boundary conditions (src/nd.f90:1048-1075): hold[np] = hold group of each
+   * atom (0: none); per group g = 1..nhold: bc_disp[g-1] (bcTyp 'd'), bc_dir[3(g-1)+a],
+   * bc_parm[3(g-1)+a] (scaled displacement per step).  nhold = 0: none. */
+  int nhold;
+  const int *hold, *bc_disp, *bc_dir;
+  const double *bc_parm;
 } ndo_run_params;
 
 /* One thermo line (src/read_input.f90:433-446) plus the full virial and box. */

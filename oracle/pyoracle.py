@@ -36,7 +36,8 @@ class RunParams(C.Structure):
                 ("isoenergetic", C.c_int), ("e_tau", C.c_double), ("e_freq", C.c_int),
                 ("box_load", C.c_int), ("box_vel", C.c_double * 3),
                 ("springs", C.c_int), ("ipos", C.c_void_p), ("init_box", C.c_double * 3),
-                ("ipos_set", C.c_int)]
+                ("ipos_set", C.c_int), ("nhold", C.c_int), ("hold", C.c_void_p),
+                ("bc_disp", C.c_void_p), ("bc_dir", C.c_void_p), ("bc_parm", C.c_void_p)]
 
 
 class Thermo(C.Structure):
@@ -233,10 +234,11 @@ def _compute(fn, sys, vnl, neimax, tables, cv, cu):
 def run(sys, nsteps, dt, skin=-1.0, out_steps=None, step0=0, neimax=0, renei=True,
         renei_freq=1, trigger=TRIGGER_COMPAT, brute=False, acc=None,
         isokinetic=None, isobaric=None, isoenergetic=None, box_vel=None, tables=None,
-        ipos=None, init_box=None):
+        ipos=None, init_box=None, holds=None):
     """ndo_run on a copy of sys. isokinetic=(ke_target, tau, freq); isobaric=((px,py,pz),
     tau, freq); isoenergetic=(tau, freq).  With spring pairs, ipos/init_box are the initial
     positions and box (default: pos and box at the start of the Run, as dynamics sets them).
+    holds=(hold[np], bc_disp[nhold], bc_dir[nhold,3], bc_parm[nhold,3]) sets Hold/Disp.
     Returns (state dict, thermo list, err); the state holds ipos/init_box."""
     if tables is None:
         tables = pair_tables(sys.kinds, sys.params, sys.ptrmin, sys.ptsize)
@@ -273,6 +275,13 @@ def run(sys, nsteps, dt, skin=-1.0, out_steps=None, step0=0, neimax=0, renei=Tru
             rp.init_box[a] = init_box[a]
         rp.ipos_set = 1
     rp.ipos = ipos_buf.ctypes.data
+    if holds is not None:
+        hk = [np.ascontiguousarray(holds[0], dtype=np.int32),
+              np.ascontiguousarray(holds[1], dtype=np.int32),
+              np.ascontiguousarray(holds[2], dtype=np.int32).ravel(),
+              np.ascontiguousarray(holds[3], dtype=np.float64).ravel()]
+        rp.nhold = len(hk[1])
+        rp.hold, rp.bc_disp, rp.bc_dir, rp.bc_parm = (a.ctypes.data for a in hk)
     pos = np.ascontiguousarray(sys.pos, dtype=np.float64).copy()
     vel = np.ascontiguousarray(sys.vel, dtype=np.float64).copy()
     accv = np.zeros_like(pos) if acc is None else np.ascontiguousarray(acc, dtype=np.float64).copy()

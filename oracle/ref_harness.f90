@@ -16,6 +16,8 @@
 !   real64 pairparams(4,npairs); int32 typ(np); real64 pos(3,np), vel(3,np), acc(3,np)
 !   int32  boxload, have_ipos; real64 boxvel(3)
 !   (have_ipos /= 0) real64 ipos(3,np), initbox(3)
+!   int32 nhold; (nhold > 0) int32 hold(np), bcdisp(nhold), bcdir(3,nhold);
+!   real64 bcparm(3,nhold)
 ! Pair kinds: 1 morse, 2 buck, 3 LJ, 5 spring.
 ! Modes: 1 tables (buildPairTables), 2 half list (buildCellList),
 !        3 forces (buildCellList + the force routine dynamics would call: compute0,
@@ -31,7 +33,8 @@ program ref_harness
   implicit none
   character(len=1024) :: fin, fout
   integer :: mode, nthr, i
-  integer :: iv(18), iv2(2)
+  integer :: iv(18), iv2(2), nh
+  integer, allocatable :: ibc(:), ibcdir(:,:)
   real(8) :: dv(13)
   integer, allocatable :: kinds(:)
 
@@ -91,12 +94,29 @@ program ref_harness
     read(20) ipos
     read(20) initBox
   endif
+  read(20) nh
   close(20)
   force = 0.d0; pene = 0.d0
   grpBuff = 100; holdBuff = 100; Ngrp = 0; Nhold = 0
   allocate(grp(np), groupName(grpBuff), hold(np), holdName(holdBuff), &
            bcTyp(holdBuff), bcParm(3,holdBuff), bcDir(3,holdBuff))
   grp = 0; hold = 0; bcTyp = 'X'; bcParm = 0.d0; bcDir = .false.
+  if (nh > 0) then
+    ! Hold groups and their Disp conditions (src/read_input.f90:177-190)
+    allocate(ibc(nh), ibcdir(3,nh))
+    open(20, file=trim(fin), access='stream', form='unformatted', status='old')
+    call skip_to_holds()
+    read(20) hold
+    read(20) ibc
+    read(20) ibcdir
+    read(20) bcParm(1:3,1:nh)
+    close(20)
+    Nhold = nh
+    do i = 1, nh
+      if (ibc(i) /= 0) bcTyp(i) = 'd'
+      bcDir(1:3,i) = ibcdir(1:3,i) /= 0
+    enddo
+  endif
   boxLoad = iv2(1) /= 0
   Model = 'NA'; ModelInType = 'NA'; ModelOutType = 'dump'; ConfOuts = 0
   potential = 0.d0; kinetic = 0.d0; virial = 0.d0; e0 = 0.d0
@@ -144,4 +164,14 @@ program ref_harness
     stop 'harness: unknown mode'
   end select
   close(21)
+contains
+  ! position the input at the hold block: after the fixed header, the per-type and per-pair
+  ! arrays, the state, the BoxVel block and the optional initial positions
+  subroutine skip_to_holds()
+    integer(8) :: off
+    off = 4*18 + 8*13 + 8*Ntypes + 4*Ntypes*Ntypes + 4*NPairs + 8*4*NPairs + 4*np + 3*8*3*np &
+          + 4*2 + 8*3 + 4
+    if (iv2(2) /= 0) off = off + 8*3*np + 8*3
+    read(20, pos=off+1)
+  end subroutine skip_to_holds
 end program ref_harness

@@ -107,6 +107,10 @@ def golden_run(fname, s, nsteps, dt, skin, outsteps, **kw):
     res["out_e0"], res["out_nupdates"], res["out_step"] = a[o + 11], a[o + 12], a[o + 13]
     meta = dict(nsteps=np.int64(nsteps), dt=np.float64(dt), skin=np.float64(skin),
                 outsteps=np.int64(outsteps))
+    holds = kw.pop("holds", None)
+    if holds is not None:
+        meta.update(hold=np.asarray(holds[0], np.int32), bc_disp=np.asarray(holds[1], np.int32),
+                    bc_dir=np.asarray(holds[2], np.int32), bc_parm=np.asarray(holds[3], np.float64))
     for k, v in kw.items():
         if v is None or isinstance(v, bool):
             meta[k] = np.array(-1 if v is None else int(v))
@@ -146,6 +150,18 @@ def spring_fixtures():
                60, W.DT_1FS, 0.5, 20)
 
 
+def hold_fixtures():
+    """Hold/Disp boundary conditions (src/nd.f90:1048-1075) on a thermalised Cu block: a
+    slab frozen (Disp 0 0 0), a slab pulled along x (Disp 2e-5 F F), a hold group without
+    Disp (free), the rest free."""
+    s = W.cu_morse(7, temp_k=600.0, seed=12370)
+    x = s.pos[:, 0]
+    hold = np.where(x < -0.35, 1, np.where(x > 0.35, 2, np.where(np.abs(x) < 0.05, 3, 0)))
+    holds = (hold.astype(np.int32), [1, 1, 0], [[1, 1, 1], [1, 0, 0], [1, 1, 1]],
+             [[0.0, 0.0, 0.0], [2.0e-5, 0.0, 0.0], [0.0, 0.0, 0.0]])
+    golden_run("run_hold_cu7.npz", s, 60, W.DT_1FS, 0.5, 20, holds=holds)
+
+
 def main():
     if not os.path.exists(HARNESS):
         raise SystemExit("build the reference harness first: make -C oracle ref")
@@ -154,6 +170,9 @@ def main():
         return
     if "--spring-only" in sys.argv:
         spring_fixtures()
+        return
+    if "--hold-only" in sys.argv:
+        hold_fixtures()
         return
     golden_tables()
     # half lists: Cu 7^3 (MCELL 3), c0-size Cu 10^3 (MCELL 4), Buckingham alloy (3 pairs)
@@ -177,6 +196,7 @@ def main():
                W.DT_1FS, 0.5, 20, iso_e=(50 * W.DT_1FS, 5))
     asym_fixtures()
     spring_fixtures()
+    hold_fixtures()
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f"{f:28s} {os.path.getsize(os.path.join(OUT, f)) / 1024:8.1f} KiB")

@@ -1,0 +1,86 @@
+"""Hold/Disp boundary conditions (integrate1, src/nd.f90:1048-1075) on the device.
+
+Atoms of a hold group with a displacement condition ('Disp', src/read_input.f90:184-190)
+move by the prescribed scaled displacement per step along the constrained axes instead of
+integrating; their velocities still follow the forces.  The per-atom condition travels with
+the atoms through every cell sort.  Bars: FP64 path final positions within 1e-10 (scaled)
+and velocities 1e-8 relative of the compiled reference (golden), mixed path positions
+within 1e-5 A; frozen atoms bit for bit where they started, pulled atoms displaced by
+n * Disp to the rounding of n additions (1e-13 scaled); with list rebuilds (skin trigger)
+the same against the oracle.
+"""
+import numpy as np
+import pytest
+
+from conftest import golden, system_from_golden
+from ndyn_omp_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+G = pytest.importorskip("ndyn_omp_amd.engine")
+
+PATHS = [("fp64", 0), ("mixed", 1)]
+
+
+def _holds(g):
+    return g["hold"], g["bc_disp"], g["bc_dir"], g["bc_parm"]
+
+
+@pytest.mark.parametrize("path_name,path", PATHS)
+def test_hold_dynamics_matches_reference(path_name, path):
+    g = golden("run_hold_cu7.npz")
+    s = system_from_golden(g)
+    print(f"case run_hold_cu7 {path_name}", flush=True)
+    sim = G.Simulation(s, path=path)
+    sim.set_holds(*_holds(g))
+    rp = G.run_params(int(g["nsteps"]), float(g["dt"]), out_steps=int(g["outsteps"]),
+                      skin=float(g["skin"]))
+    recs, nup, step = sim.run(rp)
+    st = sim.state()
+    sim.close()
+    hold = g["hold"]
+    d = st["pos"] - s.pos
+    assert np.abs(d[hold == 1]).max() == 0.0
+    # n additions of Disp to positions of order 0.5: rounding leaves ~n ulp(0.5)
+    np.testing.assert_allclose(d[hold == 2, 0], 60 * 2.0e-5, rtol=0, atol=1e-13)
+    if path == G.PATH_FP64:
+        np.testing.assert_allclose(st["pos"], g["out_pos"], rtol=0, atol=1e-10)
+        np.testing.assert_allclose(st["vel"], g["out_vel"], rtol=1e-8, atol=1e-10)
+    else:
+        assert np.abs((st["pos"] - g["out_pos"]) * g["out_box"]).max() < 1e-5
+    t = g["thermo"]
+    for row, rec in zip(t, recs):
+        assert rec["pe"] == pytest.approx(row[1], rel=1e-5)
+        assert rec["ke"] == pytest.approx(row[2], rel=1e-5)
+
+
+@pytest.mark.parametrize("path_name,path", PATHS)
+def test_hold_through_list_rebuilds(path_name, path, oracle):
+    """Skin trigger at 900 K: the lists (and the atom order) are rebuilt several times; the
+    conditions follow their atoms.  Then the holds are cleared on the resident system and
+    the next Run integrates every atom."""
+    s = W.cu_morse(7, temp_k=900.0, seed=181)
+    x = s.pos[:, 0]
+    hold = np.where(x < -0.3, 1, np.where(x > 0.3, 2, 0)).astype(np.int32)
+    holds = (hold, [1, 1], [[1, 1, 1], [1, 0, 1]], [[0.0, 0.0, 0.0], [3.0e-5, 0.0, -1.0e-5]])
+    print(f"case hold rebuilds {path_name}", flush=True)
+    sim = G.Simulation(s, path=path)
+    sim.set_holds(*holds)
+    rp = G.run_params(150, W.DT_1FS, out_steps=50, skin=0.3, trigger=G.TRIGGER_SKIN)
+    _, nup, _ = sim.run(rp)
+    a = sim.state()
+    sim.set_holds(np.zeros(s.np, np.int32), [], np.zeros((0, 3)), np.zeros((0, 3)))
+    sim.run(G.run_params(50, W.DT_1FS, out_steps=50, skin=0.3, trigger=G.TRIGGER_SKIN),
+            step0=150)
+    b = sim.state()
+    sim.close()
+    assert nup >= 2
+    d = a["pos"] - s.pos
+    assert np.abs(d[hold == 1]).max() == 0.0
+    np.testing.assert_allclose(d[hold == 2, 0], 150 * 3.0e-5, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(d[hold == 2, 2], -150 * 1.0e-5, rtol=0, atol=1e-13)
+    o, _, _ = oracle.run(s, 150, W.DT_1FS, skin=0.3, out_steps=50, trigger=oracle.TRIGGER_SKIN,
+                         holds=holds)
+    tol = 1e-10 if path == G.PATH_FP64 else 1e-6
+    np.testing.assert_allclose(a["pos"], o["pos"], rtol=0, atol=tol)
+    # after clearing, the frozen slab moves
+    assert np.abs(b["pos"][hold == 1] - a["pos"][hold == 1]).max() > 1e-4

@@ -322,7 +322,8 @@ def test_reference_log_known_answers(oracle):
 # ----------------------------------------------------------------- dynamics
 @pytest.mark.parametrize("fname", ["run_cu10_nve.npz", "run_ar6_nvt.npz", "run_cu7_npt.npz",
                                    "run_alloy5_nve.npz", "run_asym2_nve.npz",
-                                   "run_spring_pd.npz", "run_spring_mix1_nve.npz"])
+                                   "run_spring_pd.npz", "run_spring_mix1_nve.npz",
+                                   "run_hold_cu7.npz"])
 def test_dynamics_matches_reference(oracle, fname):
     g = golden(fname)
     s = system_from_golden(g)
@@ -342,6 +343,8 @@ def test_dynamics_matches_reference(oracle, fname):
         kw["renei"] = bool(int(g["renei"]))
     if s.ipos is not None:
         kw.update(ipos=s.ipos, init_box=s.init_box)
+    if "hold" in g.files:
+        kw["holds"] = (g["hold"], g["bc_disp"], g["bc_dir"], g["bc_parm"])
     st, thermo, err = oracle.run(s, int(g["nsteps"]), float(g["dt"]), skin=float(g["skin"]),
                                  out_steps=int(g["outsteps"]), **kw)
     assert err == 0
@@ -357,6 +360,26 @@ def test_dynamics_matches_reference(oracle, fname):
         assert rec["step"] == int(row[0])
         for k, col in (("pe", 1), ("ke", 2), ("volume", 5)):
             assert rec[k] == pytest.approx(row[col], rel=1e-5, abs=1e-9)
+
+
+def test_hold_displacements_exact(oracle):
+    """Hold/Disp (src/nd.f90:1055-1064): a frozen group does not move, a pulled group moves
+    by n * Disp along its constrained axis (the step's scaled displacement replaces the
+    integrated one) and freely along the others; a hold group without Disp and the other
+    atoms integrate as usual."""
+    g = golden("run_hold_cu7.npz")
+    s = system_from_golden(g)
+    hold = g["hold"]
+    st, _, err = oracle.run(s, 60, float(g["dt"]), skin=float(g["skin"]), out_steps=20,
+                            holds=(hold, g["bc_disp"], g["bc_dir"], g["bc_parm"]))
+    assert err == 0
+    d = st["pos"] - s.pos
+    assert np.abs(d[hold == 1]).max() == 0.0
+    np.testing.assert_allclose(d[hold == 2, 0], 60 * 2.0e-5, rtol=0, atol=1e-13)
+    assert np.abs(d[hold == 2, 1:]).max() > 1e-3
+    assert np.abs(d[hold == 3]).max() > 1e-3 and np.abs(d[hold == 0]).max() > 1e-3
+    free, _, _ = oracle.run(s, 60, float(g["dt"]), skin=float(g["skin"]), out_steps=20)
+    assert np.abs(free["pos"][hold == 3] - st["pos"][hold == 3]).max() > 0.0   # coupled
 
 
 def test_testT_barostat_log(oracle):
