@@ -12,9 +12,10 @@
 ! 'ListTrigger compat|skin' the neighbour-list rebuild criterion, and
 ! 'FrameWriter engine|fortran' who formats the trajectory frames: the library's writer
 ! thread (default; same bytes, formatted while the device keeps stepping) or write_model.
-! Asymmetric pairs ('asym', the reference's compute1), spring pairs (compute2/3) and
-! InitPos run on the GPU as well.  Group/Hold/Disp/ChangeType are not part of this engine;
-! they are reported and ignored.
+! Asymmetric pairs ('asym', the reference's compute1), spring pairs (compute2/3), InitPos
+! and the Hold/Disp boundary conditions run on the GPU as well; Group/Hold select atoms by
+! type or geometric domain (src/nd.f90:1625-1730, src/domtype.f90) and ChangeType and
+! Velocity act on groups, on the host between Runs.
 program ndyn_gpu
   use iso_c_binding
   use ndg_iface
@@ -40,6 +41,19 @@ program ndyn_gpu
   ! frames formatted by the library's writer thread (engine) or by write_model (fortran)
   logical :: engine_frames = .true.
   logical :: initpos_pending = .false.
+  ! groups and hold groups (module groupStuff, src/mods.f90:11-17)
+  integer :: ngrp = 0, nhold = 0, grpbuff = 100, holdbuff = 100
+  integer, allocatable :: grp(:), hold(:), bcdisp(:), bcdir(:,:)
+  character(len=8), allocatable :: groupname(:), holdname(:)
+  real(8), allocatable :: bcparm(:,:)
+  logical :: holds_dirty = .false.
+  ! a geometric domain (type domain, src/domtype.f90:10-16)
+  type :: domain_t
+    character(len=3) :: ct = '   '
+    logical :: inside = .true.
+    integer :: axis = 0
+    real(4) :: parm(8) = 0.0
+  end type domain_t
   ! engine handle
   type(c_ptr) :: h = c_null_ptr
   logical :: tables_dirty = .true., state_dirty = .true., have_model = .false.
@@ -88,7 +102,11 @@ program ndyn_gpu
       call random_model
     case ('Velocity')
       read(w(2),*) rtmp
-      call velocities(rtmp)
+      if (nw >= 3) then
+        call velocities(rtmp, trim(w(3)))
+      else
+        call velocities(rtmp, 'N/A')
+      endif
     case ('ReNeibour', 'ReNeighbor')
       read(w(2),*) renei
       if (nw >= 3) read(w(3),*) renei_freq
@@ -179,9 +197,47 @@ program ndyn_gpu
       if (allocated(pairkind)) then
         if (any(pairkind == NDG_SPRING)) initpos_pending = .true.
       endif
-    case ('Group', 'ClearGroups', 'Hold', 'ClearHolds', 'ChangeType', 'Disp', &
-          'GroupBufferSize', 'HoldBufferSize')
-      write(0,'(A)') '#  read_input: WARNING: directive not supported by this engine: '//trim(w(1))
+    case ('GroupBufferSize')
+      read(w(2),*) grpbuff
+      call alloc_groups(.true.)
+    case ('HoldBufferSize')
+      read(w(2),*) holdbuff
+      call alloc_groups(.true.)
+    case ('Group')   ! src/read_input.f90:171-174
+      call alloc_groups(.false.)
+      if (ngrp >= grpbuff) call die('ERROR: more groups than GroupBufferSize')
+      ngrp = ngrp + 1
+      groupname(ngrp) = trim(w(2))
+      call set_group('g', ngrp)
+    case ('ClearGroups')
+      ngrp = 0
+      if (allocated(grp)) grp = 0
+    case ('Hold')     ! src/read_input.f90:177-180
+      call alloc_groups(.false.)
+      if (nhold >= holdbuff) call die('ERROR: more hold groups than HoldBufferSize')
+      nhold = nhold + 1
+      holdname(nhold) = trim(w(2))
+      call set_group('h', nhold)
+      holds_dirty = .true.
+    case ('ClearHolds')
+      ! the Disp settings of the cleared groups stay with their indices, as in the reference
+      nhold = 0
+      if (allocated(hold)) hold = 0
+      holds_dirty = .true.
+    case ('Disp')     ! src/read_input.f90:184-190
+      call alloc_groups(.false.)
+      i = hold_id(w(2))
+      if (i == 0) call die('ERROR: Disp: unknown hold group '//trim(w(2)))
+      bcdisp(i) = 1
+      do j = 1, 3
+        if (trim(w(2+j)) /= 'F') then
+          bcdir(j,i) = 1
+          read(w(2+j),*) bcparm(j,i)
+        endif
+      enddo
+      holds_dirty = .true.
+    case ('ChangeType')   ! setType, src/nd.f90:1700-1730
+      call change_type(w(2), w(3))
     case ('END', 'DONE', 'end', 'done')
       exit
     case ('EXIT', 'exit', 'quit', 'abort')
@@ -269,7 +325,196 @@ contains
     pos = 0.d0; vel = 0.d0; acc = 0.d0; typ = 1; pene = 0.d0; frc = 0.d0
     have_model = .true.
     state_dirty = .true.
+    if (allocated(grp)) deallocate(grp, hold)
+    allocate(grp(np), hold(np))
+    grp = 0; hold = 0
+    ngrp = 0; nhold = 0
+    holds_dirty = .true.
   end subroutine alloc_atoms
+
+  ! group tables (src/read_input.f90:292-296); fresh = .true. reallocates at a new size
+  subroutine alloc_groups(fresh)
+    logical, intent(in) :: fresh
+    if (allocated(groupname) .and. .not. fresh) return
+    if (allocated(groupname)) deallocate(groupname, holdname, bcdisp, bcdir, bcparm)
+    allocate(groupname(grpbuff), holdname(holdbuff), bcdisp(holdbuff), bcdir(3,holdbuff), &
+             bcparm(3,holdbuff))
+    bcdisp = 0; bcdir = 0; bcparm = 0.d0
+    ngrp = min(ngrp, grpbuff); nhold = min(nhold, holdbuff)
+  end subroutine alloc_groups
+
+  integer function group_id(name)
+    character(*), intent(in) :: name
+    integer :: k
+    group_id = 0
+    do k = 1, ngrp
+      if (trim(name) == trim(groupname(k))) then
+        group_id = k
+        return
+      endif
+    enddo
+  end function group_id
+
+  integer function hold_id(name)
+    character(*), intent(in) :: name
+    integer :: k
+    hold_id = 0
+    do k = 1, nhold
+      if (trim(name) == trim(holdname(k))) then
+        hold_id = k
+        return
+      endif
+    enddo
+  end function hold_id
+
+  ! setGroup (src/nd.f90:1625-1699): words 3.. of the directive are 'typ N', 'dom ...' or
+  ! 'all'; g = 'g' sets grp, 'h' sets hold, to id
+  subroutine set_group(g, id)
+    character, intent(in) :: g
+    integer, intent(in) :: id
+    integer :: k, cht
+    type(domain_t) :: dom
+    if (.not. have_model) call die('ERROR: Group/Hold needs a model')
+    select case (w(3)(1:3))
+    case ('typ')
+      read(w(4),*) cht
+      do k = 1, np
+        if (typ(k) == cht) call put(k)
+      enddo
+    case ('dom')
+      call read_domain(dom)
+      do k = 1, np
+        if (in_domain(real(pos(1:3,k)), dom)) call put(k)
+      enddo
+    case ('all')
+      if (g == 'g') grp = 0
+      if (g == 'h') hold = 0
+    case default
+      call die('ERROR: Group/Hold: expected typ, dom or all')
+    end select
+  contains
+    subroutine put(k)
+      integer, intent(in) :: k
+      if (g == 'g') grp(k) = id
+      if (g == 'h') hold(k) = id
+    end subroutine put
+  end subroutine set_group
+
+  ! readDomain (src/domtype.f90:24-86): words 4.. = type, inside flag, [axis,] parameters
+  subroutine read_domain(dom)
+    type(domain_t), intent(out) :: dom
+    integer :: k, np_, first
+    dom%ct = trim(w(4))
+    read(w(5),*) dom%inside
+    select case (dom%ct)
+    case ('rec'); np_ = 6; first = 6
+    case ('per'); np_ = 7; first = 6
+    case ('sph'); np_ = 4; first = 6
+    case ('ell'); np_ = 4; first = 7
+    case ('tub'); np_ = 8; first = 7
+    case ('wed'); np_ = 5; first = 7
+    case default
+      call die('ERROR: unknown domain type '//trim(w(4)))
+    end select
+    if (first == 7) read(w(6),*) dom%axis
+    do k = 1, np_
+      read(w(first+k-1),*) dom%parm(k)
+    enddo
+  end subroutine read_domain
+
+  ! inDomain (src/domtype.f90:278-434), in single precision as the reference evaluates it
+  logical function in_domain(p, dom)
+    real(4), intent(in) :: p(3)
+    type(domain_t), intent(in) :: dom
+    real(4), parameter :: small = tiny(1.0)
+    real(4) :: dmin(3), dmax(3), depth, dist, slope, re, ro, u, v
+    integer :: a, dir(3), i1, i2
+    logical :: good(3)
+    in_domain = .false.
+    select case (dom%ct)
+    case ('rec', 'per')
+      dmin = dom%parm(1:5:2); dmax = dom%parm(2:6:2)
+      dir = 1
+      do a = 1, 3
+        if ((abs(dmax(a)) + abs(dmin(a))) < small) dir(a) = 0
+        good(a) = dir(a) == 0 .or. (p(a) > dmin(a) .and. p(a) <= dmax(a))
+      enddo
+      in_domain = all(good)
+      if (dom%ct == 'per') then
+        depth = dom%parm(7)
+        do a = 1, 3
+          good(a) = dir(a) == 0 .or. (p(a) > dmin(a) + depth .and. p(a) <= dmax(a) - depth)
+        enddo
+        if (all(good)) in_domain = .false.
+      endif
+    case ('sph')
+      in_domain = sum((p - dom%parm(1:3))**2.) <= dom%parm(4)**2.
+    case ('ell', 'tub')
+      call plane(dom%axis, i1, i2)
+      u = p(i1) - dom%parm(1); v = p(i2) - dom%parm(2)
+      dist = u**2. + v**2.
+      slope = (u + v) / (u - v)
+      if (abs(u - v) < small) slope = 0.0
+      re = dom%parm(3)*dom%parm(4)*sqrt(slope**2.+1.) / sqrt(dom%parm(3)**2.*slope**2.+dom%parm(4)**2.)
+      if (dom%ct == 'ell') then
+        in_domain = dist <= re**2.
+      else
+        ro = dom%parm(5)*dom%parm(6)*sqrt(slope**2.+1.) / sqrt(dom%parm(5)**2.*slope**2.+dom%parm(6)**2.)
+        in_domain = dist <= re**2. .and. dist > ro**2. .and. &
+                    p(dom%axis) > dom%parm(7) .and. p(dom%axis) < dom%parm(8)
+      endif
+    case ('wed')
+      call plane(dom%axis, i1, i2)
+      ! the wedge's centre coordinates are parm(i1), parm(i2) (src/domtype.f90:402-421)
+      u = p(i1) - dom%parm(i1); v = p(i2) - dom%parm(i2)
+      dist = u**2. + v**2.
+      if (abs(u - v) < small) then
+        re = 0.0
+      else
+        re = full_angle(u, v)
+      endif
+      in_domain = dist <= dom%parm(3)**2 .and. re > dom%parm(4) - dom%parm(5) .and. &
+                  re <= dom%parm(4)
+    end select
+    if (.not. dom%inside) in_domain = .not. in_domain
+  end function in_domain
+
+  ! the two axes across a domain's axial direction
+  subroutine plane(axis, i1, i2)
+    integer, intent(in) :: axis
+    integer, intent(out) :: i1, i2
+    select case (axis)
+    case (1); i1 = 2; i2 = 3
+    case (2); i1 = 1; i2 = 3
+    case default; i1 = 1; i2 = 2
+    end select
+  end subroutine plane
+
+  ! fullangle (src/domtype.f90:576-598): angle of (u,v) from the first axis in [0, 2 pi)
+  real(4) function full_angle(u, v)
+    real(4), intent(in) :: u, v
+    real(4), parameter :: pi = 3.1415926535, small = tiny(1.0)
+    real(4) :: at
+    if (abs(u) < small .and. abs(v) < small) then
+      at = pi / 2.0
+    else
+      at = atan2(v, u)
+    endif
+    full_angle = at
+    if (at < 0.0) full_angle = at + 2*pi
+  end function full_angle
+
+  ! setType (src/nd.f90:1700-1730): the atoms of a group change type
+  subroutine change_type(gname, tname)
+    character(*), intent(in) :: gname, tname
+    integer :: gid, tid
+    gid = group_id(gname)
+    tid = type_id(tname)
+    if (gid == 0) call die('ERROR: ChangeType: unknown group '//trim(gname))
+    if (tid == 0) call die('ERROR: ChangeType: unknown type '//trim(tname))
+    where (grp == gid) typ = tid
+    state_dirty = .true.
+  end subroutine change_type
 
   ! read_model (src/read_input.f90:277-392): xyz (real coordinates) or LAMMPS dump
   ! (scaled xs in [0,1), single precision as the reference reads them)
@@ -337,12 +582,25 @@ contains
 
   ! normalVelocityDistribution (src/nd.f90:1500-1624): v = N(0,1) sqrt(2 kt / (3 m)) per
   ! component (Box-Muller here), then the mean velocity is removed
-  subroutine velocities(kt)
+  ! normalVelocityDistribution (src/nd.f90:1500-1624) with its own normal deviates: the
+  ! atoms of group gname (all atoms for 'N/A') get new velocities; the sum of the new
+  ! velocities over np is then removed from every atom, as the reference does
+  subroutine velocities(kt, gname)
     real(8), intent(in) :: kt
+    character(*), intent(in) :: gname
     real(8) :: u1, u2, z(3), vsum(3)
-    integer :: k, a
+    integer :: k, a, gt
+    logical :: ss
     if (.not. have_model) call die('ERROR: Velocity needs a model')
+    ss = trim(gname) /= 'N/A'
+    gt = 0
+    if (ss) gt = group_id(gname)
+    if (ss .and. gt == 0) call die('ERROR: Velocity: unknown group '//trim(gname))
+    vsum = 0.d0
     do k = 1, np
+      if (ss) then
+        if (grp(k) /= gt) cycle
+      endif
       do a = 1, 3
         call random_number(u1)
         call random_number(u2)
@@ -350,9 +608,10 @@ contains
         z(a) = sqrt(-2.d0*log(u1)) * cos(2.d0*3.141592653589793d0*u2)
       enddo
       vel(1:3,k) = z * sqrt(2.d0*kt/(3.d0*mass(typ(k))))
+      vsum = vsum + vel(1:3,k)
     enddo
+    vsum = vsum / dble(np)
     do a = 1, 3
-      vsum(a) = sum(vel(a,:)) / dble(np)
       vel(a,:) = vel(a,:) - vsum(a)
     enddo
     state_dirty = .true.
@@ -492,6 +751,11 @@ contains
       call check(ndg_init_positions(h))
       initpos_pending = .false.
     endif
+    if (holds_dirty) then
+      call alloc_groups(.false.)
+      call check(ndg_set_holds(h, hold, int(nhold, c_int), bcdisp, bcdir, bcparm))
+      holds_dirty = .false.
+    endif
     rcut = maxval(pairparams(1,:))
     sk = skin
     if (sk < 0.d0) sk = rcut * 0.2d0
@@ -603,7 +867,8 @@ contains
     write(6,'(A)') ' Skin NeiMax Types Pairs WritePotentialTable Isokinetic Anisokinetic'
     write(6,'(A)') ' Isoenergetic Anisoenergetic Isobaric Anisobaric BoxVel StopBoxVel Exec Print'
     write(6,'(A)') ' OutFreq StepNumber ConfOutFreq Run END; engine: Precision mixed|fp64,'
-    write(6,'(A)') ' ListTrigger compat|skin, FrameWriter engine|fortran.'
+    write(6,'(A)') ' ListTrigger compat|skin, FrameWriter engine|fortran; Group/ClearGroups/Hold/'
+    write(6,'(A)') ' ClearHolds/Disp/ChangeType/InitPos/GroupBufferSize/HoldBufferSize.'
     write(6,'(A)') ' Pair kinds: morse morse_phys buck LJ spring (sym or asym).'
     call flush(6)
   end subroutine help

@@ -258,3 +258,80 @@ END
         sum(1 for l in out["fortran"].splitlines() if l.strip() == str(s.np).encode())
     assert frames == 4 + 3      # Run 1: steps 0, 10, 20, 30; Run 2: 30, 40, 50
     assert out["engine"] == out["fortran"]
+
+
+def dump_frames(path):
+    """out.dump -> list of (step, ids, types, xs(n,3))."""
+    lines = open(path).read().splitlines()
+    out = []
+    k = 0
+    while k < len(lines):
+        if lines[k].startswith("ITEM: TIMESTEP"):
+            step = int(lines[k + 1])
+            n = int(lines[k + 3])
+            rows = [l.split() for l in lines[k + 9:k + 9 + n]]
+            out.append((step, np.array([int(r[0]) for r in rows]), np.array([int(r[1]) for r in rows]),
+                        np.array([[float(x) for x in r[2:5]] for r in rows])))
+            k += 9 + n
+        else:
+            k += 1
+    return out
+
+
+@pytest.mark.gpu
+@needs_driver
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_driver_groups_holds_and_displacements(tmp_path, precision):
+    """Group by a sphere domain + ChangeType, Hold by two slab domains (rec) with Disp
+    (src/read_input.f90:171-190, src/nd.f90:1625-1730, src/domtype.f90:278-434): the
+    sphere's atoms change type, the frozen slab keeps its printed positions, the pulled slab
+    moves by 100 * 1e-5 along x, and the atoms selected follow the reference's
+    single-precision domain tests."""
+    write_fcc_dump(tmp_path / "model.dump")
+    deck = f"""START
+Model 	model.dump dump
+Seed 1234567890
+Types 2
+ Cu 63.546
+ Ag 107.87
+Pairs 3
+ Cu Cu sym morse 6.0 0.3429 1.359 2.866
+ Cu Ag sym morse 6.0 0.3 1.3 2.9
+ Ag Ag sym morse 6.0 0.33 1.32 2.95
+Group ball dom sph T 0.1 0.1 0.1 0.21
+ChangeType ball Ag
+Velocity 0.03877798635
+Hold fixed dom rec T -0.5 -0.41 0 0 0 0
+Hold pull dom rec T 0.39 0.5 0 0 0 0
+Hold other typ 2
+Disp fixed 0 0 0
+Disp pull 1e-5 F F
+Precision {precision}
+OutFreq 50
+ConfOutFreq 100 dump
+Run 100 0.05
+END
+"""
+    print(f"case groups/holds {precision}", flush=True)
+    p = run_deck(deck, tmp_path)
+    assert p.returncode == 0, p.stderr
+    frames = dump_frames(tmp_path / "out.dump")
+    assert [f[0] for f in frames] == [0, 100]
+    (_, ids, t0, x0), (_, _, t1, x1) = frames
+    # the model's scaled positions as the driver holds them: real(4) xs - 0.5 (read_model)
+    pos, box = W.fcc(10, 10, 10, 3.68)
+    xs = ((pos + 0.5).astype(np.float32) - np.float32(0.5)).astype(np.float32)
+    c = np.float32(0.1)
+    # (boundaries chosen between lattice planes and shells)
+    ball = ((xs - c) ** 2).sum(axis=1) <= np.float32(0.21) ** 2
+    assert 50 < ball.sum() < len(xs)
+    np.testing.assert_array_equal(t0, np.where(ball, 2, 1))
+    np.testing.assert_array_equal(t1, t0)
+    f32 = np.float32
+    fixed = (xs[:, 0] > f32(-0.5)) & (xs[:, 0] <= f32(-0.41))
+    pull = (xs[:, 0] > f32(0.39)) & (xs[:, 0] <= f32(0.5))
+    assert fixed.sum() > 100 and pull.sum() > 100
+    np.testing.assert_array_equal(x1[fixed], x0[fixed])
+    np.testing.assert_allclose(x1[pull, 0] - x0[pull, 0], 100 * 1e-5, atol=1.1e-5)
+    free = ~fixed & ~pull & ~ball
+    assert np.abs(x1[free] - x0[free]).max() > 1e-4
