@@ -295,6 +295,22 @@ static uint64_t fnv1a(const void *p, size_t n, uint64_t h = 1469598103934665603u
   return h;
 }
 
+// Content key of large arrays (the pair tables, hashed at every admission): 8-byte words
+// through a multiply-rotate mix, the tail bytewise; ~8x faster than the bytewise FNV-1a.
+static uint64_t hash_words(const void *p, size_t n, uint64_t h = 0x9e3779b97f4a7c15ull) {
+  const unsigned char *c = (const unsigned char *)p;
+  size_t i = 0;
+  for (; i + 8 <= n; i += 8) {
+    uint64_t w;
+    memcpy(&w, c + i, 8);
+    h ^= w * 0xbf58476d1ce4e5b9ull;
+    h = (h << 31) | (h >> 33);
+    h *= 0x94d049bb133111ebull;
+  }
+  h ^= (uint64_t)n;
+  return fnv1a(c + i, n - i, h);
+}
+
 static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 struct Layout {
@@ -636,8 +652,8 @@ static int put_state_dev(ndg_engine *e, int r, const RepState *S) {
 // Tables: dedupe identical tables between replicas.
 static int attach_tables(ndg_engine *e, Slot &s, const HostState &h) {
   const size_t n = (size_t)h.npairs * (h.ptsize + 1);
-  uint64_t key = fnv1a(h.P.data(), n * 8);
-  key = fnv1a(h.F.data(), n * 8, key);
+  uint64_t key = hash_words(h.P.data(), n * 8);
+  key = hash_words(h.F.data(), n * 8, key);
   key = fnv1a(&h.npairs, sizeof(int), key);
   key = fnv1a(&h.ptsize, sizeof(int), key);
   auto it = e->tables.find(key);
