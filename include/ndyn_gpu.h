@@ -53,8 +53,15 @@ enum {
 /* trajectory formats of write_model (src/read_input.f90:403-421), ndg_frame_write */
 enum { NDG_FRAME_XYZ = 1, NDG_FRAME_DUMP = 2 };
 
-/* pair kinds for ndg_pair_table (reference potentials.f90 names) */
-enum { NDG_MORSE = 1, NDG_BUCK = 2, NDG_LJ = 3, NDG_MORSE_PHYS = 4, NDG_SPRING = 5 };
+/* pair kinds for ndg_pair_table (reference potentials.f90 names, src/potentials.f90:82-147):
+ * morse, buck, LJ, spring, sine, cosine, soft, John, and MorseC / BuckC / LJC, the first
+ * three with the damped shifted Coulomb term of the pair's charge product added;
+ * NDG_MORSE_PHYS is the energy-consistent Morse (not in the reference) */
+enum {
+  NDG_MORSE = 1, NDG_BUCK = 2, NDG_LJ = 3, NDG_MORSE_PHYS = 4, NDG_SPRING = 5,
+  NDG_SINE = 6, NDG_COSINE = 7, NDG_SOFT = 8, NDG_JOHNSON = 9,
+  NDG_MORSE_C = 10, NDG_BUCK_C = 11, NDG_LJ_C = 12
+};
 
 typedef struct ndg_system ndg_t;
 typedef struct ndg_engine ndg_engine_t;
@@ -89,6 +96,10 @@ typedef struct {
  * in deck order.  Host-only helper for callers without their own table builder. */
 int ndg_pair_table(int kind, const double params[4], double ptrmin, int ptsize,
                    double *P, double *F, double *rc2, double *dr2);
+/* The same with the pair's charge product (PairChargeProd, src/read_input.f90:134), for the
+ * Coulomb kinds NDG_MORSE_C, NDG_BUCK_C and NDG_LJ_C (ignored by the others). */
+int ndg_pair_table_q(int kind, const double params[4], double qprd, double ptrmin, int ptsize,
+                     double *P, double *F, double *rc2, double *dr2);
 
 /* ------------------------------------------------------------------ single system */
 

@@ -67,24 +67,65 @@ static const char *err_text(int code) {
 // buildPairTables (src/potentials.f90:2-56) with pot/dpot (src/potentials.f90:59-247):
 // Morse (:94-110, :184-204, the /r of :198 kept), Buckingham (:112-122, :206-216),
 // Lennard-Jones (:124-133, :218-227), spring (:88-92, :178-183: a table written for output
-// only, the force kernels take spring forces from the strain); NDG_MORSE_PHYS tabulates
+// only, the force kernels take spring forces from the strain), sine / cosine / soft
+// (:82-87, :172-177; soft's force takes parm(1), parm(2) where its energy takes parm(2),
+// parm(3), as the reference does), Johnson (:136-147, :230-241) and the Coulomb variants
+// (:106-108, :120, :131 and :199-201, :213-214, :224-225); NDG_MORSE_PHYS tabulates
 // -r V'(r) for Morse.
 
-static double pair_v(int kind, double r, const double *p) {
+namespace {
+constexpr double kPi = 3.141592653589793;   // interpot's PI
+constexpr double kAlphaC = 0.2;             // damping of the shifted Coulomb term (:11)
+
+// the shifted Coulomb constants of a pair with cutoff rc (src/potentials.f90:23-26)
+struct Coul {
+  double vcon, fcon;
+  explicit Coul(double rc) {
+    const double rc2 = rc * rc;
+    vcon = std::erfc(kAlphaC * rc) / rc;
+    fcon = std::erfc(kAlphaC * rc) / rc2 + 2.0 * kAlphaC / std::sqrt(kPi) * std::exp(-kAlphaC * kAlphaC * rc2) / rc;
+  }
+  double v(double r, double rc, double q) const {
+    return q * (std::erfc(kAlphaC * r) / r - vcon + fcon * (r - rc));
+  }
+  double dv(double r, double q) const {
+    return q * (std::erfc(kAlphaC * r) / (r * r) +
+                2.0 * kAlphaC / std::sqrt(kPi) * std::exp(-(kAlphaC * kAlphaC) * (r * r)) / r - fcon);
+  }
+};
+}  // namespace
+
+static double pair_v(int kind, double r, const double *p, double qprd = 0.0) {
   switch (kind) {
     case NDG_MORSE:
+    case NDG_MORSE_C:
     case NDG_MORSE_PHYS: {
       double x = p[3] <= 0.0 ? 0.0 : r - p[3];
       double e = std::exp(-p[2] * x);
-      return p[1] * (e * e - 2.0 * e);
+      double v = p[1] * (e * e - 2.0 * e);
+      return kind == NDG_MORSE_C ? v + Coul(p[0]).v(r, p[0], qprd) : v;
     }
-    case NDG_BUCK: {
+    case NDG_BUCK:
+    case NDG_BUCK_C: {
       double r6 = r * r * r * r * r * r;
-      return p[2] <= 0.0 ? p[1] - p[3] / r6 : p[1] * std::exp(-r / p[2]) - p[3] / r6;
+      double v = p[2] <= 0.0 ? p[1] - p[3] / r6 : p[1] * std::exp(-r / p[2]) - p[3] / r6;
+      return kind == NDG_BUCK_C ? v + Coul(p[0]).v(r, p[0], qprd) : v;
     }
-    case NDG_LJ: {
+    case NDG_LJ:
+    case NDG_LJ_C: {
       double q = p[2] / r, q6 = q * q * q * q * q * q;
-      return 4.0 * p[1] * (q6 * q6 - q6);
+      double v = 4.0 * p[1] * (q6 * q6 - q6);
+      return kind == NDG_LJ_C ? v + Coul(p[0]).v(r, p[0], qprd) : v;
+    }
+    case NDG_SINE: return std::sin(r) * std::sin(r) - 1.0;
+    case NDG_COSINE: return std::cos(r) * std::cos(r) - 1.0;
+    case NDG_SOFT: return p[1] * (1.0 + std::cos(kPi * r / p[2]));
+    case NDG_JOHNSON: {   // eps, r0, rc = PairParams(2:4)
+      const double eps = p[1], r0 = p[2], rc = p[3];
+      const double A = 2.0 * (r0 * r0 * r0) / ((rc - r0) * (rc - r0) * (rc - r0));
+      const double B = 3.0 * (r0 * r0) / ((rc - r0) * (rc - r0));
+      const double t = (r - rc) / r0;
+      return r < rc ? -eps * (A * (t * t * t) + B * (t * t)) : 0.0;
     }
     case NDG_SPRING: {   // D (r/rc C)^2 - D C^2 with rc, D, C = PairParams(1:3)
       double x = r / p[0] * p[2];
@@ -94,23 +135,41 @@ static double pair_v(int kind, double r, const double *p) {
   return 0.0;
 }
 
-static double pair_dv(int kind, double r, const double *p) {  // -dV/dr (reference Morse: /r)
+static double pair_dv(int kind, double r, const double *p, double qprd = 0.0) {
+  // -dV/dr (reference Morse: /r)
   switch (kind) {
     case NDG_MORSE:
+    case NDG_MORSE_C:
     case NDG_MORSE_PHYS: {
       double x = p[3] <= 0.0 ? 0.0 : r - p[3];
       double e = std::exp(-p[2] * x);
       double d = 2.0 * p[1] * p[2] * (e * e - e);
-      return kind == NDG_MORSE ? d / r : d;
+      if (kind == NDG_MORSE_PHYS) return d;
+      d = d / r;
+      return kind == NDG_MORSE_C ? d + Coul(p[0]).dv(r, qprd) : d;
     }
-    case NDG_BUCK: {
+    case NDG_BUCK:
+    case NDG_BUCK_C: {
       double r7 = r * r * r * r * r * r * r;
-      return p[2] <= 0.0 ? p[1] - 6.0 * p[3] / r7
-                         : p[1] * std::exp(-r / p[2]) / p[2] - 6.0 * p[3] / r7;
+      double d = p[2] <= 0.0 ? p[1] - 6.0 * p[3] / r7
+                             : p[1] * std::exp(-r / p[2]) / p[2] - 6.0 * p[3] / r7;
+      return kind == NDG_BUCK_C ? d + Coul(p[0]).dv(r, qprd) : d;
     }
-    case NDG_LJ: {
+    case NDG_LJ:
+    case NDG_LJ_C: {
       double q = p[2] / r, q6 = q * q * q * q * q * q;
-      return 24.0 * p[1] * (2.0 * q6 * q6 / r - q6 / r);
+      double d = 24.0 * p[1] * (2.0 * q6 * q6 / r - q6 / r);
+      return kind == NDG_LJ_C ? d + Coul(p[0]).dv(r, qprd) : d;
+    }
+    case NDG_SINE: return std::sin(2.0 * r);
+    case NDG_COSINE: return -std::sin(2.0 * r);
+    case NDG_SOFT: return kPi * p[0] * std::sin(kPi * r / p[1]) / p[1];
+    case NDG_JOHNSON: {
+      const double eps = p[1], r0 = p[2], rc = p[3];
+      const double A = 2.0 * (r0 * r0 * r0) / ((rc - r0) * (rc - r0) * (rc - r0));
+      const double B = 3.0 * (r0 * r0) / ((rc - r0) * (rc - r0));
+      const double t = (r - rc) / r0;
+      return r < rc ? eps / r0 * (3.0 * A * (t * t) + 2.0 * B * t) : 0.0;
     }
     case NDG_SPRING: {
       double x = r / p[0] * p[2];
@@ -120,9 +179,9 @@ static double pair_dv(int kind, double r, const double *p) {  // -dV/dr (referen
   return 0.0;
 }
 
-extern "C" int ndg_pair_table(int kind, const double params[4], double ptrmin, int ptsize,
-                              double *P, double *F, double *rc2, double *dr2) {
-  if (kind < NDG_MORSE || kind > NDG_SPRING || ptsize < 2 || !P || !F)
+extern "C" int ndg_pair_table_q(int kind, const double params[4], double qprd, double ptrmin,
+                                int ptsize, double *P, double *F, double *rc2, double *dr2) {
+  if (kind < NDG_MORSE || kind > NDG_LJ_C || ptsize < 2 || !P || !F)
     return fail(NDG_ERR_ARG, "ndg_pair_table: bad arguments");
   const double rc = params[0];
   const double c2 = rc * rc;
@@ -130,12 +189,17 @@ extern "C" int ndg_pair_table(int kind, const double params[4], double ptrmin, i
   for (int i = 0; i <= ptsize; ++i) {
     const double s = ptrmin + (double)i * h;
     const double r = std::sqrt(s);
-    P[i] = pair_v(kind, r, params);
-    F[i] = r * pair_dv(kind, r, params);
+    P[i] = pair_v(kind, r, params, qprd);
+    F[i] = r * pair_dv(kind, r, params, qprd);
   }
   if (rc2) *rc2 = c2;
   if (dr2) *dr2 = h;
   return NDG_OK;
+}
+
+extern "C" int ndg_pair_table(int kind, const double params[4], double ptrmin, int ptsize,
+                              double *P, double *F, double *rc2, double *dr2) {
+  return ndg_pair_table_q(kind, params, 0.0, ptrmin, ptsize, P, F, rc2, dr2);
 }
 
 // ------------------------------------------------------------------ engine structures
@@ -1839,7 +1903,7 @@ extern "C" int ndg_set_pair_kinds(ndg_t *h, const int *kinds) {
   HostState &hs = h->hs;
   std::vector<int> k(kinds, kinds + hs.npairs);
   for (int v : k)
-    if (v < NDG_MORSE || v > NDG_SPRING) return fail(NDG_ERR_ARG, "ndg_set_pair_kinds: unknown kind");
+    if (v < NDG_MORSE || v > NDG_LJ_C) return fail(NDG_ERR_ARG, "ndg_set_pair_kinds: unknown kind");
   if (k == hs.kinds) return NDG_OK;
   Slot &s = h->e->slots[h->slot];
   if (s.block)

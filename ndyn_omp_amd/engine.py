@@ -72,6 +72,8 @@ def load_library():
     P = C.POINTER
     sig = {
         "ndg_pair_table": [C.c_int, dp, C.c_double, C.c_int, dp, dp, P(C.c_double), P(C.c_double)],
+        "ndg_pair_table_q": [C.c_int, dp, C.c_double, C.c_double, C.c_int, dp, dp, P(C.c_double),
+                             P(C.c_double)],
         "ndg_create": [P(V), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
         "ndg_set_tables": [V, dp, dp, dp, dp, C.c_double, ip, dp, dp],
         "ndg_set_state": [V, dp, V, V, ip, dp],
@@ -137,7 +139,7 @@ def _ptr(a: Optional[np.ndarray]):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
-def pair_tables(kinds, params, ptrmin, ptsize):
+def pair_tables(kinds, params, ptrmin, ptsize, qprd=None):
     """Tables of every pair with the engine's builder: P, F (npairs, ptsize+1), rc2, dr2."""
     L = load_library()
     kinds = np.asarray(kinds, dtype=np.int32)
@@ -150,8 +152,9 @@ def pair_tables(kinds, params, ptrmin, ptsize):
     for p in range(npairs):
         a, b = C.c_double(), C.c_double()
         Pp, Fp = np.zeros(ptsize + 1), np.zeros(ptsize + 1)
-        _check(L.ndg_pair_table(int(kinds[p]), np.ascontiguousarray(params[p]), float(ptrmin),
-                                int(ptsize), Pp, Fp, C.byref(a), C.byref(b)))
+        q = 0.0 if qprd is None else float(qprd[p])
+        _check(L.ndg_pair_table_q(int(kinds[p]), np.ascontiguousarray(params[p]), q,
+                                  float(ptrmin), int(ptsize), Pp, Fp, C.byref(a), C.byref(b)))
         P[p], F[p], rc2[p], dr2[p] = Pp, Fp, a.value, b.value
     return P, F, rc2, dr2
 

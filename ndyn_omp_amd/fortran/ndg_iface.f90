@@ -13,7 +13,8 @@ module ndg_iface
   integer(c_int), parameter :: NDG_PATH_FP64 = 0, NDG_PATH_MIXED = 1
   integer(c_int), parameter :: NDG_TRIGGER_COMPAT = 0, NDG_TRIGGER_SKIN = 1
   integer(c_int), parameter :: NDG_MORSE = 1, NDG_BUCK = 2, NDG_LJ = 3, NDG_MORSE_PHYS = 4, &
-                                NDG_SPRING = 5
+                                NDG_SPRING = 5, NDG_SINE = 6, NDG_COSINE = 7, NDG_SOFT = 8, &
+                                NDG_JOHNSON = 9, NDG_MORSE_C = 10, NDG_BUCK_C = 11, NDG_LJ_C = 12
   integer(c_int), parameter :: NDG_FRAME_XYZ = 1, NDG_FRAME_DUMP = 2
 
   type, bind(C) :: ndg_run_params
@@ -45,6 +46,15 @@ module ndg_iface
   end type ndg_thermo
 
   interface
+    integer(c_int) function ndg_pair_table_q(kind, params, qprd, ptrmin, ptsize, P, F, rc2, &
+        dr2) bind(C, name='ndg_pair_table_q')
+      import :: c_int, c_double
+      integer(c_int), value :: kind, ptsize
+      real(c_double), intent(in) :: params(4)
+      real(c_double), value :: qprd, ptrmin
+      real(c_double), intent(out) :: P(*), F(*), rc2, dr2
+    end function
+
     integer(c_int) function ndg_pair_table(kind, params, ptrmin, ptsize, P, F, rc2, dr2) &
         bind(C, name='ndg_pair_table')
       import :: c_int, c_double

@@ -27,6 +27,7 @@ program ndyn_gpu
   integer, allocatable :: typ(:), pairmat(:,:), pairkind(:)
   real(8), allocatable :: pos(:,:), vel(:,:), acc(:,:), pene(:), frc(:,:)
   real(8), allocatable :: mass(:), pairparams(:,:), ptab(:,:), ftab(:,:), rc2(:), dr2(:)
+  real(8), allocatable :: charge(:), pairq(:)
   character(len=8), allocatable :: typename(:), pairname(:)
   ! control (defaults of src/read_input.f90:18-41)
   real(8) :: dt = 0.0001d0, skin = -1.d0
@@ -116,14 +117,18 @@ program ndyn_gpu
       read(w(2),*) neimax
     case ('Types')
       read(w(2),*) ntypes
-      if (allocated(mass)) deallocate(mass, typename)
-      allocate(mass(ntypes), typename(ntypes))
+      if (allocated(mass)) deallocate(mass, typename, charge)
+      allocate(mass(ntypes), typename(ntypes), charge(ntypes))
+      charge = 0.d0
       do i = 1, ntypes
         read(5,'(A)',iostat=ios) line
         if (ios /= 0) call die('#  read_input: ERROR: premature end of input file. No END directive found')
+        call strip_comment(line)
         call split(line, w, nw)
         typename(i) = trim(w(1))
         read(w(2),*) mass(i)
+        ! name mass [charge] (src/read_input.f90:100-108)
+        if (nw >= 3) read(w(3),*) charge(i)
       enddo
       tables_dirty = .true.
     case ('WritePotentialTable')
@@ -622,9 +627,10 @@ contains
     integer :: k, p1, p2, q, e
     real(8) :: rcut
     if (ntypes == 0) call die('ERROR: must first define particle Types before Pairs')
-    if (allocated(pairmat)) deallocate(pairmat, pairkind, pairname, pairparams, ptab, ftab, rc2, dr2)
+    if (allocated(pairmat)) deallocate(pairmat, pairkind, pairname, pairparams, ptab, ftab, rc2, &
+                                       dr2, pairq)
     allocate(pairmat(ntypes,ntypes), pairkind(npairs), pairname(npairs), pairparams(4,npairs), &
-             ptab(ptsize+1,npairs), ftab(ptsize+1,npairs), rc2(npairs), dr2(npairs))
+             ptab(ptsize+1,npairs), ftab(ptsize+1,npairs), rc2(npairs), dr2(npairs), pairq(npairs))
     pairmat = 0
     pairparams = 0.d0
     pairparams(1,:) = 3.141592653589793d0 / 2.d0
@@ -637,13 +643,32 @@ contains
       p2 = type_id(w(2))
       if (p1 == 0 .or. p2 == 0) call die('ERROR: unknown type in Pairs')
       pairmat(p1,p2) = k
+      pairq(k) = charge(p1) * charge(p2)   ! PairChargeProd (src/read_input.f90:134)
       ! 'asym' leaves PairMat(p2,p1) alone; the engine then applies compute1's rules
       ! (src/read_input.f90:130-133, src/nd.f90:155-163)
       if (trim(w(3)) == 'sym') pairmat(p2,p1) = k
       pairname(k) = trim(w(4))
+      ! the names pot/dpot accept (src/potentials.f90:81-147): 'morsec', 'buckC' and 'buckc'
+      ! take their plain forms there; only MorseC/morseC, BuckC and LJC add the Coulomb term
       select case (trim(w(4)))
-      case ('morse', 'Morse')
+      case ('morse', 'Morse', 'morsec')
         pairkind(k) = NDG_MORSE
+      case ('MorseC', 'morseC')
+        pairkind(k) = NDG_MORSE_C
+      case ('BuckC')
+        pairkind(k) = NDG_BUCK_C
+      case ('buckC', 'buckc')
+        pairkind(k) = NDG_BUCK
+      case ('LJC')
+        pairkind(k) = NDG_LJ_C
+      case ('sine')
+        pairkind(k) = NDG_SINE
+      case ('cosine')
+        pairkind(k) = NDG_COSINE
+      case ('soft')
+        pairkind(k) = NDG_SOFT
+      case ('John')
+        pairkind(k) = NDG_JOHNSON
       case ('morse_phys')
         pairkind(k) = NDG_MORSE_PHYS
       case ('buck', 'Buck')
@@ -662,8 +687,8 @@ contains
     write(6,'(A)') '# pair potential type matrix'
     rcut = 0.d0
     do k = 1, npairs
-      call check(ndg_pair_table(int(pairkind(k), c_int), pairparams(:,k), ptrmin, &
-                                int(ptsize, c_int), ptab(:,k), ftab(:,k), rc2(k), dr2(k)))
+      call check(ndg_pair_table_q(int(pairkind(k), c_int), pairparams(:,k), pairq(k), ptrmin, &
+                                  int(ptsize, c_int), ptab(:,k), ftab(:,k), rc2(k), dr2(k)))
       write(6,'(a,i3,2f12.6)') '# pair,rcut2,dr2', k, rc2(k), dr2(k)
       rcut = max(rcut, pairparams(1,k))
     enddo
@@ -869,7 +894,8 @@ contains
     write(6,'(A)') ' OutFreq StepNumber ConfOutFreq Run END; engine: Precision mixed|fp64,'
     write(6,'(A)') ' ListTrigger compat|skin, FrameWriter engine|fortran; Group/ClearGroups/Hold/'
     write(6,'(A)') ' ClearHolds/Disp/ChangeType/InitPos/GroupBufferSize/HoldBufferSize.'
-    write(6,'(A)') ' Pair kinds: morse morse_phys buck LJ spring (sym or asym).'
+    write(6,'(A)') ' Pair kinds: morse morsec MorseC morse_phys buck BuckC LJ LJC spring sine cosine'
+    write(6,'(A)') ' soft John (sym or asym).'
     call flush(6)
   end subroutine help
 

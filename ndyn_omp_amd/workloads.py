@@ -21,6 +21,7 @@ DT_2FS = 0.196454             # 2 fs
 
 # pair kinds (same ids as the C ABI / oracle)
 MORSE, BUCK, LJ, MORSE_PHYS, SPRING = 1, 2, 3, 4, 5
+SINE, COSINE, SOFT, JOHNSON, MORSE_C, BUCK_C, LJ_C = 6, 7, 8, 9, 10, 11, 12
 
 CU_MORSE = (7.0, 0.3429, 1.3588, 2.866)        # rc, D, alpha, r0
 # peridynamic bond of examples/testPDspringTension/input.sh: rc, C, max strain s0, Vi

@@ -32,9 +32,38 @@
 
 /* ---------------------------------------------------------------- tables */
 
+/* The damped shifted Coulomb term of MorseC / BuckC / LJC (src/potentials.f90:11,23-26;
+ * energy :106-108, force :199-201), alpha = 0.2, ke = 1, for a pair of cutoff rc. */
+#define ALPHA_C 0.2
+static double coul_v(double r, double rc, double q) {
+  double vcon = erfc(ALPHA_C * rc) / rc;
+  double fcon = erfc(ALPHA_C * rc) / (rc * rc) +
+                2.0 * ALPHA_C / sqrt(PI_D) * exp(-ALPHA_C * ALPHA_C * (rc * rc)) / rc;
+  return q * (erfc(ALPHA_C * r) / r - vcon + fcon * (r - rc));
+}
+static double coul_dv(double r, double rc, double q) {
+  double fcon = erfc(ALPHA_C * rc) / (rc * rc) +
+                2.0 * ALPHA_C / sqrt(PI_D) * exp(-ALPHA_C * ALPHA_C * (rc * rc)) / rc;
+  return q * (erfc(ALPHA_C * r) / (r * r) +
+              2.0 * ALPHA_C / sqrt(PI_D) * exp(-(ALPHA_C * ALPHA_C) * (r * r)) / r - fcon);
+}
+
 /* pot()  src/potentials.f90:59-152 (Morse :94-110, Buck :112-122, LJ :124-133,
- * spring :88-92) */
-static double pot_fn(int kind, double r, const double *p) {
+ * spring :88-92, sine/cosine/soft :82-87, John :136-147) */
+static double pot_fn(int kind, double r, const double *p, double q) {
+  if (kind == NDO_SINE) return sin(r) * sin(r) - 1.0;
+  if (kind == NDO_COSINE) return cos(r) * cos(r) - 1.0;
+  if (kind == NDO_SOFT) return p[1] * (1.0 + cos(PI_D * r / p[2]));
+  if (kind == NDO_JOHNSON) {
+    double eps = p[1], r0 = p[2], rc = p[3];
+    double A = 2.0 * pow(r0, 3) / pow(rc - r0, 3);
+    double B = 3.0 * pow(r0, 2) / pow(rc - r0, 2);
+    double t = (r - rc) / r0;
+    return r < rc ? -eps * (A * pow(t, 3) + B * pow(t, 2)) : 0.0;
+  }
+  if (kind == NDO_MORSE_C) return pot_fn(NDO_MORSE, r, p, 0.0) + coul_v(r, p[0], q);
+  if (kind == NDO_BUCK_C) return pot_fn(NDO_BUCK, r, p, 0.0) + coul_v(r, p[0], q);
+  if (kind == NDO_LJ_C) return pot_fn(NDO_LJ, r, p, 0.0) + coul_v(r, p[0], q);
   if (kind == NDO_SPRING) {
     double rc = p[0], D = p[1], C = p[2];
     double r0 = r / rc * C;
@@ -66,7 +95,21 @@ static double pot_fn(int kind, double r, const double *p) {
 /* dpot()  src/potentials.f90:154-247 (Morse :184-204 with the /r at :198,
  * Buck :206-216, LJ :218-227).  Returns -dV/dr except for the reference Morse,
  * which returns -dV/dr / r. */
-static double dpot_fn(int kind, double r, const double *p) {
+static double dpot_fn(int kind, double r, const double *p, double q) {
+  if (kind == NDO_SINE) return sin(2.0 * r);
+  if (kind == NDO_COSINE) return -sin(2.0 * r);
+  /* soft's force reads parm(1), parm(2) (src/potentials.f90:176), its energy parm(2), parm(3) */
+  if (kind == NDO_SOFT) return PI_D * p[0] * sin(PI_D * r / p[1]) / p[1];
+  if (kind == NDO_JOHNSON) {   /* src/potentials.f90:230-241 */
+    double eps = p[1], r0 = p[2], rc = p[3];
+    double A = 2.0 * pow(r0, 3) / pow(rc - r0, 3);
+    double B = 3.0 * pow(r0, 2) / pow(rc - r0, 2);
+    double t = (r - rc) / r0;
+    return r < rc ? eps / r0 * (3.0 * A * pow(t, 2) + 2.0 * B * t) : 0.0;
+  }
+  if (kind == NDO_MORSE_C) return dpot_fn(NDO_MORSE, r, p, 0.0) + coul_dv(r, p[0], q);
+  if (kind == NDO_BUCK_C) return dpot_fn(NDO_BUCK, r, p, 0.0) + coul_dv(r, p[0], q);
+  if (kind == NDO_LJ_C) return dpot_fn(NDO_LJ, r, p, 0.0) + coul_dv(r, p[0], q);
   if (kind == NDO_SPRING) {   /* src/potentials.f90:178-183 */
     double rc = p[0], D = p[1], C = p[2];
     double r0 = r / rc * C;
@@ -99,7 +142,12 @@ static double dpot_fn(int kind, double r, const double *p) {
 /* buildPairTables  src/potentials.f90:2-56 */
 int ndo_pair_table(int kind, const double params[4], double ptrmin, int ptsize,
                    double *P, double *F, double *rc2, double *dr2) {
-  if (kind < NDO_MORSE || kind > NDO_SPRING || ptsize < 2) return NDO_ERR_ARG;
+  return ndo_pair_table_q(kind, params, 0.0, ptrmin, ptsize, P, F, rc2, dr2);
+}
+
+int ndo_pair_table_q(int kind, const double params[4], double qprd, double ptrmin, int ptsize,
+                     double *P, double *F, double *rc2, double *dr2) {
+  if (kind < NDO_MORSE || kind > NDO_LJ_C || ptsize < 2) return NDO_ERR_ARG;
   double rcut = params[0];
   double c2 = rcut * rcut;
   double d2 = (c2 - ptrmin) / (double)(ptsize - 1);
@@ -108,8 +156,8 @@ int ndo_pair_table(int kind, const double params[4], double ptrmin, int ptsize,
   for (int i = 0; i <= ptsize; ++i) {
     double s = ptrmin + (double)i * d2;
     double r = sqrt(s);
-    P[i] = pot_fn(kind, r, params);
-    F[i] = r * dpot_fn(kind, r, params);
+    P[i] = pot_fn(kind, r, params, qprd);
+    F[i] = r * dpot_fn(kind, r, params, qprd);
   }
   return NDO_OK;
 }

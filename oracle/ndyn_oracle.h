@@ -28,8 +28,11 @@ enum {
   NDO_BUCK = 2,       /* Buckingham A exp(-r/rho) - C/r^6        (src/potentials.f90:112-122,206-216) */
   NDO_LJ = 3,         /* Lennard-Jones 4 eps((s/r)^12-(s/r)^6)   (src/potentials.f90:124-133,218-227) */
   NDO_MORSE_PHYS = 4, /* energy-consistent Morse: table force = -r V'(r) (not in the reference) */
-  NDO_SPRING = 5      /* 'spring' (src/potentials.f90:88-92,178-183): a table for output only;
+  NDO_SPRING = 5,     /* 'spring' (src/potentials.f90:88-92,178-183): a table for output only;
                          compute2/3 take the force from the strain against ipos */
+  NDO_SINE = 6, NDO_COSINE = 7, NDO_SOFT = 8,  /* src/potentials.f90:82-87, 172-177 */
+  NDO_JOHNSON = 9,                              /* 'John' :136-147, :230-241 */
+  NDO_MORSE_C = 10, NDO_BUCK_C = 11, NDO_LJ_C = 12  /* + damped shifted Coulomb term */
 };
 
 /* list-rebuild trigger */
@@ -50,6 +53,9 @@ enum {
  * i = 0..ptsize (src/potentials.f90:19-34). params = {rc, p2, p3, p4} in deck order. */
 int ndo_pair_table(int kind, const double params[4], double ptrmin, int ptsize,
                    double *P, double *F, double *rc2, double *dr2);
+/* the same with the pair's charge product qprd (Coulomb kinds) */
+int ndo_pair_table_q(int kind, const double params[4], double qprd, double ptrmin, int ptsize,
+                     double *P, double *F, double *rc2, double *dr2);
 
 /* Reference NeiMax default (src/nd.f90:84-88). */
 int ndo_neimax_estimate(double rcut, double skin, int np, const double box[3]);

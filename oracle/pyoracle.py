@@ -21,6 +21,7 @@ dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 
 MORSE, BUCK, LJ, MORSE_PHYS, SPRING = 1, 2, 3, 4, 5
+SINE, COSINE, SOFT, JOHNSON, MORSE_C, BUCK_C, LJ_C = 6, 7, 8, 9, 10, 11, 12
 TRIGGER_COMPAT, TRIGGER_SKIN = 0, 1
 
 
@@ -56,6 +57,8 @@ def lib():
         if not os.path.exists(_LIB_PATH):
             build()
         L = C.CDLL(_LIB_PATH)
+        L.ndo_pair_table_q.argtypes = [C.c_int, dp, C.c_double, C.c_double, C.c_int, dp, dp,
+                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.ndo_pair_table.argtypes = [C.c_int, dp, C.c_double, C.c_int, dp, dp,
                                      C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.ndo_neimax_estimate.argtypes = [C.c_double, C.c_double, C.c_int, dp]
@@ -96,7 +99,7 @@ def lib():
     return _lib
 
 
-def pair_tables(kinds, params, ptrmin, ptsize):
+def pair_tables(kinds, params, ptrmin, ptsize, qprd=None):
     """Tables for all pairs: P, F (npairs, ptsize+1); rc2, dr2 (npairs,)."""
     kinds = np.asarray(kinds, dtype=np.int32)
     params = np.ascontiguousarray(params, dtype=np.float64).reshape(-1, 4)
@@ -109,8 +112,9 @@ def pair_tables(kinds, params, ptrmin, ptsize):
         a, b = C.c_double(), C.c_double()
         Pp = np.zeros(ptsize + 1)
         Fp = np.zeros(ptsize + 1)
-        rc = lib().ndo_pair_table(int(kinds[p]), np.ascontiguousarray(params[p]), ptrmin,
-                                  ptsize, Pp, Fp, C.byref(a), C.byref(b))
+        q = 0.0 if qprd is None else float(qprd[p])
+        rc = lib().ndo_pair_table_q(int(kinds[p]), np.ascontiguousarray(params[p]), q, ptrmin,
+                                    ptsize, Pp, Fp, C.byref(a), C.byref(b))
         if rc != 0:
             raise ValueError(f"ndo_pair_table failed ({rc})")
         P[p], F[p] = Pp, Fp

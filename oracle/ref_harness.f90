@@ -17,8 +17,9 @@
 !   int32  boxload, have_ipos; real64 boxvel(3)
 !   (have_ipos /= 0) real64 ipos(3,np), initbox(3)
 !   int32 nhold; (nhold > 0) int32 hold(np), bcdisp(nhold), bcdir(3,nhold);
-!   real64 bcparm(3,nhold)
-! Pair kinds: 1 morse, 2 buck, 3 LJ, 5 spring.
+!   real64 bcparm(3,nhold); real64 pairchargeprod(npairs)
+! Pair kinds: 1 morse, 2 buck, 3 LJ, 5 spring, 6 sine, 7 cosine, 8 soft, 9 John,
+! 10 MorseC, 11 BuckC, 12 LJC.
 ! Modes: 1 tables (buildPairTables), 2 half list (buildCellList),
 !        3 forces (buildCellList + the force routine dynamics would call: compute0,
 !        compute1 for an asymmetric PairMat, compute2/compute3 with spring pairs),
@@ -75,6 +76,13 @@ program ref_harness
       case (2); PairName(i) = 'buck'
       case (3); PairName(i) = 'LJ'
       case (5); PairName(i) = 'spring'
+      case (6); PairName(i) = 'sine'
+      case (7); PairName(i) = 'cosine'
+      case (8); PairName(i) = 'soft'
+      case (9); PairName(i) = 'John'
+      case (10); PairName(i) = 'MorseC'
+      case (11); PairName(i) = 'BuckC'
+      case (12); PairName(i) = 'LJC'
       case default; stop 'harness: unknown pair kind'
     end select
   enddo
@@ -96,6 +104,7 @@ program ref_harness
   endif
   read(20) nh
   close(20)
+  call read_charge_products()
   force = 0.d0; pene = 0.d0
   grpBuff = 100; holdBuff = 100; Ngrp = 0; Nhold = 0
   allocate(grp(np), groupName(grpBuff), hold(np), holdName(holdBuff), &
@@ -165,6 +174,18 @@ program ref_harness
   end select
   close(21)
 contains
+  ! PairChargeProd, the input's last block
+  subroutine read_charge_products()
+    integer(8) :: off
+    off = 4*18 + 8*13 + 8*Ntypes + 4*Ntypes*Ntypes + 4*NPairs + 8*4*NPairs + 4*np + 3*8*3*np &
+          + 4*2 + 8*3 + 4
+    if (iv2(2) /= 0) off = off + 8*3*np + 8*3
+    if (nh > 0) off = off + 4*np + 4*nh + 4*3*nh + 8*3*nh
+    open(20, file=trim(fin), access='stream', form='unformatted', status='old')
+    read(20, pos=off+1) PairChargeProd
+    close(20)
+  end subroutine read_charge_products
+
   ! position the input at the hold block: after the fixed header, the per-type and per-pair
   ! arrays, the state, the BoxVel block and the optional initial positions
   subroutine skip_to_holds()

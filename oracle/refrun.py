@@ -24,7 +24,7 @@ def available() -> bool:
 
 def write_input(path, mode, s, nsteps=0, outsteps=1, step0=0, neimax=0, skin=-1.0, dt=0.0,
                 renei=True, renei_freq=1, iso_ke=None, iso_p=None, iso_e=None, acc=None,
-                nthreads=1, box_vel=None, ipos=None, init_box=None, holds=None):
+                nthreads=1, box_vel=None, ipos=None, init_box=None, holds=None, qprd=None):
     """Binary input of ref_harness.f90 (its header documents the layout)."""
     iv = np.array([mode, s.np, s.ntypes, s.npairs, s.ptsize, neimax, nsteps, outsteps, step0,
                    int(renei), renei_freq,
@@ -66,6 +66,7 @@ def write_input(path, mode, s, nsteps=0, outsteps=1, step0=0, neimax=0, skin=-1.
             f.write(np.asarray(holds[1], "<i4").tobytes())
             f.write(np.asarray(holds[2], "<i4").tobytes())       # (nhold,3) = bcDir(3,nhold)
             f.write(np.asarray(holds[3], "<f8").tobytes())
+        f.write(np.asarray(np.zeros(s.npairs) if qprd is None else qprd, "<f8").tobytes())
 
 
 def run_harness(mode, s, timeout=3600, **kw):

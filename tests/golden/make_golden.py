@@ -71,6 +71,34 @@ def golden_tables():
     np.savez_compressed(os.path.join(OUT, "tables.npz"), **out)
 
 
+# the other potentials of buildPairTables (src/potentials.f90:82-147): name, kind, params
+# {rc, p2, p3, p4}, charge product
+MORE_TABLES = [("sine", W.SINE, (6.0, 0.0, 0.0, 0.0), 0.0),
+               ("cosine", W.COSINE, (6.0, 0.0, 0.0, 0.0), 0.0),
+               ("soft", W.SOFT, (6.0, 0.5, 6.0, 0.0), 0.0),          # examples/SineTest
+               ("john", W.JOHNSON, (6.0, 0.5, 2.5, 5.5), 0.0),
+               ("morsec", W.MORSE_C, W.CU_MORSE, 0.8),
+               ("buckc", W.BUCK_C, W.ALLOY_BUCK, -1.2),
+               ("ljc", W.LJ_C, W.AR_LJ, 0.35)]
+
+
+def golden_tables_more():
+    out = {}
+    for name, kind, prm, q in MORE_TABLES:
+        s = W.cu_morse(1, temp_k=0.0)
+        s.kinds = np.array([kind], np.int32)
+        s.params = np.array([prm], np.float64)
+        raw, _ = run_harness(1, s, qprd=[q])
+        n = s.ptsize + 1
+        a = np.frombuffer(raw, "<f8")
+        out[f"{name}_kind"], out[f"{name}_params"], out[f"{name}_qprd"] = kind, np.array(prm), q
+        out[f"{name}_rc2"], out[f"{name}_dr2"] = a[0], a[1]
+        out[f"{name}_P"] = a[2:2 + n]
+        out[f"{name}_F"] = a[2 + n:2 + 2 * n]
+    out["ptrmin"], out["ptsize"] = np.float64(0.25), np.int64(5000)
+    np.savez_compressed(os.path.join(OUT, "tables_more.npz"), **out)
+
+
 def golden_list(fname, s, skin):
     raw, _ = run_harness(2, s, skin=skin)
     hdr = np.frombuffer(raw[:16], "<i4")
@@ -174,7 +202,11 @@ def main():
     if "--hold-only" in sys.argv:
         hold_fixtures()
         return
+    if "--tables-more" in sys.argv:
+        golden_tables_more()
+        return
     golden_tables()
+    golden_tables_more()
     # half lists: Cu 7^3 (MCELL 3), c0-size Cu 10^3 (MCELL 4), Buckingham alloy (3 pairs)
     golden_list("list_cu7.npz", W.cu_morse(7, jitter_a=0.05, seed=101), 0.5)
     golden_list("list_cu10.npz", W.cu_morse(10, jitter_a=0.05, seed=102), 0.5)

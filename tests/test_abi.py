@@ -46,6 +46,23 @@ def test_library_loads_and_tables_match_oracle(oracle):
             np.testing.assert_allclose(x, y, rtol=1e-14, atol=1e-15)
 
 
+@pytest.mark.skipif(not os.path.exists(LIB), reason="HIP library not built")
+def test_library_builds_every_reference_potential():
+    """The engine's host table builder against the compiled reference's tables for sine,
+    cosine, soft, John, MorseC, BuckC and LJC (golden), as the oracle is."""
+    from conftest import golden
+    from ndyn_omp_amd import engine as G
+    g = golden("tables_more.npz")
+    for name in ("sine", "cosine", "soft", "john", "morsec", "buckc", "ljc"):
+        P, F, rc2, dr2 = G.pair_tables([int(g[f"{name}_kind"])], g[f"{name}_params"],
+                                       float(g["ptrmin"]), int(g["ptsize"]),
+                                       qprd=[float(g[f"{name}_qprd"])])
+        assert rc2[0] == g[f"{name}_rc2"] and dr2[0] == g[f"{name}_dr2"], name
+        for got, want in ((P[0], g[f"{name}_P"]), (F[0], g[f"{name}_F"])):
+            np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-13 * np.abs(want).max(),
+                                       err_msg=name)
+
+
 def test_fortran_module_binds_every_entry_point():
     """ndyn_omp_amd/fortran/ndg_iface.f90 declares a bind(C) interface for every
     function of include/ndyn_gpu.h."""

@@ -72,7 +72,7 @@ END
 
 @needs_driver
 def test_driver_rejects_unknown_potential(tmp_path):
-    deck = "START\nTypes 1\n Cu 63.546\nPairs 1\n Cu Cu sym sine 1.0\nEND\n"
+    deck = "START\nTypes 1\n Cu 63.546\nPairs 1\n Cu Cu sym tersoff 1.0\nEND\n"
     p = run_deck(deck, tmp_path)
     assert p.returncode != 0
     assert "No potential available for Potential Type" in p.stderr
@@ -335,3 +335,37 @@ END
     np.testing.assert_allclose(x1[pull, 0] - x0[pull, 0], 100 * 1e-5, atol=1.1e-5)
     free = ~fixed & ~pull & ~ball
     assert np.abs(x1[free] - x0[free]).max() > 1e-4
+
+
+@pytest.mark.gpu
+@needs_driver
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_driver_sinetest_soft_deck(tmp_path, precision):
+    """examples/SineTest/input.sh (the 'soft' potential on the MorseTest lattice, Skin 1.0;
+    Run shortened to 400 steps): examples/SineTest/out:36-46 -> PE 26491.8, pressure
+    -55.6990, volume 49836.0 at steps 1, 40, ..., 400.  The pressure follows the
+    reference's soft force, which reads parm(1), parm(2) where the energy reads parm(2),
+    parm(3) (src/potentials.f90:86, :176)."""
+    write_fcc_dump(tmp_path / "model.dump")
+    deck = f"""START
+Model 	model.dump dump
+Skin	1.0
+Types 1  #atom and particle types
+ Cu 63.546
+Pairs 1
+ Cu Cu sym soft 6.0 0.5 6.0
+Precision {precision}
+OutFreq 40
+ConfOutFreq 0 dump # configuration output frequency
+Run 400 0.00001    # run for [i] steps each at [r] deltat
+END
+"""
+    print(f"case sinetest {precision}", flush=True)
+    p = run_deck(deck, tmp_path)
+    assert p.returncode == 0, p.stderr
+    rows = thermo(p.stdout)
+    assert [int(r[0]) for r in rows] == [1] + list(range(40, 401, 40))
+    for r in rows:
+        assert float(f"{r[1]:.6g}") == 26491.8
+        assert float(f"{r[4]:.6g}") == -55.699
+        assert float(f"{r[5]:.6g}") == 49836.0

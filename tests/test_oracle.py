@@ -28,6 +28,50 @@ def test_tables_match_reference(oracle, name, kind):
     np.testing.assert_allclose(F[0], g[f"{name}_F"], rtol=1e-13, atol=1e-14)
 
 
+MORE = ["sine", "cosine", "soft", "john", "morsec", "buckc", "ljc"]
+
+
+@pytest.mark.parametrize("name", MORE)
+def test_more_tables_match_reference(oracle, name):
+    """The other potentials of buildPairTables (src/potentials.f90:82-147, 172-241) against
+    the compiled reference's tables: sine, cosine, soft, John and the Coulomb variants
+    MorseC / BuckC / LJC with a nonzero charge product."""
+    g = golden("tables_more.npz")
+    P, F, rc2, dr2 = oracle.pair_tables([int(g[f"{name}_kind"])], g[f"{name}_params"],
+                                        float(g["ptrmin"]), int(g["ptsize"]),
+                                        qprd=[float(g[f"{name}_qprd"])])
+    assert rc2[0] == g[f"{name}_rc2"] and dr2[0] == g[f"{name}_dr2"]
+    for got, want in ((P[0], g[f"{name}_P"]), (F[0], g[f"{name}_F"])):
+        np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-13 * np.abs(want).max())
+
+
+def test_more_tables_closed_form(oracle):
+    """Independent forms (numpy, scipy's erfc): the Coulomb variants add
+    q (erfc(a r)/r - vcon + fcon (r - rc)) to the energy and r q (erfc(a r)/r^2 +
+    2a/sqrt(pi) exp(-a^2 r^2)/r - fcon) to the force column, a = 0.2
+    (src/potentials.f90:11,23-26,106-108,199-201); soft's force column is
+    r pi p1 sin(pi r/p2)/p2 while its energy is p2 (1 + cos(pi r/p3)) (:86, :176)."""
+    from scipy.special import erfc
+    prm, q = W.CU_MORSE, 0.8
+    P0, F0, _, dr2 = oracle.pair_tables([W.MORSE], prm, 0.25, 5000)
+    P1, F1, _, _ = oracle.pair_tables([W.MORSE_C], prm, 0.25, 5000, qprd=[q])
+    r = np.sqrt(0.25 + np.arange(5001) * dr2[0])
+    a, rc = 0.2, prm[0]
+    vcon = erfc(a * rc) / rc
+    fcon = erfc(a * rc) / rc ** 2 + 2 * a / np.sqrt(np.pi) * np.exp(-a * a * rc * rc) / rc
+    np.testing.assert_allclose(P1[0] - P0[0], q * (erfc(a * r) / r - vcon + fcon * (r - rc)),
+                               rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(F1[0] - F0[0], r * q * (erfc(a * r) / r ** 2 + 2 * a / np.sqrt(np.pi)
+                                                       * np.exp(-a * a * r * r) / r - fcon),
+                               rtol=1e-9, atol=1e-12)
+    sp = (6.0, 0.5, 6.0, 0.0)
+    P, F, _, dr2 = oracle.pair_tables([W.SOFT], sp, 0.25, 5000)
+    r = np.sqrt(0.25 + np.arange(5001) * dr2[0])
+    np.testing.assert_allclose(P[0], 0.5 * (1 + np.cos(np.pi * r / 6.0)), rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(F[0], r * np.pi * 6.0 * np.sin(np.pi * r / 0.5) / 0.5,
+                               rtol=1e-9, atol=1e-9)
+
+
 def test_tables_closed_form(oracle):
     """Grid is uniform in r^2 and values are the analytic forms; Morse F is -V'(r)
     (reference quirk), LJ/Buck F are -r V'(r)."""
