@@ -196,7 +196,8 @@ class Simulation:
         _check(L.ndg_create(C.byref(h), system.np, system.ntypes, system.npairs, system.ptsize,
                             int(neimax), int(path)))
         self.h = h
-        self.tables = pair_tables(system.kinds, system.params, system.ptrmin, system.ptsize)
+        self.tables = pair_tables(system.kinds, system.params, system.ptrmin, system.ptsize,
+                                  system.qprd)
         P, F, rc2, dr2 = self.tables
         _check(L.ndg_set_tables(h, np.ascontiguousarray(P), np.ascontiguousarray(F), rc2, dr2,
                                 float(system.ptrmin),
@@ -392,9 +393,11 @@ class BatchEngine:
         _check(load_library().ndg_engine_size_classes(self.e, len(classes), npm, tpa))
 
     def admit(self, s: W.System, rp: RunParams, neimax=0, raise_on_full=True):
-        key = (s.kinds.tobytes(), s.params.tobytes(), s.ptrmin, s.ptsize)
+        q = None if s.qprd is None else np.asarray(s.qprd, dtype=np.float64)
+        key = (s.kinds.tobytes(), s.params.tobytes(), s.ptrmin, s.ptsize,
+               None if q is None else q.tobytes())
         if key not in self._tables:
-            self._tables[key] = pair_tables(s.kinds, s.params, s.ptrmin, s.ptsize)
+            self._tables[key] = pair_tables(s.kinds, s.params, s.ptrmin, s.ptsize, q)
         P, F, rc2, dr2 = self._tables[key]
         slot = C.c_int(-1)
         vel = np.ascontiguousarray(s.vel, dtype=np.float64)

@@ -79,6 +79,8 @@ class System:
     # initial positions (scaled, (np,3)) and box of initPositionSet, for 'spring' pairs
     ipos: Optional[np.ndarray] = None
     init_box: Optional[np.ndarray] = None
+    # charge product of each pair (PairChargeProd), for the Coulomb kinds
+    qprd: Optional[np.ndarray] = None
 
     @property
     def np(self) -> int:

@@ -245,7 +245,7 @@ def run(sys, nsteps, dt, skin=-1.0, out_steps=None, step0=0, neimax=0, renei=Tru
     holds=(hold[np], bc_disp[nhold], bc_dir[nhold,3], bc_parm[nhold,3]) sets Hold/Disp.
     Returns (state dict, thermo list, err); the state holds ipos/init_box."""
     if tables is None:
-        tables = pair_tables(sys.kinds, sys.params, sys.ptrmin, sys.ptsize)
+        tables = pair_tables(sys.kinds, sys.params, sys.ptrmin, sys.ptsize, sys.qprd)
     P, F, rc2, dr2 = tables
     rp = RunParams()
     rp.nsteps, rp.dt = nsteps, dt

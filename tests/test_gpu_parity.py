@@ -271,3 +271,36 @@ def test_small_box_brute_force_list(oracle):
         for i in range(s.np):
             np.testing.assert_array_equal(nbr[i, :counts[i]], ref[i])
         sim.close()
+
+
+@pytest.mark.parametrize("kind_name,kind,prm,q", [
+    ("LJC", W.LJ_C, W.AR_LJ, 0.05),
+    ("soft", W.SOFT, (6.0, 0.5, 6.0, 0.0), 0.0),
+    ("John", W.JOHNSON, (6.0, 0.3, 2.8, 5.5), 0.0)])
+def test_other_potentials_through_the_device(kind_name, kind, prm, q, oracle):
+    """The force kernels read whatever table the builder wrote: a thermalised argon-like
+    block with the Coulomb-added LJ (charge product 0.05), soft and John tables.  FP64 path
+    forces, energy and virial within 1e-12 relative of the oracle's compute0 on the oracle's
+    own tables; mixed path RMS relative force error <= 1e-5, energy 1e-6."""
+    s = W.ar_lj(7, temp_k=90.0, seed=191, jitter_a=0.1)
+    s.kinds = np.array([kind], np.int32)
+    s.params = np.array([prm], np.float64)
+    s.qprd = np.array([q])
+    tables = oracle.pair_tables(s.kinds, s.params, s.ptrmin, s.ptsize, s.qprd)
+    _, vnl, nm = oracle.half_list(s, 0.4)
+    f_o, pe_o, vir_o, _ = oracle.compute0(s, vnl, nm, tables)
+    assert np.abs(f_o).max() > 1e-6
+    for path in (G.PATH_FP64, G.PATH_MIXED):
+        print(f"case {kind_name} path {path}", flush=True)
+        sim = G.Simulation(s, path=path)
+        sim.build_lists(0.4)
+        pe, vir = sim.compute(True, True)
+        f = sim.state()["force"]
+        sim.close()
+        if path == G.PATH_FP64:
+            assert rms_rel(f, f_o) < 1e-12
+            assert pe == pytest.approx(pe_o, rel=1e-12, abs=1e-12)
+            np.testing.assert_allclose(vir, vir_o, rtol=1e-12, atol=1e-12 * np.abs(vir_o).max())
+        else:
+            assert rms_rel(f, f_o) <= 1e-5
+            assert pe == pytest.approx(pe_o, rel=1e-6, abs=1e-9)
