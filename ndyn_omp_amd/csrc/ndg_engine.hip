@@ -293,6 +293,9 @@ struct ndg_engine {
   // output snapshot: device staging and pinned host copy of one replica's output state
   double *snap_d = nullptr, *snap_h = nullptr;
   size_t snap_cap = 0, snap_hcap = 0;
+  // pinned staging of state uploads (admission, ndg_set_state)
+  char *stage_h = nullptr;
+  size_t stage_cap = 0;
   hipEvent_t snap_ev = nullptr;
   int snap_slot = -1, snap_np = 0;
   // timing of the force kernel
@@ -854,36 +857,52 @@ static int upload_host(ndg_engine *e, Slot &s, const HostState &h, bool state_to
   HIPCHK(hipMemcpyAsync((void *)P.rc2f, rc2f.data(), 4 * h.npairs, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync((void *)P.vdr2, vdr2.data(), 4 * h.npairs, hipMemcpyHostToDevice, st));
   if (state_too) {
-    std::vector<double> soa(3 * (size_t)np);
-    auto put3 = [&](const std::vector<double> &aos, double *dx, double *dy, double *dz) -> int {
-      for (int i = 0; i < np; ++i) {
-        soa[i] = aos.empty() ? 0.0 : aos[3 * i];
-        soa[np + i] = aos.empty() ? 0.0 : aos[3 * i + 1];
-        soa[2 * np + i] = aos.empty() ? 0.0 : aos[3 * i + 2];
-      }
-      HIPCHK(hipMemcpyAsync(dx, soa.data(), 8 * np, hipMemcpyHostToDevice, st));
-      HIPCHK(hipMemcpyAsync(dy, soa.data() + np, 8 * np, hipMemcpyHostToDevice, st));
-      HIPCHK(hipMemcpyAsync(dz, soa.data() + 2 * np, 8 * np, hipMemcpyHostToDevice, st));
+    // the per-atom state is laid out in one pinned staging area, so its copies run
+    // asynchronously and are waited for once, below.  It goes up in caller order, so the
+    // initial positions and boundary conditions go up as given.
+    const bool ip = P.ipx && h.have_ipos;
+    const size_t nd = (size_t)np * 8;
+    size_t off[8], tot = 0;
+    auto seg = [&](int k, size_t bytes) { off[k] = tot; tot = align_up(tot + bytes, 256); };
+    seg(0, 3 * nd);                    // pos  (SoA)
+    seg(1, 3 * nd);                    // vel
+    seg(2, 3 * nd);                    // acc
+    seg(3, ip ? 3 * nd : 0);           // initial positions
+    seg(4, 16 * (size_t)np);           // FP32 force (F = m a)
+    seg(5, 4 * (size_t)np);            // types
+    seg(6, 4 * (size_t)np);            // caller order
+    seg(7, P.hbc ? 4 * (size_t)np : 0);
+    if (tot > e->stage_cap) {
       HIPCHK(hipStreamSynchronize(st));
+      if (e->stage_h) HIPCHK(hipHostFree(e->stage_h));
+      e->stage_h = nullptr;
+      e->stage_cap = 0;
+      HIPCHK(hipHostMalloc((void **)&e->stage_h, tot, hipHostMallocDefault));
+      e->stage_cap = tot;
+    }
+    char *b = e->stage_h;
+    auto put3 = [&](int k, const std::vector<double> &aos, double *dx, double *dy, double *dz) -> int {
+      double *x = (double *)(b + off[k]);
+      for (int i = 0; i < np; ++i) {
+        x[i] = aos.empty() ? 0.0 : aos[3 * i];
+        x[np + i] = aos.empty() ? 0.0 : aos[3 * i + 1];
+        x[2 * np + i] = aos.empty() ? 0.0 : aos[3 * i + 2];
+      }
+      HIPCHK(hipMemcpyAsync(dx, x, nd, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(dy, x + np, nd, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(dz, x + 2 * np, nd, hipMemcpyHostToDevice, st));
       return NDG_OK;
     };
     int rc;
-    if ((rc = put3(h.pos, P.px, P.py, P.pz))) return rc;
-    if ((rc = put3(h.vel, P.vx, P.vy, P.vz))) return rc;
-    // the state goes up in caller order, so the initial positions and boundary conditions
-    // go up as given
-    if (P.ipx && h.have_ipos) {
-      if ((rc = put3(h.ipos, P.ipx, P.ipy, P.ipz))) return rc;
+    if ((rc = put3(0, h.pos, P.px, P.py, P.pz))) return rc;
+    if ((rc = put3(1, h.vel, P.vx, P.vy, P.vz))) return rc;
+    if ((rc = put3(2, h.acc, P.ax, P.ay, P.az))) return rc;
+    if (ip) {
+      if ((rc = put3(3, h.ipos, P.ipx, P.ipy, P.ipz))) return rc;
       s.ipos_dirty = false;
     }
-    if (P.hbc) {
-      HIPCHK(hipMemcpyAsync(P.hbc, h.hbc.data(), 4 * (size_t)np, hipMemcpyHostToDevice, st));
-      HIPCHK(hipStreamSynchronize(st));
-      s.hbc_dirty = false;
-    }
-    if ((rc = put3(h.acc, P.ax, P.ay, P.az))) return rc;
     // mixed path keeps the latest force (F = m a) in FP32
-    std::vector<float4> f4(np);
+    float4 *f4 = (float4 *)(b + off[4]);
     for (int i = 0; i < np; ++i) {
       const double m = h.mass[h.typ[i] - 1];
       if (h.acc.empty())
@@ -892,12 +911,21 @@ static int upload_host(ndg_engine *e, Slot &s, const HostState &h, bool state_to
         f4[i] = make_float4((float)(h.acc[3 * i] * m), (float)(h.acc[3 * i + 1] * m),
                             (float)(h.acc[3 * i + 2] * m), 0.f);
     }
-    std::vector<int> oid(np);
-    for (int i = 0; i < np; ++i) oid[i] = i;
-    HIPCHK(hipMemcpyAsync(P.f32a, f4.data(), 16 * np, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(P.f32b, f4.data(), 16 * np, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(P.typ, h.typ.data(), 4 * np, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(P.oid, oid.data(), 4 * np, hipMemcpyHostToDevice, st));
+    int *ty = (int *)(b + off[5]), *oid = (int *)(b + off[6]);
+    for (int i = 0; i < np; ++i) {
+      ty[i] = h.typ[i];
+      oid[i] = i;
+    }
+    HIPCHK(hipMemcpyAsync(P.f32a, f4, 16 * np, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(P.f32b, f4, 16 * np, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(P.typ, ty, 4 * np, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(P.oid, oid, 4 * np, hipMemcpyHostToDevice, st));
+    if (P.hbc) {
+      int *hb = (int *)(b + off[7]);
+      std::copy(h.hbc.begin(), h.hbc.end(), hb);
+      HIPCHK(hipMemcpyAsync(P.hbc, hb, 4 * (size_t)np, hipMemcpyHostToDevice, st));
+      s.hbc_dirty = false;
+    }
     HIPCHK(hipMemsetAsync(P.fx, 0, 8 * np, st));
     HIPCHK(hipMemsetAsync(P.fy, 0, 8 * np, st));
     HIPCHK(hipMemsetAsync(P.fz, 0, 8 * np, st));
@@ -1462,6 +1490,7 @@ extern "C" int ndg_engine_destroy(ndg_engine_t *e) {
   if (e->snap_d) hipFree(e->snap_d);
   if (e->snap_h) hipHostFree(e->snap_h);
   if (e->snap_ev) hipEventDestroy(e->snap_ev);
+  if (e->stage_h) hipHostFree(e->stage_h);
   hipStreamDestroy(e->stream);
   delete e;
   return NDG_OK;
