@@ -1683,13 +1683,13 @@ static int begin_run(ndg_engine *e, int r, HostState &h, const ndg_run_params &r
   return build_with_growth(e, r, skin);
 }
 
-extern "C" int ndg_engine_admit(ndg_engine_t *e, int *slot, int np, int ntypes, int npairs,
-                                int ptsize, int neimax, const double *P, const double *F,
-                                const double *rc2, const double *dr2, double ptrmin,
-                                const int *pairmat, const double *pairparams,
-                                const double *mass, const double *pos, const double *vel,
-                                const int *typ, const double box[3],
-                                const ndg_run_params *rp) {
+static int engine_admit(ndg_engine_t *e, int *slot, int np, int ntypes, int npairs, int ptsize,
+                        int neimax, const double *P, const double *F, const double *rc2,
+                        const double *dr2, double ptrmin, const int *pairmat,
+                        const double *pairparams, const double *mass, const double *pos,
+                        const double *vel, const int *typ, const double box[3],
+                        const ndg_run_params *rp, const int *kinds, const double *ipos,
+                        const double *init_box) {
   if (!e || !slot || !rp || !pos || !typ || !box) return fail(NDG_ERR_ARG, "ndg_engine_admit: null argument");
   HostState h;
   fill_host(h, np, ntypes, npairs, ptsize, P, F, rc2, dr2, ptrmin, pairmat, pairparams, mass);
@@ -1699,6 +1699,18 @@ extern "C" int ndg_engine_admit(ndg_engine_t *e, int *slot, int np, int ntypes, 
   for (int a = 0; a < 3; ++a) h.box[a] = box[a];
   int rc = check_host(h);
   if (rc) return rc;
+  if (kinds) {
+    h.kinds.assign(kinds, kinds + npairs);
+    for (int v : h.kinds)
+      if (v < NDG_MORSE || v > NDG_LJ_C) return fail(NDG_ERR_ARG, "ndg_engine_admit_kinds: unknown kind");
+    if (ipos) {
+      // given initial positions; without them the Run's first state is taken, as
+      // initPositionSet at the start of dynamics
+      h.ipos.assign(ipos, ipos + 3 * (size_t)np);
+      for (int a = 0; a < 3; ++a) h.ibox[a] = init_box[a];
+      h.have_ipos = true;
+    }
+  }
   if (e->path == NDG_PATH_FP64 && e->tpa_override > 1) e->tpa_override = 1;
   const int r = new_slot(e);
   rc = begin_run(e, r, h, *rp, 0, neimax, true);
@@ -1712,6 +1724,33 @@ extern "C" int ndg_engine_admit(ndg_engine_t *e, int *slot, int np, int ntypes, 
   }
   *slot = r;
   return NDG_OK;
+}
+
+extern "C" int ndg_engine_admit(ndg_engine_t *e, int *slot, int np, int ntypes, int npairs,
+                                int ptsize, int neimax, const double *P, const double *F,
+                                const double *rc2, const double *dr2, double ptrmin,
+                                const int *pairmat, const double *pairparams,
+                                const double *mass, const double *pos, const double *vel,
+                                const int *typ, const double box[3],
+                                const ndg_run_params *rp) {
+  return engine_admit(e, slot, np, ntypes, npairs, ptsize, neimax, P, F, rc2, dr2, ptrmin,
+                      pairmat, pairparams, mass, pos, vel, typ, box, rp, nullptr, nullptr,
+                      nullptr);
+}
+
+extern "C" int ndg_engine_admit_kinds(ndg_engine_t *e, int *slot, int np, int ntypes,
+                                      int npairs, int ptsize, int neimax, const double *P,
+                                      const double *F, const double *rc2, const double *dr2,
+                                      double ptrmin, const int *pairmat,
+                                      const double *pairparams, const double *mass,
+                                      const double *pos, const double *vel, const int *typ,
+                                      const double box[3], const ndg_run_params *rp,
+                                      const int *kinds, const double *ipos,
+                                      const double init_box[3]) {
+  if (!kinds || (ipos && !init_box))
+    return fail(NDG_ERR_ARG, "ndg_engine_admit_kinds: null argument");
+  return engine_admit(e, slot, np, ntypes, npairs, ptsize, neimax, P, F, rc2, dr2, ptrmin,
+                      pairmat, pairparams, mass, pos, vel, typ, box, rp, kinds, ipos, init_box);
 }
 
 extern "C" int ndg_engine_step(ndg_engine_t *e, int nsteps, int *ndone) {

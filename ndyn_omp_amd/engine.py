@@ -103,6 +103,9 @@ def load_library():
         "ndg_engine_destroy": [V],
         "ndg_engine_admit": [V, P(C.c_int), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp,
                              dp, dp, C.c_double, ip, dp, dp, dp, V, ip, dp, P(RunParams)],
+        "ndg_engine_admit_kinds": [V, P(C.c_int), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                   dp, dp, dp, dp, C.c_double, ip, dp, dp, dp, V, ip, dp,
+                                   P(RunParams), ip, V, V],
         "ndg_engine_step": [V, C.c_int, P(C.c_int)],
         "ndg_engine_status": [V, C.c_int, P(C.c_int), P(C.c_int), P(Thermo), P(C.c_int)],
         "ndg_engine_thermo": [V, C.c_int, P(Thermo), C.c_int, P(C.c_int)],
@@ -401,14 +404,22 @@ class BatchEngine:
         P, F, rc2, dr2 = self._tables[key]
         slot = C.c_int(-1)
         vel = np.ascontiguousarray(s.vel, dtype=np.float64)
-        rc = load_library().ndg_engine_admit(
-            self.e, C.byref(slot), s.np, s.ntypes, s.npairs, s.ptsize, int(neimax),
-            np.ascontiguousarray(P), np.ascontiguousarray(F), rc2, dr2, float(s.ptrmin),
-            np.ascontiguousarray(s.pairmat.T).ravel().astype(np.int32),
-            np.ascontiguousarray(s.params, dtype=np.float64).ravel(),
-            np.ascontiguousarray(s.masses, dtype=np.float64),
-            np.ascontiguousarray(s.pos, dtype=np.float64), _ptr(vel), s.typ.astype(np.int32),
-            np.ascontiguousarray(s.box, dtype=np.float64), C.byref(rp))
+        args = (self.e, C.byref(slot), s.np, s.ntypes, s.npairs, s.ptsize, int(neimax),
+                np.ascontiguousarray(P), np.ascontiguousarray(F), rc2, dr2, float(s.ptrmin),
+                np.ascontiguousarray(s.pairmat.T).ravel().astype(np.int32),
+                np.ascontiguousarray(s.params, dtype=np.float64).ravel(),
+                np.ascontiguousarray(s.masses, dtype=np.float64),
+                np.ascontiguousarray(s.pos, dtype=np.float64), _ptr(vel), s.typ.astype(np.int32),
+                np.ascontiguousarray(s.box, dtype=np.float64), C.byref(rp))
+        if s.springs:
+            # spring pairs need their kinds and the initial positions of their strains
+            # (None: the admitted state, as initPositionSet)
+            ipos = None if s.ipos is None else np.ascontiguousarray(s.ipos, dtype=np.float64)
+            ibox = None if s.ipos is None else np.ascontiguousarray(s.init_box, dtype=np.float64)
+            rc = load_library().ndg_engine_admit_kinds(
+                *args, np.ascontiguousarray(s.kinds, dtype=np.int32), _ptr(ipos), _ptr(ibox))
+        else:
+            rc = load_library().ndg_engine_admit(*args)
         if rc == ERR_NOMEM and not raise_on_full:
             return None
         _check(rc)

@@ -305,6 +305,23 @@ module ndg_iface
       type(ndg_run_params), intent(in) :: rp
     end function
 
+    ! ipos: c_loc of the initial positions (3,np), or c_null_ptr to take the admitted state
+    integer(c_int) function ndg_engine_admit_kinds(e, slot, np, ntypes, npairs, ptsize, &
+        neimax, P, F, rc2, dr2, ptrmin, pairmat, pairparams, mass, pos, vel, typ, box, rp, &
+        kinds, ipos, init_box) bind(C, name='ndg_engine_admit_kinds')
+      import :: c_int, c_double, c_ptr, ndg_run_params
+      type(c_ptr), value :: e
+      integer(c_int), intent(out) :: slot
+      integer(c_int), value :: np, ntypes, npairs, ptsize, neimax
+      real(c_double), intent(in) :: P(*), F(*), rc2(*), dr2(*)
+      real(c_double), value :: ptrmin
+      integer(c_int), intent(in) :: pairmat(*), typ(*), kinds(*)
+      real(c_double), intent(in) :: pairparams(*), mass(*), pos(*), vel(*), box(3)
+      type(ndg_run_params), intent(in) :: rp
+      type(c_ptr), value :: ipos
+      real(c_double), intent(in) :: init_box(3)
+    end function
+
     integer(c_int) function ndg_engine_step(e, nsteps, ndone) bind(C, name='ndg_engine_step')
       import :: c_int, c_ptr
       type(c_ptr), value :: e

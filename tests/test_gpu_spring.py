@@ -222,3 +222,53 @@ def same_6(got, want):
     if want == 0.0:
         return abs(got) < 1e-12
     return abs(got - want) <= 10.0 ** (math.floor(math.log10(abs(want))) - 5)
+
+
+@pytest.mark.parametrize("path_name,path", [("fp64", 0), ("mixed", 1)])
+def test_spring_replicas_in_a_batch(path_name, path):
+    """Spring systems admitted into the batched engine (ndg_engine_admit_kinds) beside a
+    replica without springs, so the spring kernel instance also runs a replica that has
+    none.  run_spring_pd takes its initial positions from the admitted state (as
+    initPositionSet at the start of 'dynamics'), run_spring_mix1_nve is given them.  Bar:
+    every replica bit-identical (positions, velocities, box, thermo) to its run through the
+    single-system handle, and the spring replicas within the reference bars above."""
+    cases = []
+    for fname in ["run_spring_pd.npz", "run_spring_mix1_nve.npz"]:
+        g = golden(fname)
+        rp = G.run_params(int(g["nsteps"]), float(g["dt"]), out_steps=int(g["outsteps"]),
+                          skin=float(g["skin"]), **_run_kw(g))
+        cases.append((fname, system_from_golden(g), rp, g))
+    plain = W.cu_morse(6, temp_k=700.0, seed=31)
+    cases.append(("cu_morse_6", plain, G.run_params(40, W.DT_1FS, out_steps=10, skin=0.5), None))
+    alone = {}
+    for name, s, rp, _ in cases:
+        print(f"case {name} {path_name} alone", flush=True)
+        sim = G.Simulation(s.copy(), path=path)
+        recs, _, _ = sim.run(rp)
+        alone[name] = (sim.state(), recs)
+        sim.close()
+    print(f"case batch {path_name}", flush=True)
+    eng = G.BatchEngine(path=path)
+    slots = [eng.admit(s.copy(), rp) for _, s, rp, _ in cases]
+    for _ in range(200):
+        if eng.step(64) == len(slots):
+            break
+    for (name, s, rp, g), sl in zip(cases, slots):
+        st = eng.status(sl)
+        assert st["finished"] and not st["err"], name
+        recs = eng.thermo(sl)
+        got = eng.fetch(sl, s.np)
+        ref, ref_recs = alone[name]
+        np.testing.assert_array_equal(got["pos"], ref["pos"], err_msg=name)
+        np.testing.assert_array_equal(got["vel"], ref["vel"], err_msg=name)
+        np.testing.assert_array_equal(got["box"], ref["box"], err_msg=name)
+        assert len(recs) == len(ref_recs), name
+        for rec, want in zip(recs, ref_recs):     # 'eerr' is NaN before the first energy
+            for k in want:
+                np.testing.assert_array_equal(rec[k], want[k], err_msg=f"{name} {k}")
+        if g is not None:
+            np.testing.assert_allclose(got["box"], g["out_box"], rtol=1e-12)
+            assert np.abs((got["pos"] - g["out_pos"]) * g["out_box"]).max() < 1e-5
+            for row, rec in zip(g["thermo"], recs):
+                assert rec["pe"] == pytest.approx(row[1], rel=1e-5, abs=1e-9)
+    eng.close()
