@@ -248,18 +248,21 @@ int ndg_engine_admit(ndg_engine_t *e, int *slot, int np, int ntypes, int npairs,
                      const double *vel, const int *typ, const double box[3],
                      const ndg_run_params *rp);
 
-/* ndg_engine_admit for a system with spring pairs (or other non-default pair kinds):
- * kinds[npairs] as in ndg_set_pair_kinds; ipos (3*np, caller order) and init_box are the
- * initial positions and box of the spring strains, or ipos = NULL to take the admitted
- * state, as initPositionSet at the start of 'dynamics' does.  Replaces, for a batched
- * replica, the compute2/compute3 set-up of src/nd.f90:103 and :1731-1763. */
-int ndg_engine_admit_kinds(ndg_engine_t *e, int *slot, int np, int ntypes, int npairs,
-                           int ptsize, int neimax, const double *P, const double *F,
-                           const double *rc2, const double *dr2, double ptrmin,
-                           const int *pairmat, const double *pairparams, const double *mass,
-                           const double *pos, const double *vel, const int *typ,
-                           const double box[3], const ndg_run_params *rp, const int *kinds,
-                           const double *ipos, const double init_box[3]);
+/* ndg_engine_admit with a deck's per-system extras, for a batched replica:
+ *  - kinds[npairs] as in ndg_set_pair_kinds (NULL: none) -- spring pairs (compute2/compute3);
+ *    ipos (3*np, caller order) and init_box are the initial positions and box of the spring
+ *    strains, or ipos = NULL to take the admitted state, as initPositionSet at the start of
+ *    'dynamics' does (src/nd.f90:103, :1731-1763);
+ *  - hold[np], nhold, bc_disp, bc_dir, bc_parm as in ndg_set_holds (nhold = 0: none) --
+ *    the Hold/Disp boundary conditions of integrate1 (src/nd.f90:1048-1075). */
+int ndg_engine_admit_ex(ndg_engine_t *e, int *slot, int np, int ntypes, int npairs, int ptsize,
+                        int neimax, const double *P, const double *F, const double *rc2,
+                        const double *dr2, double ptrmin, const int *pairmat,
+                        const double *pairparams, const double *mass, const double *pos,
+                        const double *vel, const int *typ, const double box[3],
+                        const ndg_run_params *rp, const int *kinds, const double *ipos,
+                        const double init_box[3], const int *hold, int nhold,
+                        const int *bc_disp, const int *bc_dir, const double *bc_parm);
 
 /* Advance every admitted, unfinished system by up to nsteps steps in batched launches.
  * Systems stop at the end of their Run.  *ndone = systems that are finished. */

@@ -1582,6 +1582,36 @@ static int upload_hbc_sorted(ndg_engine *e, Slot &s, const HostState &h) {
   return NDG_OK;
 }
 
+// Compact the hold groups with a displacement condition on some axis into the device
+// table (at most 16) and the per-atom index into it; atoms of other groups integrate
+// freely (src/nd.f90:1058-1064).
+static int holds_to_host(HostState &hs, const int *hold, int nhold, const int *bc_disp,
+                         const int *bc_dir, const double *bc_parm) {
+  std::vector<int> index(nhold + 1, 0), dir;
+  std::vector<double> du;
+  for (int g = 1; g <= nhold; ++g) {
+    const int d = (bc_dir[3 * (g - 1)] ? 1 : 0) | (bc_dir[3 * (g - 1) + 1] ? 2 : 0) |
+                  (bc_dir[3 * (g - 1) + 2] ? 4 : 0);
+    if (!bc_disp[g - 1] || !d) continue;
+    if ((int)dir.size() == 16) return fail(NDG_ERR_ARG, "holds: at most 16 hold groups with Disp");
+    dir.push_back(d);
+    for (int a = 0; a < 3; ++a) du.push_back(bc_parm[3 * (g - 1) + a]);
+    index[g] = (int)dir.size();
+  }
+  std::vector<int> hbc(hs.np, 0);
+  if (!dir.empty())
+    for (int i = 0; i < hs.np; ++i) {
+      const int g = hold[i];
+      if (g < 0 || g > nhold) return fail(NDG_ERR_ARG, "holds: hold id out of range");
+      hbc[i] = index[g];
+    }
+  hs.hbc.swap(hbc);
+  hs.bc_dir.swap(dir);
+  hs.bc_du.swap(du);
+  if (!hs.holds()) hs.hbc.clear();
+  return NDG_OK;
+}
+
 // Make slot r resident for a Run: allocate for the list capacity, upload, build lists.
 static int begin_run(ndg_engine *e, int r, HostState &h, const ndg_run_params &rp, int step0,
                      int neimax_req, bool fresh_state) {
@@ -1689,7 +1719,8 @@ static int engine_admit(ndg_engine_t *e, int *slot, int np, int ntypes, int npai
                         const double *pairparams, const double *mass, const double *pos,
                         const double *vel, const int *typ, const double box[3],
                         const ndg_run_params *rp, const int *kinds, const double *ipos,
-                        const double *init_box) {
+                        const double *init_box, const int *hold, int nhold,
+                        const int *bc_disp, const int *bc_dir, const double *bc_parm) {
   if (!e || !slot || !rp || !pos || !typ || !box) return fail(NDG_ERR_ARG, "ndg_engine_admit: null argument");
   HostState h;
   fill_host(h, np, ntypes, npairs, ptsize, P, F, rc2, dr2, ptrmin, pairmat, pairparams, mass);
@@ -1702,7 +1733,7 @@ static int engine_admit(ndg_engine_t *e, int *slot, int np, int ntypes, int npai
   if (kinds) {
     h.kinds.assign(kinds, kinds + npairs);
     for (int v : h.kinds)
-      if (v < NDG_MORSE || v > NDG_LJ_C) return fail(NDG_ERR_ARG, "ndg_engine_admit_kinds: unknown kind");
+      if (v < NDG_MORSE || v > NDG_LJ_C) return fail(NDG_ERR_ARG, "ndg_engine_admit_ex: unknown kind");
     if (ipos) {
       // given initial positions; without them the Run's first state is taken, as
       // initPositionSet at the start of dynamics
@@ -1711,6 +1742,7 @@ static int engine_admit(ndg_engine_t *e, int *slot, int np, int ntypes, int npai
       h.have_ipos = true;
     }
   }
+  if (nhold > 0 && (rc = holds_to_host(h, hold, nhold, bc_disp, bc_dir, bc_parm))) return rc;
   if (e->path == NDG_PATH_FP64 && e->tpa_override > 1) e->tpa_override = 1;
   const int r = new_slot(e);
   rc = begin_run(e, r, h, *rp, 0, neimax, true);
@@ -1735,22 +1767,25 @@ extern "C" int ndg_engine_admit(ndg_engine_t *e, int *slot, int np, int ntypes, 
                                 const ndg_run_params *rp) {
   return engine_admit(e, slot, np, ntypes, npairs, ptsize, neimax, P, F, rc2, dr2, ptrmin,
                       pairmat, pairparams, mass, pos, vel, typ, box, rp, nullptr, nullptr,
-                      nullptr);
+                      nullptr, nullptr, 0, nullptr, nullptr, nullptr);
 }
 
-extern "C" int ndg_engine_admit_kinds(ndg_engine_t *e, int *slot, int np, int ntypes,
-                                      int npairs, int ptsize, int neimax, const double *P,
-                                      const double *F, const double *rc2, const double *dr2,
-                                      double ptrmin, const int *pairmat,
-                                      const double *pairparams, const double *mass,
-                                      const double *pos, const double *vel, const int *typ,
-                                      const double box[3], const ndg_run_params *rp,
-                                      const int *kinds, const double *ipos,
-                                      const double init_box[3]) {
-  if (!kinds || (ipos && !init_box))
-    return fail(NDG_ERR_ARG, "ndg_engine_admit_kinds: null argument");
+extern "C" int ndg_engine_admit_ex(ndg_engine_t *e, int *slot, int np, int ntypes, int npairs,
+                                   int ptsize, int neimax, const double *P, const double *F,
+                                   const double *rc2, const double *dr2, double ptrmin,
+                                   const int *pairmat, const double *pairparams,
+                                   const double *mass, const double *pos, const double *vel,
+                                   const int *typ, const double box[3],
+                                   const ndg_run_params *rp, const int *kinds,
+                                   const double *ipos, const double init_box[3],
+                                   const int *hold, int nhold, const int *bc_disp,
+                                   const int *bc_dir, const double *bc_parm) {
+  if ((ipos && (!kinds || !init_box)) || nhold < 0 ||
+      (nhold > 0 && (!hold || !bc_disp || !bc_dir || !bc_parm)))
+    return fail(NDG_ERR_ARG, "ndg_engine_admit_ex: bad arguments");
   return engine_admit(e, slot, np, ntypes, npairs, ptsize, neimax, P, F, rc2, dr2, ptrmin,
-                      pairmat, pairparams, mass, pos, vel, typ, box, rp, kinds, ipos, init_box);
+                      pairmat, pairparams, mass, pos, vel, typ, box, rp, kinds, ipos, init_box,
+                      hold, nhold, bc_disp, bc_dir, bc_parm);
 }
 
 extern "C" int ndg_engine_step(ndg_engine_t *e, int nsteps, int *ndone) {
@@ -2005,31 +2040,7 @@ extern "C" int ndg_set_holds(ndg_t *h, const int *hold, int nhold, const int *bc
   int rc = settle(h);
   if (rc) return rc;
   HostState &hs = h->hs;
-  // compact the hold groups with a displacement condition on some axis into the
-  // device table; atoms of other groups integrate freely (src/nd.f90:1058-1064)
-  std::vector<int> index(nhold + 1, 0), dir;
-  std::vector<double> du;
-  for (int g = 1; g <= nhold; ++g) {
-    const int d = (bc_dir[3 * (g - 1)] ? 1 : 0) | (bc_dir[3 * (g - 1) + 1] ? 2 : 0) |
-                  (bc_dir[3 * (g - 1) + 2] ? 4 : 0);
-    if (!bc_disp[g - 1] || !d) continue;
-    if ((int)dir.size() == 16)
-      return fail(NDG_ERR_ARG, "ndg_set_holds: at most 16 hold groups with Disp");
-    dir.push_back(d);
-    for (int a = 0; a < 3; ++a) du.push_back(bc_parm[3 * (g - 1) + a]);
-    index[g] = (int)dir.size();
-  }
-  std::vector<int> hbc(hs.np, 0);
-  if (!dir.empty())
-    for (int i = 0; i < hs.np; ++i) {
-      const int g = hold[i];
-      if (g < 0 || g > nhold) return fail(NDG_ERR_ARG, "ndg_set_holds: hold id out of range");
-      hbc[i] = index[g];
-    }
-  hs.hbc.swap(hbc);
-  hs.bc_dir.swap(dir);
-  hs.bc_du.swap(du);
-  if (!hs.holds()) hs.hbc.clear();
+  if ((rc = holds_to_host(hs, hold, nhold, bc_disp, bc_dir, bc_parm))) return rc;
   Slot &s = h->e->slots[h->slot];
   s.hbc_dirty = true;
   // resident with room for them (or cleared): apply now; a resident block without room

@@ -103,9 +103,9 @@ def load_library():
         "ndg_engine_destroy": [V],
         "ndg_engine_admit": [V, P(C.c_int), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp,
                              dp, dp, C.c_double, ip, dp, dp, dp, V, ip, dp, P(RunParams)],
-        "ndg_engine_admit_kinds": [V, P(C.c_int), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
-                                   dp, dp, dp, dp, C.c_double, ip, dp, dp, dp, V, ip, dp,
-                                   P(RunParams), ip, V, V],
+        "ndg_engine_admit_ex": [V, P(C.c_int), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                dp, dp, dp, dp, C.c_double, ip, dp, dp, dp, V, ip, dp,
+                                P(RunParams), V, V, V, V, C.c_int, V, V, V],
         "ndg_engine_step": [V, C.c_int, P(C.c_int)],
         "ndg_engine_status": [V, C.c_int, P(C.c_int), P(C.c_int), P(Thermo), P(C.c_int)],
         "ndg_engine_thermo": [V, C.c_int, P(Thermo), C.c_int, P(C.c_int)],
@@ -395,7 +395,10 @@ class BatchEngine:
         tpa = np.array([c[1] for c in classes], dtype=np.int32)
         _check(load_library().ndg_engine_size_classes(self.e, len(classes), npm, tpa))
 
-    def admit(self, s: W.System, rp: RunParams, neimax=0, raise_on_full=True):
+    def admit(self, s: W.System, rp: RunParams, neimax=0, raise_on_full=True, holds=None):
+        """Admit one system; holds = (hold, bc_disp, bc_dir, bc_parm) as Simulation.set_holds.
+        Spring systems carry their pair kinds and initial positions (None: the admitted
+        state, as initPositionSet)."""
         q = None if s.qprd is None else np.asarray(s.qprd, dtype=np.float64)
         key = (s.kinds.tobytes(), s.params.tobytes(), s.ptrmin, s.ptsize,
                None if q is None else q.tobytes())
@@ -411,13 +414,23 @@ class BatchEngine:
                 np.ascontiguousarray(s.masses, dtype=np.float64),
                 np.ascontiguousarray(s.pos, dtype=np.float64), _ptr(vel), s.typ.astype(np.int32),
                 np.ascontiguousarray(s.box, dtype=np.float64), C.byref(rp))
-        if s.springs:
-            # spring pairs need their kinds and the initial positions of their strains
-            # (None: the admitted state, as initPositionSet)
-            ipos = None if s.ipos is None else np.ascontiguousarray(s.ipos, dtype=np.float64)
-            ibox = None if s.ipos is None else np.ascontiguousarray(s.init_box, dtype=np.float64)
-            rc = load_library().ndg_engine_admit_kinds(
-                *args, np.ascontiguousarray(s.kinds, dtype=np.int32), _ptr(ipos), _ptr(ibox))
+        if s.springs or holds is not None:
+            kinds = ipos = ibox = None
+            if s.springs:
+                kinds = np.ascontiguousarray(s.kinds, dtype=np.int32)
+                if s.ipos is not None:
+                    ipos = np.ascontiguousarray(s.ipos, dtype=np.float64)
+                    ibox = np.ascontiguousarray(s.init_box, dtype=np.float64)
+            hk = [None] * 4
+            if holds is not None:
+                hk = [np.ascontiguousarray(holds[0], dtype=np.int32),
+                      np.ascontiguousarray(holds[1], dtype=np.int32),
+                      np.ascontiguousarray(holds[2], dtype=np.int32).ravel(),
+                      np.ascontiguousarray(holds[3], dtype=np.float64).ravel()]
+            nhold = 0 if holds is None else len(hk[1])
+            rc = load_library().ndg_engine_admit_ex(
+                *args, _ptr(kinds), _ptr(ipos), _ptr(ibox), _ptr(hk[0]), nhold, _ptr(hk[1]),
+                _ptr(hk[2]), _ptr(hk[3]))
         else:
             rc = load_library().ndg_engine_admit(*args)
         if rc == ERR_NOMEM and not raise_on_full:

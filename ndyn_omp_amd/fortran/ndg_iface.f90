@@ -305,20 +305,22 @@ module ndg_iface
       type(ndg_run_params), intent(in) :: rp
     end function
 
-    ! ipos: c_loc of the initial positions (3,np), or c_null_ptr to take the admitted state
-    integer(c_int) function ndg_engine_admit_kinds(e, slot, np, ntypes, npairs, ptsize, &
+    ! kinds, ipos, hold, bc_disp, bc_dir, bc_parm: c_loc of the arrays or c_null_ptr (no
+    ! pair kinds / take the admitted state as initial positions / no holds, nhold = 0)
+    integer(c_int) function ndg_engine_admit_ex(e, slot, np, ntypes, npairs, ptsize, &
         neimax, P, F, rc2, dr2, ptrmin, pairmat, pairparams, mass, pos, vel, typ, box, rp, &
-        kinds, ipos, init_box) bind(C, name='ndg_engine_admit_kinds')
+        kinds, ipos, init_box, hold, nhold, bc_disp, bc_dir, bc_parm) &
+        bind(C, name='ndg_engine_admit_ex')
       import :: c_int, c_double, c_ptr, ndg_run_params
       type(c_ptr), value :: e
       integer(c_int), intent(out) :: slot
-      integer(c_int), value :: np, ntypes, npairs, ptsize, neimax
+      integer(c_int), value :: np, ntypes, npairs, ptsize, neimax, nhold
       real(c_double), intent(in) :: P(*), F(*), rc2(*), dr2(*)
       real(c_double), value :: ptrmin
-      integer(c_int), intent(in) :: pairmat(*), typ(*), kinds(*)
+      integer(c_int), intent(in) :: pairmat(*), typ(*)
       real(c_double), intent(in) :: pairparams(*), mass(*), pos(*), vel(*), box(3)
       type(ndg_run_params), intent(in) :: rp
-      type(c_ptr), value :: ipos
+      type(c_ptr), value :: kinds, ipos, hold, bc_disp, bc_dir, bc_parm
       real(c_double), intent(in) :: init_box(3)
     end function
 

@@ -84,3 +84,43 @@ def test_hold_through_list_rebuilds(path_name, path, oracle):
     np.testing.assert_allclose(a["pos"], o["pos"], rtol=0, atol=tol)
     # after clearing, the frozen slab moves
     assert np.abs(b["pos"][hold == 1] - a["pos"][hold == 1]).max() > 1e-4
+
+
+@pytest.mark.parametrize("path_name,path", PATHS)
+def test_hold_replica_in_a_batch(path_name, path):
+    """The hold golden admitted into the batched engine with its Hold/Disp groups
+    (ndg_engine_admit_ex) beside a replica without holds.  Bar: both bit-identical
+    (positions, velocities, box) to their runs through the single-system handle, and the
+    hold replica within the reference bars above."""
+    g = golden("run_hold_cu7.npz")
+    s = system_from_golden(g)
+    rp = G.run_params(int(g["nsteps"]), float(g["dt"]), out_steps=int(g["outsteps"]),
+                      skin=float(g["skin"]))
+    free = W.cu_morse(6, temp_k=700.0, seed=31)
+    rp_free = G.run_params(40, W.DT_1FS, out_steps=10, skin=0.5)
+    print(f"case hold {path_name} alone", flush=True)
+    sim = G.Simulation(s.copy(), path=path)
+    sim.set_holds(*_holds(g))
+    sim.run(rp)
+    ref = sim.state()
+    sim.close()
+    sim = G.Simulation(free.copy(), path=path)
+    sim.run(rp_free)
+    ref_free = sim.state()
+    sim.close()
+    print(f"case hold {path_name} batch", flush=True)
+    eng = G.BatchEngine(path=path)
+    slots = [eng.admit(s.copy(), rp, holds=_holds(g)), eng.admit(free.copy(), rp_free)]
+    for _ in range(200):
+        if eng.step(64) == len(slots):
+            break
+    for sl, n, want in [(slots[0], s.np, ref), (slots[1], free.np, ref_free)]:
+        st = eng.status(sl)
+        assert st["finished"] and not st["err"]
+        got = eng.fetch(sl, n)
+        for k in ("pos", "vel", "box"):
+            np.testing.assert_array_equal(got[k], want[k], err_msg=k)
+    got = eng.fetch(slots[0], s.np)
+    eng.close()
+    np.testing.assert_allclose(got["box"], g["out_box"], rtol=1e-12)
+    assert np.abs((got["pos"] - g["out_pos"]) * g["out_box"]).max() < 1e-5

@@ -226,7 +226,7 @@ def same_6(got, want):
 
 @pytest.mark.parametrize("path_name,path", [("fp64", 0), ("mixed", 1)])
 def test_spring_replicas_in_a_batch(path_name, path):
-    """Spring systems admitted into the batched engine (ndg_engine_admit_kinds) beside a
+    """Spring systems admitted into the batched engine (ndg_engine_admit_ex) beside a
     replica without springs, so the spring kernel instance also runs a replica that has
     none.  run_spring_pd takes its initial positions from the admitted state (as
     initPositionSet at the start of 'dynamics'), run_spring_mix1_nve is given them.  Bar:
