@@ -32,6 +32,9 @@ class Job:
     temp_k: float = 300.0
     arrival: float = 0.0           # seconds after the server start
     skin: float = 0.5
+    # Hold/Disp groups (hold, bc_disp, bc_dir, bc_parm) as Simulation.set_holds; spring
+    # pairs and their initial positions come with the system
+    holds: Optional[tuple] = None
     # filled by the server
     slot: Optional[int] = None
     admitted: Optional[float] = None
@@ -72,7 +75,8 @@ class Server:
         """Admit queued jobs in arrival order while their footprint fits."""
         while self.queue:
             job = self.queue[0]
-            slot = self.eng.admit(job.system, job.run_params(), raise_on_full=False)
+            slot = self.eng.admit(job.system, job.run_params(), raise_on_full=False,
+                                  holds=job.holds)
             if slot is None:
                 if not self.active:
                     raise RuntimeError(f"job {job.jid} ({job.system.np} atoms) exceeds the HBM budget")
