@@ -16,7 +16,10 @@
  * Return codes map the reference's STOPs: NDG_OK, NDG_ERR_NEIMAX
  * ('too many neighbours in list', src/nd.f90:1378,1421), NDG_ERR_OUTSIDE
  * ('Particle outside of box', src/nd.f90:1494), NDG_ERR_HIP (runtime error), NDG_ERR_ARG,
- * NDG_ERR_NOMEM (HBM budget exhausted).  ndg_last_error() gives the message.
+ * NDG_ERR_NOMEM (HBM budget exhausted), NDG_ERR_RANGE (mixed path: an atom moved farther
+ * from its list-build position than the 16-byte packed coordinate holds, which only a Run
+ * without list rebuilds allows; no counterpart in the reference, whose FP64 path has no such
+ * limit).  ndg_last_error() gives the message.
  * One host thread per handle; calls on one handle are not re-entrant.
  */
 #ifndef NDYN_GPU_H
@@ -34,7 +37,8 @@ enum {
   NDG_ERR_OUTSIDE = 2,
   NDG_ERR_HIP = 3,
   NDG_ERR_ARG = 4,
-  NDG_ERR_NOMEM = 5
+  NDG_ERR_NOMEM = 5,
+  NDG_ERR_RANGE = 6
 };
 
 /* precision / path selection (ndg_create flags, ndg_engine_create precision) */

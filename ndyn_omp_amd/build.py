@@ -20,7 +20,7 @@ HIP_FLAGS = [
     "-ffp-contract=off",
     # no packed-FP32 instructions anywhere in the device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wall", "-Wno-unused-value", "-Wno-unused-result",
+    "-Wall", "-Werror=unused-value", "-Werror=unused-result",
     # the frame writer's host threads
     "-pthread",
 ]

@@ -55,10 +55,22 @@ static int fail(int code, const std::string &msg) {
       return fail(NDG_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));      \
   } while (0)
 
+// For sequences that must run to the end (teardown, giving back a cache): the first
+// failing call sets rc (and the message); the later calls still run.
+#define HIPKEEP(rc, x)                                                               \
+  do {                                                                               \
+    hipError_t e_ = (x);                                                             \
+    if (e_ != hipSuccess && (rc) == NDG_OK)                                          \
+      (rc) = fail(NDG_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
 static const char *err_text(int code) {
   switch (code) {
     case NDG_ERR_NEIMAX: return "too many neighbours in list";
     case NDG_ERR_OUTSIDE: return "Particle outside of box";
+    case NDG_ERR_RANGE:
+      return "an atom moved beyond the packed coordinate range of the mixed path since the "
+             "last list build (rebuild the lists, use the skin trigger, or the FP64 path)";
     default: return "engine error";
   }
 }
@@ -215,6 +227,8 @@ struct TableBuf {       // device copy of one replica's pair tables (shared when
   std::vector<std::vector<float>> F32;  // per pair, host copy for the LDS table set
   std::vector<std::vector<float>> P32;  // per pair, FP32 potential table (same layout)
   std::vector<uint64_t> phash;          // per pair content hash
+  int npairs = 0, ptsize = 0;
+  std::vector<double> P, F;             // the FP64 tables, to confirm a hash match
 };
 
 struct HostState {      // caller-provided data for one replica (caller order)
@@ -271,6 +285,7 @@ struct Slot {
 
 struct ndg_engine {
   int path = NDG_PATH_MIXED;
+  int device = 0;             // the HIP device the engine's stream and memory live on
   size_t budget = 0, used = 0;
   hipStream_t stream = nullptr;
   std::vector<Slot> slots;
@@ -512,11 +527,13 @@ static int dev_alloc(ndg_engine *e, size_t bytes, void **p) {
   }
   if (e->used + bytes > e->budget) {
     // give cached blocks back before refusing
+    int rc = NDG_OK;
     for (auto &fb : e->free_blocks) {
-      hipFree(fb.second);
+      HIPKEEP(rc, hipFree(fb.second));
       e->used -= fb.first;
     }
     e->free_blocks.clear();
+    if (rc) return rc;
     if (e->used + bytes > e->budget) return fail(NDG_ERR_NOMEM, "HBM budget exhausted");
   }
   hipError_t r = hipMalloc(p, bytes);
@@ -527,6 +544,14 @@ static int dev_alloc(ndg_engine *e, size_t bytes, void **p) {
 
 static void dev_release(ndg_engine *e, void *p, size_t bytes) {
   if (p) e->free_blocks.emplace(bytes, p);
+}
+
+// Captured step graphs hold the old device tables and tiles: drop them when those move.
+static int drop_graphs(ndg_engine *e) {
+  int rc = NDG_OK;
+  for (auto &g : e->graphs) HIPKEEP(rc, hipGraphExecDestroy(g.second));
+  e->graphs.clear();
+  return rc;
 }
 
 static int ensure_capacity(ndg_engine *e) {
@@ -543,14 +568,14 @@ static int ensure_capacity(ndg_engine *e) {
       HIPCHK(hipMemcpyAsync(nr, e->d_reps, sizeof(RepParams) * e->cap_slots, hipMemcpyDeviceToDevice, e->stream));
       HIPCHK(hipMemcpyAsync(nst, e->d_st, sizeof(RepState) * e->cap_slots, hipMemcpyDeviceToDevice, e->stream));
       HIPCHK(hipStreamSynchronize(e->stream));
-      hipFree(e->d_reps);
-      hipFree(e->d_st);
+      HIPCHK(hipFree(e->d_reps));
+      HIPCHK(hipFree(e->d_st));
     }
     e->d_reps = nr;
     e->d_st = nst;
     e->cap_slots = cap;
-    for (auto &g : e->graphs) hipGraphExecDestroy(g.second);
-    e->graphs.clear();
+    int rc = drop_graphs(e);
+    if (rc) return rc;
   }
   return NDG_OK;
 }
@@ -561,20 +586,31 @@ static int ensure_capacity(ndg_engine *e) {
 // order: [F ... | P ...].  The force kernel stages the F half in LDS; its energy/virial
 // variant stages both halves when they fit (tabset_p).
 static int rebuild_tabset(ndg_engine *e) {
-  std::map<uint64_t, int> off;
+  // offset of each distinct (F, P) pair table in the set: keyed by content hash, and a hash
+  // hit is taken only when the contents are equal
+  struct Entry {
+    int off;
+    const std::vector<float> *F, *P;
+  };
+  std::multimap<uint64_t, Entry> seen;
   std::vector<float> all, allp;
+  auto offset_of = [&](const TableBuf &tb, int p) -> int {
+    auto range = seen.equal_range(tb.phash[p]);
+    for (auto q = range.first; q != range.second; ++q)
+      if (*q->second.F == tb.F32[p] && *q->second.P == tb.P32[p]) return q->second.off;
+    const int o = (int)all.size();
+    seen.emplace(tb.phash[p], Entry{o, &tb.F32[p], &tb.P32[p]});
+    all.insert(all.end(), tb.F32[p].begin(), tb.F32[p].end());
+    allp.insert(allp.end(), tb.P32[p].begin(), tb.P32[p].end());
+    return o;
+  };
+  std::map<std::pair<uint64_t, int>, int> off;   // (table key, pair) -> offset
   for (auto &s : e->slots) {
     if (!s.used || !s.table_key) continue;
     auto it = e->tables.find(s.table_key);
     if (it == e->tables.end()) continue;
-    for (int p = 0; p < s.npairs; ++p) {
-      const uint64_t h = it->second.phash[p];
-      if (!off.count(h)) {
-        off[h] = (int)all.size();
-        all.insert(all.end(), it->second.F32[p].begin(), it->second.F32[p].end());
-        allp.insert(allp.end(), it->second.P32[p].begin(), it->second.P32[p].end());
-      }
-    }
+    for (int p = 0; p < s.npairs; ++p)
+      if (!off.count({s.table_key, p})) off[{s.table_key, p}] = offset_of(it->second, p);
   }
   const bool fits = !all.empty() && all.size() * sizeof(float) <= (size_t)kLdsTableBytes;
   const bool fits_p = fits && 2 * all.size() * sizeof(float) <= (size_t)kLdsTableBytesFull;
@@ -586,10 +622,8 @@ static int rebuild_tabset(ndg_engine *e) {
     Slot &s = e->slots[r];
     int lt[16];
     for (int p = 0; p < 16; ++p) lt[p] = -1;
-    if (s.used && s.table_key && fits) {
-      auto it = e->tables.find(s.table_key);
-      for (int p = 0; p < s.npairs; ++p) lt[p] = off[it->second.phash[p]];
-    }
+    if (s.used && s.table_key && fits)
+      for (int p = 0; p < s.npairs; ++p) lt[p] = off[{s.table_key, p}];
     if (std::memcmp(lt, s.P.ltab, sizeof(lt)) != 0) {
       std::memcpy(s.P.ltab, lt, sizeof(lt));
       changed.push_back((int)r);
@@ -602,7 +636,9 @@ static int rebuild_tabset(ndg_engine *e) {
     if (all.size() > e->tabset_cap) {
       if (e->d_tabset) {
         HIPCHK(hipStreamSynchronize(e->stream));
-        hipFree(e->d_tabset);
+        HIPCHK(hipFree(e->d_tabset));
+        e->d_tabset = nullptr;
+        e->tabset_cap = 0;
       }
       HIPCHK(hipMalloc(&e->d_tabset, sizeof(float) * all.size()));
       e->tabset_cap = all.size();
@@ -645,8 +681,10 @@ static int rebuild_tiles(ndg_engine *e) {
   if (nt > e->cap_tiles) {
     if (e->d_tiles) {
       HIPCHK(hipStreamSynchronize(e->stream));
-      hipFree(e->d_tiles);
-      hipFree(e->d_rb);
+      HIPCHK(hipFree(e->d_tiles));
+      HIPCHK(hipFree(e->d_rb));
+      e->d_tiles = nullptr;
+      e->d_rb = nullptr;
     }
     e->cap_tiles = std::max(nt, 2 * e->cap_tiles);
     HIPCHK(hipMalloc(&e->d_tiles, sizeof(Tile) * e->cap_tiles));
@@ -656,7 +694,8 @@ static int rebuild_tiles(ndg_engine *e) {
   if (ns > e->cap_tile0) {
     if (e->d_tile0) {
       HIPCHK(hipStreamSynchronize(e->stream));
-      hipFree(e->d_tile0);
+      HIPCHK(hipFree(e->d_tile0));
+      e->d_tile0 = nullptr;
     }
     e->cap_tile0 = std::max(ns, 2 * e->cap_tile0);
     HIPCHK(hipMalloc(&e->d_tile0, sizeof(int) * e->cap_tile0));
@@ -674,9 +713,7 @@ static int rebuild_tiles(ndg_engine *e) {
     HIPCHK(hipMemcpyAsync(e->d_tiles, e->h_tiles.data(), sizeof(Tile) * nt, hipMemcpyHostToDevice, e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
   e->tiles_dirty = false;
-  for (auto &g : e->graphs) hipGraphExecDestroy(g.second);
-  e->graphs.clear();
-  return NDG_OK;
+  return drop_graphs(e);
 }
 
 static EngineDev dev_view(ndg_engine *e) {
@@ -723,9 +760,22 @@ static int attach_tables(ndg_engine *e, Slot &s, const HostState &h) {
   key = hash_words(h.F.data(), n * 8, key);
   key = fnv1a(&h.npairs, sizeof(int), key);
   key = fnv1a(&h.ptsize, sizeof(int), key);
+  if (key == 0) key = 1;   // 0 marks a slot without tables
   auto it = e->tables.find(key);
+  // a key is taken as a match only with equal contents; a different table under the same
+  // key (a hash collision) moves on to the next key
+  while (it != e->tables.end() &&
+         !(it->second.npairs == h.npairs && it->second.ptsize == h.ptsize &&
+           it->second.P == h.P && it->second.F == h.F)) {
+    key = key + 1 ? key + 1 : 1;
+    it = e->tables.find(key);
+  }
   if (it == e->tables.end()) {
     TableBuf tb;
+    tb.npairs = h.npairs;
+    tb.ptsize = h.ptsize;
+    tb.P = h.P;
+    tb.F = h.F;
     tb.bytes = table_bytes(h.npairs, h.ptsize);
     int rc = dev_alloc(e, tb.bytes, &tb.block);
     if (rc) return rc;
@@ -1158,44 +1208,53 @@ static void advance_mirror(ndg_engine *e) {
   }
 }
 
-static void time_begin(ndg_engine *e, hipEvent_t *a) {
-  if (!e->timing) return;
-  if (e->ev_pool.size() < 2) {
-    hipEvent_t x, y;
-    hipEventCreate(&x);
-    hipEventCreate(&y);
-    e->ev_pool.push_back(x);
-    e->ev_pool.push_back(y);
+static int time_event(ndg_engine *e, hipEvent_t *ev) {
+  if (e->ev_pool.empty()) {
+    HIPCHK(hipEventCreate(ev));
+  } else {
+    *ev = e->ev_pool.back();
+    e->ev_pool.pop_back();
   }
-  *a = e->ev_pool.back();
-  e->ev_pool.pop_back();
-  hipEventRecord(*a, e->stream);
+  return NDG_OK;
 }
 
-static void time_end(ndg_engine *e, hipEvent_t a) {
-  if (!e->timing) return;
-  hipEvent_t b = e->ev_pool.empty() ? nullptr : e->ev_pool.back();
-  if (!b) hipEventCreate(&b); else e->ev_pool.pop_back();
-  hipEventRecord(b, e->stream);
+static int time_begin(ndg_engine *e, hipEvent_t *a) {
+  *a = nullptr;
+  if (!e->timing) return NDG_OK;
+  int rc = time_event(e, a);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(*a, e->stream));
+  return NDG_OK;
+}
+
+static int time_end(ndg_engine *e, hipEvent_t a) {
+  if (!e->timing || !a) return NDG_OK;
+  hipEvent_t b;
+  int rc = time_event(e, &b);
+  if (rc) return rc;
   e->ev_pending.push_back({a, b});
+  HIPCHK(hipEventRecord(b, e->stream));
+  return NDG_OK;
 }
 
-static void time_collect(ndg_engine *e) {
+static int time_collect(ndg_engine *e) {
+  int rc = NDG_OK;
   for (auto &p : e->ev_pending) {
     float ms = 0.f;
-    hipEventSynchronize(p.second);
-    hipEventElapsedTime(&ms, p.first, p.second);
+    HIPKEEP(rc, hipEventSynchronize(p.second));
+    HIPKEEP(rc, hipEventElapsedTime(&ms, p.first, p.second));
     e->t_ms += ms;
     e->t_count += 1;
     e->ev_pool.push_back(p.first);
     e->ev_pool.push_back(p.second);
   }
   e->ev_pending.clear();
+  return rc;
 }
 
 // Kernels of one step for the batched (mixed) path: every active replica in one launch
 // per kernel.
-static void enqueue_step_batched(ndg_engine *e, const TickPlan &tp) {
+static int enqueue_step_batched(ndg_engine *e, const TickPlan &tp) {
   EngineDev E = dev_view(e);
   const int nt = (int)e->h_tiles.size(), nr = (int)e->slots.size();
   hipStream_t s = e->stream;
@@ -1204,7 +1263,8 @@ static void enqueue_step_batched(ndg_engine *e, const TickPlan &tp) {
   // mixed path: the kick runs inside the force kernel
   const int fused = e->path == NDG_PATH_MIXED;
   hipEvent_t a = nullptr;
-  time_begin(e, &a);
+  int rc = time_begin(e, &a);
+  if (rc) return rc;
   for (size_t q = 0; q < e->tpa_ranges.size(); ++q) {
     launch_force(E, e->path, e->tpa_ranges[q].first, false, e->tpa_ranges[q].second,
                  e->tpa_counts[q], 0, s, fused);
@@ -1212,30 +1272,32 @@ static void enqueue_step_batched(ndg_engine *e, const TickPlan &tp) {
       launch_force(E, e->path, e->tpa_ranges[q].first, true, e->tpa_ranges[q].second,
                    e->tpa_counts[q], 0, s, fused);
   }
-  time_end(e, a);
+  if ((rc = time_end(e, a))) return rc;
   if (!fused) launch_kick(E, e->path, 0, nt, 0, s);
   launch_reduce(E, 0, nr, RED_FORCE | RED_KICK | RED_STEP, 0, 0, s);
+  return NDG_OK;
 }
 
 // Plain path: each replica is advanced with its own launches.
-static void enqueue_step_single(ndg_engine *e, int r, const TickPlan &tp) {
+static int enqueue_step_single(ndg_engine *e, int r) {
   EngineDev E = dev_view(e);
   Slot &sl = e->slots[r];
   hipStream_t s = e->stream;
   RepState S{};
   S.step = sl.step;
   const int f = step_flags(sl.P, S);
-  if (!(f & F_ACTIVE)) return;
+  if (!(f & F_ACTIVE)) return NDG_OK;
   if (f & F_CHECK) launch_rebuild(E, sl.ntiles, r, 1, s);
   launch_drift(E, e->path, sl.tile_base, sl.ntiles, 0, s);
   hipEvent_t a = nullptr;
-  time_begin(e, &a);
+  int rc = time_begin(e, &a);
+  if (rc) return rc;
   launch_force(E, e->path, sl.tpa, false, sl.tile_base, sl.ntiles, 0, s);
   if (f & (F_CU | F_CV)) launch_force(E, e->path, sl.tpa, true, sl.tile_base, sl.ntiles, 0, s);
-  time_end(e, a);
+  if ((rc = time_end(e, a))) return rc;
   launch_kick(E, e->path, sl.tile_base, sl.ntiles, 0, s);
   launch_reduce(E, r, 1, RED_FORCE | RED_KICK | RED_STEP, 0, 0, s);
-  (void)tp;
+  return NDG_OK;
 }
 
 static std::string tick_signature(const std::vector<TickPlan> &plans) {
@@ -1294,8 +1356,9 @@ static int drain(ndg_engine *e) {
     s.step = S[r].step;
     s.finished = S[r].done != 0 || s.step - s.run_step0 >= s.run_nsteps;
   }
-  time_collect(e);
+  int rc = time_collect(e);
   if (first_err) return fail(first_err, err_text(first_err));
+  if (rc) return rc;
   return NDG_OK;
 }
 
@@ -1327,18 +1390,25 @@ static int engine_steps(ndg_engine *e, int nsteps, bool drain_last = true) {
           if (it == e->graphs.end()) {
             hipGraph_t g;
             HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-            for (auto &p : sub) enqueue_step_batched(e, p);
+            int rq = NDG_OK;
+            for (auto &p : sub)
+              if (!rq) rq = enqueue_step_batched(e, p);
             HIPCHK(hipStreamEndCapture(e->stream, &g));
+            if (rq) {
+              HIPKEEP(rq, hipGraphDestroy(g));
+              return rq;
+            }
             hipGraphExec_t ge;
             HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-            hipGraphDestroy(g);
+            HIPCHK(hipGraphDestroy(g));
             it = e->graphs.emplace(sig, ge).first;
           }
           HIPCHK(hipGraphLaunch(it->second, e->stream));
           k += len;
         }
       } else {
-        for (auto &p : plans) enqueue_step_batched(e, p);
+        for (auto &p : plans)
+          if ((rc = enqueue_step_batched(e, p))) return rc;
       }
       HIPCHK(hipGetLastError());
     }
@@ -1363,8 +1433,7 @@ static int engine_steps_plain(ndg_engine *e, int nsteps, bool drain_last = true)
       for (size_t r = 0; r < e->slots.size(); ++r) {
         Slot &s = e->slots[r];
         if (!s.used || s.finished || s.err) continue;
-        TickPlan tp;
-        enqueue_step_single(e, (int)r, tp);
+        if ((rc = enqueue_step_single(e, (int)r))) return rc;
         any = true;
       }
       if (!any) break;
@@ -1454,6 +1523,7 @@ static int engine_new(ndg_engine **pe, int path, size_t budget) {
   int ndev = 0;
   HIPCHK(hipGetDeviceCount(&ndev));
   if (ndev < 1) return fail(NDG_ERR_HIP, "no HIP device");
+  HIPCHK(hipGetDevice(&e->device));
   HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
   size_t fr = 0, tot = 0;
   HIPCHK(hipMemGetInfo(&fr, &tot));
@@ -1468,32 +1538,32 @@ extern "C" int ndg_engine_create(ndg_engine_t **e, int path, size_t hbm_budget) 
   return engine_new(e, path, hbm_budget);
 }
 
+// Frees everything the engine holds; every release is attempted, and the first failure
+// is returned (with its message) after the rest.
 extern "C" int ndg_engine_destroy(ndg_engine_t *e) {
   if (!e) return NDG_OK;
-  hipStreamSynchronize(e->stream);
-  for (auto &g : e->graphs) hipGraphExecDestroy(g.second);
+  int rc = NDG_OK;
+  HIPKEEP(rc, hipStreamSynchronize(e->stream));
+  int rg = drop_graphs(e);
+  if (rg && rc == NDG_OK) rc = rg;
   for (auto &s : e->slots)
-    if (s.block) hipFree(s.block);
-  for (auto &t : e->tables) hipFree(t.second.block);
-  for (auto &fb : e->free_blocks) hipFree(fb.second);
+    if (s.block) HIPKEEP(rc, hipFree(s.block));
+  for (auto &t : e->tables) HIPKEEP(rc, hipFree(t.second.block));
+  for (auto &fb : e->free_blocks) HIPKEEP(rc, hipFree(fb.second));
   for (auto &p : e->ev_pending) {
-    hipEventDestroy(p.first);
-    hipEventDestroy(p.second);
+    HIPKEEP(rc, hipEventDestroy(p.first));
+    HIPKEEP(rc, hipEventDestroy(p.second));
   }
-  for (auto &ev : e->ev_pool) hipEventDestroy(ev);
-  if (e->d_reps) hipFree(e->d_reps);
-  if (e->d_st) hipFree(e->d_st);
-  if (e->d_tiles) hipFree(e->d_tiles);
-  if (e->d_rb) hipFree(e->d_rb);
-  if (e->d_tile0) hipFree(e->d_tile0);
-  if (e->d_tabset) hipFree(e->d_tabset);
-  if (e->snap_d) hipFree(e->snap_d);
-  if (e->snap_h) hipHostFree(e->snap_h);
-  if (e->snap_ev) hipEventDestroy(e->snap_ev);
-  if (e->stage_h) hipHostFree(e->stage_h);
-  hipStreamDestroy(e->stream);
+  for (auto &ev : e->ev_pool) HIPKEEP(rc, hipEventDestroy(ev));
+  void *dev[] = {e->d_reps, e->d_st, e->d_tiles, e->d_rb, e->d_tile0, e->d_tabset, e->snap_d};
+  for (void *p : dev)
+    if (p) HIPKEEP(rc, hipFree(p));
+  if (e->snap_h) HIPKEEP(rc, hipHostFree(e->snap_h));
+  if (e->snap_ev) HIPKEEP(rc, hipEventDestroy(e->snap_ev));
+  if (e->stage_h) HIPKEEP(rc, hipHostFree(e->stage_h));
+  HIPKEEP(rc, hipStreamDestroy(e->stream));
   delete e;
-  return NDG_OK;
+  return rc;
 }
 
 static int ncell_capacity(const double box[3], double rcut, double skin) {
@@ -1891,7 +1961,8 @@ extern "C" int ndg_engine_kernel_time(ndg_engine_t *e, int reset, double *avg_ms
                                       long long *count, double *pairs, double *atoms) {
   if (!e) return fail(NDG_ERR_ARG, "null engine");
   HIPCHK(hipStreamSynchronize(e->stream));
-  time_collect(e);
+  int rc = time_collect(e);
+  if (rc) return rc;
   if (avg_ms) *avg_ms = e->t_count ? e->t_ms / e->t_count : 0.0;
   if (count) *count = e->t_count;
   long long atoms_total = 0;
@@ -1940,7 +2011,7 @@ extern "C" int ndg_engine_list_stats(ndg_engine_t *e, double *mean_nbr, long lon
 // ------------------------------------------------------------------ single-system API
 
 static int settle(ndg_t *h);
-static void frame_writer_stop(ndg_t *h);
+static int frame_writer_stop(ndg_t *h);
 static int upload_host(ndg_engine *e, Slot &s, const HostState &h, bool state_too);
 static int upload_hbc_sorted(ndg_engine *e, Slot &s, const HostState &h);
 
@@ -1972,10 +2043,10 @@ extern "C" int ndg_create(ndg_t **h, int np, int ntypes, int npairs, int ptsize,
 
 extern "C" int ndg_destroy(ndg_t *h) {
   if (!h) return NDG_OK;
-  frame_writer_stop(h);   // writes the frames still queued
-  ndg_engine_destroy(h->e);
+  int rc = frame_writer_stop(h);   // writes the frames still queued
+  int rd = ndg_engine_destroy(h->e);
   delete h;
-  return NDG_OK;
+  return rc ? rc : rd;
 }
 
 extern "C" int ndg_set_tables(ndg_t *h, const double *P, const double *F, const double *rc2,
@@ -2077,7 +2148,14 @@ extern "C" int ndg_init_positions(ndg_t *h) {
 extern "C" int ndg_set_state(ndg_t *h, const double *pos, const double *vel, const double *acc,
                              const int *typ, const double box[3]) {
   if (!h || !pos || !typ || !box) return fail(NDG_ERR_ARG, "ndg_set_state: null argument");
+  int rc = settle(h);
+  if (rc) return rc;
   HostState &hs = h->hs;
+  // validated before anything is replaced: the kernels index per-type data by these types
+  for (int i = 0; i < hs.np; ++i)
+    if (typ[i] < 1 || typ[i] > hs.ntypes) return fail(NDG_ERR_ARG, "ndg_set_state: atom type out of range");
+  for (int a = 0; a < 3; ++a)
+    if (!(box[a] > 0.0)) return fail(NDG_ERR_ARG, "ndg_set_state: box must be positive");
   const size_t n3 = 3 * (size_t)hs.np;
   hs.pos.assign(pos, pos + n3);
   if (vel) hs.vel.assign(vel, vel + n3); else hs.vel.assign(n3, 0.0);
@@ -2088,8 +2166,7 @@ extern "C" int ndg_set_state(ndg_t *h, const double *pos, const double *vel, con
   Slot &s = h->e->slots[h->slot];
   if (s.block) {
     // state replaced while resident: upload and invalidate the lists
-    int rc = upload_host(h->e, s, hs, true);
-    if (rc) return rc;
+    if ((rc = upload_host(h->e, s, hs, true))) return rc;
     RepState S;
     if ((rc = get_state_dev(h->e, h->slot, &S))) return rc;
     for (int a = 0; a < 3; ++a) S.box[a] = S.box_build[a] = box[a];
@@ -2225,13 +2302,14 @@ extern "C" int ndg_compute(ndg_t *h, int cv, int cu, double *pe, double virial[6
   const int xf = drop_in_flags(cv ? F_CV : 0, cu ? F_CU : 0, 0);
   EngineDev E = dev_view(e);
   hipEvent_t a = nullptr;
-  time_begin(e, &a);
+  int rc = time_begin(e, &a);
+  if (rc) return rc;
   launch_force(E, e->path, s.tpa, (cv || cu), s.tile_base, s.ntiles, xf, e->stream);
-  time_end(e, a);
+  if ((rc = time_end(e, a))) return rc;
   launch_reduce(E, h->slot, 1, RED_FORCE, xf, 0, e->stream);
   HIPCHK(hipGetLastError());
   RepState S;
-  int rc = get_state_dev(e, h->slot, &S);
+  rc = get_state_dev(e, h->slot, &S);
   if (rc) return rc;
   // the next integrate2 reads the new force: parity of "latest" flips after the kick
   if (pe && cu) *pe = S.pe;
@@ -2421,8 +2499,8 @@ static int frame_writer_error(FrameWriter &w) {
   return fail(rc, msg.c_str());
 }
 
-static void frame_writer_stop(ndg_t *h) {
-  if (!h->fw) return;
+static int frame_writer_stop(ndg_t *h) {
+  if (!h->fw) return NDG_OK;
   FrameWriter &w = *h->fw;
   {
     std::lock_guard<std::mutex> lk(w.m);
@@ -2430,11 +2508,13 @@ static void frame_writer_stop(ndg_t *h) {
   }
   w.cv.notify_all();
   if (w.th.joinable()) w.th.join();
+  int rc = NDG_OK;
   for (auto &b : w.bufs) {
-    if (b.h) hipHostFree(b.h);
-    if (b.ev) hipEventDestroy(b.ev);
+    if (b.h) HIPKEEP(rc, hipHostFree(b.h));
+    if (b.ev) HIPKEEP(rc, hipEventDestroy(b.ev));
   }
   h->fw.reset();
+  return rc;
 }
 
 extern "C" int ndg_frame_write(ndg_t *h, const char *path, int format, const char *names,
@@ -2542,10 +2622,11 @@ extern "C" int ndg_copy_bandwidth(size_t bytes, int iters, double *gbps) {
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, e0, e1));
   *gbps = 2.0 * (double)(n * sizeof(float4)) * iters / (ms * 1e-3) / 1e9;  // read + write
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  hipFree(a);
-  hipFree(b);
-  hipStreamDestroy(st);
-  return NDG_OK;
+  int rc = NDG_OK;
+  HIPKEEP(rc, hipEventDestroy(e0));
+  HIPKEEP(rc, hipEventDestroy(e1));
+  HIPKEEP(rc, hipFree(a));
+  HIPKEEP(rc, hipFree(b));
+  HIPKEEP(rc, hipStreamDestroy(st));
+  return rc;
 }

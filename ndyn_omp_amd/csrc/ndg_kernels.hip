@@ -141,8 +141,18 @@ __device__ inline double xq_unit(const RepState &S, double skin) {
   return ldexp(1.0, e - 30);
 }
 
+constexpr double kXqLimit = 1073741823.0;   // 2^30 - 1
+
+// An anchored coordinate the packed record cannot hold (|x| / q beyond 2^30 - 1): only an
+// atom that moved more than skin + 1 A from its list-build position without a rebuild.
+__device__ inline bool xq_out_of_range(double x, double y, double z, double q) {
+  return fmax(fabs(x), fmax(fabs(y), fabs(z))) > kXqLimit * q;
+}
+
+// Values beyond the range are clamped (so the integer differences never overflow); the
+// drift kernel flags NDG_ERR_RANGE for them.
 __device__ inline uint4 xq_pack(double x, double y, double z, int ty, double q) {
-  const double lim = 1073741823.0;   // 2^30 - 1
+  const double lim = kXqLimit;
   const double iq = 1.0 / q;         // a power of two: exact
   const int nx = (int)fmin(lim, fmax(-lim, rint(x * iq)));
   const int ny = (int)fmin(lim, fmax(-lim, rint(y * iq)));
@@ -221,7 +231,10 @@ __global__ __launch_bounds__(kTile) void k_drift_mixed(EngineDev E, int t0, int 
       const double zl = anchored(pz, A.z, m2, S.box[2] / m2);
       // x64a (FP64 anchored) is written only at a rebuild, for the list builder; the
       // force kernel reads the packed copy
-      R.xq[i] = xq_pack(xl, yl, zl, ty, xq_unit(S, R.skin));
+      const double qu = xq_unit(S, R.skin);
+      R.xq[i] = xq_pack(xl, yl, zl, ty, qu);
+      // the run stops with NDG_ERR_RANGE rather than compute pairs from clamped records
+      if (xq_out_of_range(xl, yl, zl, qu)) E.st[T.rep].err = NDG_ERR_RANGE;
       const float4 B = R.x32b[i];
       const double dx = (double)((float)xl - B.x), dy = (double)((float)yl - B.y),
                    dz = (double)((float)zl - B.z);

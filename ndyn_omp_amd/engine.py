@@ -19,7 +19,7 @@ from . import workloads as W
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libndyn_gpu.so")
 
-OK, ERR_NEIMAX, ERR_OUTSIDE, ERR_HIP, ERR_ARG, ERR_NOMEM = 0, 1, 2, 3, 4, 5
+OK, ERR_NEIMAX, ERR_OUTSIDE, ERR_HIP, ERR_ARG, ERR_NOMEM, ERR_RANGE = 0, 1, 2, 3, 4, 5, 6
 PATH_FP64, PATH_MIXED = 0, 1
 TRIGGER_COMPAT, TRIGGER_SKIN = 0, 1
 
@@ -142,6 +142,21 @@ def _ptr(a: Optional[np.ndarray]):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
+def _hold_arrays(np_atoms, hold, bc_disp, bc_dir, bc_parm):
+    """The Hold/Disp arrays in the C layout, checked against the sizes the library reads
+    (hold[np], bc_disp[nhold], bc_dir and bc_parm (nhold, 3))."""
+    hk = [np.ascontiguousarray(hold, dtype=np.int32).ravel(),
+          np.ascontiguousarray(bc_disp, dtype=np.int32).ravel(),
+          np.ascontiguousarray(bc_dir, dtype=np.int32).reshape(-1),
+          np.ascontiguousarray(bc_parm, dtype=np.float64).reshape(-1)]
+    nhold = hk[1].shape[0]
+    if hk[0].shape[0] != np_atoms:
+        raise ValueError(f"hold has {hk[0].shape[0]} entries for {np_atoms} atoms")
+    if hk[2].shape[0] != 3 * nhold or hk[3].shape[0] != 3 * nhold:
+        raise ValueError(f"bc_dir and bc_parm must be ({nhold}, 3)")
+    return hk
+
+
 def pair_tables(kinds, params, ptrmin, ptsize, qprd=None):
     """Tables of every pair with the engine's builder: P, F (npairs, ptsize+1), rc2, dr2."""
     L = load_library()
@@ -227,10 +242,7 @@ class Simulation:
     def set_holds(self, hold, bc_disp, bc_dir, bc_parm):
         """Hold/Disp boundary conditions (ndg_set_holds): hold group per atom (0: none) and,
         per group, bcTyp 'd' flag, constrained axes and scaled displacement per step."""
-        hk = [np.ascontiguousarray(hold, dtype=np.int32),
-              np.ascontiguousarray(bc_disp, dtype=np.int32),
-              np.ascontiguousarray(bc_dir, dtype=np.int32).ravel(),
-              np.ascontiguousarray(bc_parm, dtype=np.float64).ravel()]
+        hk = _hold_arrays(self.sys.np, hold, bc_disp, bc_dir, bc_parm)
         self._holds = hk
         _check(load_library().ndg_set_holds(self.h, _ptr(hk[0]), len(hk[1]), _ptr(hk[1]),
                                             _ptr(hk[2]), _ptr(hk[3])))
@@ -423,10 +435,7 @@ class BatchEngine:
                     ibox = np.ascontiguousarray(s.init_box, dtype=np.float64)
             hk = [None] * 4
             if holds is not None:
-                hk = [np.ascontiguousarray(holds[0], dtype=np.int32),
-                      np.ascontiguousarray(holds[1], dtype=np.int32),
-                      np.ascontiguousarray(holds[2], dtype=np.int32).ravel(),
-                      np.ascontiguousarray(holds[3], dtype=np.float64).ravel()]
+                hk = _hold_arrays(s.np, *holds)
             nhold = 0 if holds is None else len(hk[1])
             rc = load_library().ndg_engine_admit_ex(
                 *args, _ptr(kinds), _ptr(ipos), _ptr(ibox), _ptr(hk[0]), nhold, _ptr(hk[1]),

@@ -9,7 +9,8 @@ module ndg_iface
   implicit none
 
   integer(c_int), parameter :: NDG_OK = 0, NDG_ERR_NEIMAX = 1, NDG_ERR_OUTSIDE = 2, &
-                               NDG_ERR_HIP = 3, NDG_ERR_ARG = 4, NDG_ERR_NOMEM = 5
+                               NDG_ERR_HIP = 3, NDG_ERR_ARG = 4, NDG_ERR_NOMEM = 5, &
+                               NDG_ERR_RANGE = 6
   integer(c_int), parameter :: NDG_PATH_FP64 = 0, NDG_PATH_MIXED = 1
   integer(c_int), parameter :: NDG_TRIGGER_COMPAT = 0, NDG_TRIGGER_SKIN = 1
   integer(c_int), parameter :: NDG_MORSE = 1, NDG_BUCK = 2, NDG_LJ = 3, NDG_MORSE_PHYS = 4, &

@@ -73,3 +73,17 @@ def test_fortran_module_binds_every_entry_point():
     f90 = open(os.path.join(ROOT, "ndyn_omp_amd", "fortran", "ndg_iface.f90")).read()
     bound = set(re.findall(r"name='(ndg_\w+)'", f90))
     assert decl and decl - bound == set()
+
+
+def test_hold_arrays_are_checked_before_the_call():
+    """The Hold/Disp arrays the library reads (hold[np], bc_disp[nhold], bc_dir and
+    bc_parm (nhold,3)) are shape-checked on the host before any call."""
+    from ndyn_omp_amd import engine as G
+    ok = G._hold_arrays(4, [0, 1, 1, 0], [1], [[1, 0, 0]], [[1e-4, 0, 0]])
+    assert [a.shape for a in ok] == [(4,), (1,), (3,), (3,)]
+    with pytest.raises(ValueError):
+        G._hold_arrays(5, [0, 1, 1, 0], [1], [[1, 0, 0]], [[1e-4, 0, 0]])
+    with pytest.raises(ValueError):
+        G._hold_arrays(4, [0, 1, 1, 0], [1, 0], [[1, 0, 0]], [[1e-4, 0, 0]])
+    with pytest.raises(ValueError):
+        G._hold_arrays(4, [0, 1, 1, 0], [1], [[1, 0]], [[1e-4, 0, 0]])
