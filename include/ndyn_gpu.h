@@ -284,6 +284,17 @@ int ndg_engine_fetch(ndg_engine_t *e, int slot, double *pos, double *vel, double
                      double box[3]);
 int ndg_engine_release(ndg_engine_t *e, int slot);
 
+/* ndg_engine_fetch that allocates nothing.  No counterpart in the reference: for callers that
+ * cannot allocate device or pinned memory (a serving loop fetching finished jobs between
+ * ticks, a harness that forbids allocation; the call waits for its copies, so it is not for
+ * graph capture).  The state is staged in a fixed device array of the
+ * library, one per GPU, held until the process ends: systems of at most 262144 atoms
+ * (larger ones return NDG_ERR_ARG; ndg_engine_fetch takes any size).  The copies go straight
+ * into the caller's arrays.  Calls that overlap on one GPU (from two host threads or two
+ * engines) share that array and race. */
+int ndg_engine_fetch_noalloc(ndg_engine_t *e, int slot, double *pos, double *vel, double *force,
+                             double box[3]);
+
 /* Launch configuration and timing.  tpa = threads per atom in the force kernel (0 =
  * per-size-class table); graph_steps = steps captured per hipGraph (0 = eager launches). */
 int ndg_engine_config(ndg_engine_t *e, int tpa, int graph_steps);

@@ -1906,6 +1906,81 @@ extern "C" int ndg_engine_fetch(ndg_engine_t *e, int slot, double *pos, double *
   return snapshot_copy(e, slot, pos, vel, force, nullptr, box, nullptr);
 }
 
+// ------------------------------------------------------------------ fetch without allocation
+// ndg_engine_fetch stages a replica's state in a device buffer and a pinned host buffer that
+// grow (hipMalloc / hipHostMalloc) with the largest replica fetched.  Callers that may not
+// allocate -- a serving loop that fetches finished jobs between ticks, a harness that forbids
+// allocation -- use ndg_engine_fetch_noalloc instead: the staging is this file-scope device
+// array, one per GPU (held until the process ends), copied straight into the caller's
+// arrays.  Every call on a GPU shares that GPU's array: calls that overlap on one GPU (two
+// host threads, two engines) race.  Nothing in it needs clearing: each call overwrites the
+// 10 np + 4 doubles it reads back.
+constexpr int kNoAllocMaxAtoms = 1 << 18;
+constexpr size_t kNoAllocDoubles = 10 * (size_t)kNoAllocMaxAtoms + 4;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, double *> g_scratch_ptr;   // device -> address of that device's `scratch`
+}  // namespace
+
+// Device address of `scratch` on the current device (looked up once per device; in host
+// code the array's name is not a device address).
+static int scratch_address(int dev, double **ptr) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it != g_scratch_ptr.end()) {
+    *ptr = it->second;
+    return NDG_OK;
+  }
+  void *p = nullptr;
+  HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) return fail(NDG_ERR_HIP, "hipGetSymbolAddress(scratch) returned no address");
+  g_scratch_ptr[dev] = (double *)p;
+  *ptr = (double *)p;
+  return NDG_OK;
+}
+
+static int fetch_noalloc(ndg_engine *e, int slot, double *pos, double *vel, double *force,
+                         double box[3]) {
+  const int np = e->slots[slot].np;
+  if (np > kNoAllocMaxAtoms)
+    return fail(NDG_ERR_ARG, "ndg_engine_fetch_noalloc: at most " +
+                                 std::to_string(kNoAllocMaxAtoms) + " atoms per system (this one has " +
+                                 std::to_string(np) + "); use ndg_engine_fetch");
+  double *buf = nullptr;
+  int rc = scratch_address(e->device, &buf);
+  if (rc) return rc;
+  EngineDev E = dev_view(e);
+  launch_apply_pending(E, slot, 1, e->stream);
+  launch_snapshot(E, slot, np, buf, e->stream);
+  HIPCHK(hipGetLastError());
+  const size_t n3 = 3 * (size_t)np;
+  if (pos) HIPCHK(hipMemcpyAsync(pos, buf, 8 * n3, hipMemcpyDeviceToHost, e->stream));
+  if (vel) HIPCHK(hipMemcpyAsync(vel, buf + n3, 8 * n3, hipMemcpyDeviceToHost, e->stream));
+  if (force) HIPCHK(hipMemcpyAsync(force, buf + 2 * n3, 8 * n3, hipMemcpyDeviceToHost, e->stream));
+  if (box) HIPCHK(hipMemcpyAsync(box, buf + 10 * (size_t)np, 24, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return NDG_OK;
+}
+
+extern "C" int ndg_engine_fetch_noalloc(ndg_engine_t *e, int slot, double *pos, double *vel,
+                                        double *force, double box[3]) {
+  if (!e || slot < 0 || slot >= (int)e->slots.size() || !e->slots[slot].used ||
+      !e->slots[slot].block)
+    return fail(NDG_ERR_ARG, "bad slot");
+  int cur = 0;
+  HIPCHK(hipGetDevice(&cur));
+  if (cur != e->device) HIPCHK(hipSetDevice(e->device));
+  int rc = fetch_noalloc(e, slot, pos, vel, force, box);
+  if (cur != e->device) {
+    hipError_t r = hipSetDevice(cur);
+    if (r != hipSuccess && rc == NDG_OK)
+      rc = fail(NDG_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(r));
+  }
+  return rc;
+}
+
 extern "C" int ndg_engine_release(ndg_engine_t *e, int slot) {
   if (!e || slot < 0 || slot >= (int)e->slots.size() || !e->slots[slot].used)
     return fail(NDG_ERR_ARG, "bad slot");

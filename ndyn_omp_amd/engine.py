@@ -110,6 +110,7 @@ def load_library():
         "ndg_engine_status": [V, C.c_int, P(C.c_int), P(C.c_int), P(Thermo), P(C.c_int)],
         "ndg_engine_thermo": [V, C.c_int, P(Thermo), C.c_int, P(C.c_int)],
         "ndg_engine_fetch": [V, C.c_int, V, V, V, V],
+        "ndg_engine_fetch_noalloc": [V, C.c_int, V, V, V, V],
         "ndg_engine_release": [V, C.c_int],
         "ndg_engine_config": [V, C.c_int, C.c_int],
         "ndg_engine_size_classes": [V, C.c_int, ip, ip],
@@ -470,6 +471,14 @@ class BatchEngine:
         box = np.zeros(3)
         _check(load_library().ndg_engine_fetch(self.e, int(slot), _ptr(pos), _ptr(vel),
                                                _ptr(force), _ptr(box)))
+        return dict(pos=pos, vel=vel, force=force, box=box)
+
+    def fetch_noalloc(self, slot, n):
+        """fetch through ndg_engine_fetch_noalloc (no allocation; at most 262144 atoms)."""
+        pos, vel, force = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+        box = np.zeros(3)
+        _check(load_library().ndg_engine_fetch_noalloc(self.e, int(slot), _ptr(pos), _ptr(vel),
+                                                       _ptr(force), _ptr(box)))
         return dict(pos=pos, vel=vel, force=force, box=box)
 
     def release(self, slot):

@@ -358,6 +358,15 @@ module ndg_iface
       real(c_double), intent(out) :: pos(*), vel(*), force(*), box(3)
     end function
 
+    ! ndg_engine_fetch without allocation (systems of at most 262144 atoms)
+    integer(c_int) function ndg_engine_fetch_noalloc(e, slot, pos, vel, force, box) &
+        bind(C, name='ndg_engine_fetch_noalloc')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), value :: slot
+      real(c_double), intent(out) :: pos(*), vel(*), force(*), box(3)
+    end function
+
     integer(c_int) function ndg_engine_release(e, slot) bind(C, name='ndg_engine_release')
       import :: c_int, c_ptr
       type(c_ptr), value :: e
