@@ -71,36 +71,46 @@ def workload_systems(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def serve_bench(args):
-    """Serving workload: a bounded sample of the job stream (sizes and step counts scaled
-    down so the run takes about a minute), Poisson arrivals at a rate set for ~80% of the
-    given capacity, continuous batching with state resident on the device."""
+def serving_run(n_jobs, capacity, size_scale=1.0, step_scale=1.0, tick_steps=50,
+                graph_steps=0, load=0.8):
+    """The first n_jobs of the serving stream (W.serving_jobs, seed 12349), Poisson arrivals
+    at a rate that offers `load` x `capacity` atom-steps/s, served by continuous batching with
+    state resident on the device.  Returns (stats, description)."""
     from ndyn_omp_amd import serve as S
-    probe = W.serving_jobs(args.jobs, seed=12349, size_scale=args.size_scale,
-                           step_scale=args.step_scale, rate_hz=1.0)
+    probe = W.serving_jobs(n_jobs, seed=12349, size_scale=size_scale, step_scale=step_scale,
+                           rate_hz=1.0)
     work = np.mean([s.np * n for _, s, n, _, _ in probe])
-    rate = 0.8 * args.capacity / work
-    stream = W.serving_jobs(args.jobs, seed=12349, size_scale=args.size_scale,
-                            step_scale=args.step_scale, rate_hz=rate)
+    rate = load * capacity / work
+    stream = W.serving_jobs(n_jobs, seed=12349, size_scale=size_scale, step_scale=step_scale,
+                            rate_hz=rate)
     jobs = S.jobs_from_stream(stream)
     log(f"bench: serving {len(jobs)} jobs, mean work {work:.3e} atom-steps, rate {rate:.2f}/s")
-    srv = S.Server(tick_steps=args.tick_steps, graph_steps=args.graph_steps)
+    srv = S.Server(tick_steps=tick_steps, graph_steps=graph_steps)
     # warm the kernels (first launches compile/load code objects)
     warm = S.jobs_from_stream(W.serving_jobs(2, seed=1, size_scale=0.01, step_scale=0.001))
     srv.run(warm, realtime=False)
     st = srv.run(jobs, realtime=True, log=log)
     srv.close()
-    desc = (f"serving: {args.jobs} jobs from the config-4 generator (Morse Cu / Buckingham "
-            f"alloy, NVE/NVT 100-1200 K), sizes log-U[1e3,2e5]x{args.size_scale}, steps "
-            f"log-U[1e3,5e4]x{args.step_scale}, Poisson arrivals {rate:.2f}/s (~80% of "
-            f"{args.capacity:.2e} atom-steps/s), tick {args.tick_steps} steps")
+    scale = "" if size_scale == 1.0 and step_scale == 1.0 else \
+        f" (sizes x{size_scale}, steps x{step_scale})"
+    desc = (f"the first {n_jobs} jobs of the config-4 stream (seed 12349: Morse Cu / "
+            f"Buckingham alloy, NVE/NVT 100-1200 K, sizes log-U[1e3,2e5] atoms, steps "
+            f"log-U[1e3,5e4]{scale}), Poisson arrivals {rate:.3f}/s offering "
+            f"{load:.0%} of {capacity:.3e} atom-steps/s, tick {tick_steps} steps")
+    return st, desc
+
+
+def serve_bench(args):
+    """Serving workload alone (--workload c4): its JSON line has the served rate as value."""
+    st, desc = serving_run(args.jobs, args.capacity, args.size_scale, args.step_scale,
+                           args.tick_steps, args.graph_steps)
     out = {
         "metric": METRIC, "value": round(st["atom_steps_per_s"], 1), "unit": "atom-steps/s",
         "n_gpus": 1, "steps": args.jobs, "warmup": 2, "ms_per_step": None,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 pair math / f64 integration",
         "data": "synthetic (seeded job stream)",
-        "config": {"workload": f"c4: {desc}", "jobs": st["jobs"],
+        "config": {"workload": f"c4: serving {desc}", "jobs": st["jobs"],
                    "p50_latency_s": round(st["p50_latency_s"], 4),
                    "p99_latency_s": round(st["p99_latency_s"], 4),
                    "wall_s": round(st["wall_s"], 3), "busy_s": round(st["busy_s"], 3),
@@ -167,11 +177,27 @@ def force_error_mixed_vs_fp64(eng, slots, systems, k=6):
     return max(errs)
 
 
+def host_cpu():
+    """CPU model and logical CPU count of this host."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count() or 1
+
+
 def cpu_baseline(systems, cfg, budget_s):
-    """The compiled reference (oracle/_ref) on host cores: its own step-loop timer
-    ('Elapsed time for main dynamics') over a bounded sample of the workload."""
+    """The compiled reference on host cores, built at its own Makefile's optimisation level
+    (oracle/_ref_fast; the strict build oracle/_ref when that is absent): its own step-loop
+    timer ('Elapsed time for main dynamics') over a bounded sample of the workload."""
     from oracle import refrun
-    if not refrun.available():
+    fast = refrun.available(fast=True)
+    if not fast and not refrun.available():
         return None
     threads = min(8, os.cpu_count() or 1)   # ptmp(8)/vtmp(6,8) cap the reference at 8
     # The reference's 13-cell half stencil double counts pairs when MCELL < 3 on an axis
@@ -190,7 +216,8 @@ def cpu_baseline(systems, cfg, budget_s):
             try:
                 _, out = refrun.run_harness(refrun.MODE_RUN, s, nsteps=steps, outsteps=steps,
                                             dt=cfg["dt"], skin=cfg["skin"], nthreads=threads,
-                                            iso_ke=cfg.get("iso_ke"), iso_p=cfg.get("iso_p"))
+                                            iso_ke=cfg.get("iso_ke"), iso_p=cfg.get("iso_p"),
+                                            fast=fast)
             except RuntimeError as err:
                 log(f"bench: cpu baseline: system of {s.np} atoms left out "
                     f"({str(err).strip().splitlines()[-1][:80]})")
@@ -213,7 +240,13 @@ def cpu_baseline(systems, cfg, budget_s):
             tot_t, tot_as, used, steps = t2, n2, used2, steps2
     if tot_t <= 0:
         return None
+    model, ncpu = host_cpu()
+    build = ("reference Makefile level: amdflang -O3 -ffast-math -fstack-arrays "
+             "-march=x86-64-v3 (src/Makefile:5 -Ofast -funsafe-math-optimizations, "
+             "-march=native replaced: built in another container)" if fast else
+             "strict: amdflang -O2 -ffp-contract=off")
     return dict(value=tot_as / tot_t, unit="atom-steps/s", cores=threads, kind="reference",
+                build=build, cpu_model=model, host_cpus=ncpu,
                 sample=f"{used} systems of the workload with MCELL>=3 "
                        f"({int(round(tot_as / steps))} atoms), "
                        f"{steps} steps each, reference step-loop timer, OMP_NUM_THREADS={threads}")
@@ -282,11 +315,14 @@ def main():
     ap.add_argument("--cpu-budget-s", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--jobs", type=int, default=2000)
-    ap.add_argument("--size-scale", type=float, default=0.25)
-    ap.add_argument("--step-scale", type=float, default=0.1)
-    # saturated wall-clock throughput of this job sample (arrival rate far above it)
-    ap.add_argument("--capacity", type=float, default=1.8e9)
+    ap.add_argument("--jobs", type=int, default=300)
+    ap.add_argument("--size-scale", type=float, default=1.0)
+    ap.add_argument("--step-scale", type=float, default=1.0)
+    # offered load is 80 % of this rate (atom-steps/s)
+    ap.add_argument("--capacity", type=float, default=2.4e9)
+    ap.add_argument("--serve-jobs", type=int, default=150,
+                    help="jobs of the config-4 stream served after the default workload "
+                         "(p50/p99 latency in the line's config); 0 skips it")
     ap.add_argument("--tick-steps", type=int, default=50)
     ap.add_argument("--drift-steps", type=int, default=10000)
     args = ap.parse_args()
@@ -357,6 +393,21 @@ def main():
     if "force_ms" in mixed and mixed["force_ms"] > 0:
         roof = roofline(mixed, copy_gbs)
 
+    serving = None
+    if args.serve_jobs > 0 and args.workload == "c3":
+        # the served stream at the configuration's own sizes and step counts, offered 80 % of
+        # the rate the engine just sustained on the replica mix
+        st, sdesc = serving_run(args.serve_jobs, mixed["atom_steps_per_s"],
+                                tick_steps=args.tick_steps, graph_steps=args.graph_steps)
+        serving = {"workload": f"c4: {sdesc}", "jobs_completed": st["jobs"],
+                   "p50_latency_s": round(st["p50_latency_s"], 4),
+                   "p99_latency_s": round(st["p99_latency_s"], 4),
+                   "atom_steps_per_s": round(st["atom_steps_per_s"], 1),
+                   "atom_steps_per_busy_s": round(st["atom_steps_per_busy_s"], 1),
+                   "wall_s": round(st["wall_s"], 2), "busy_s": round(st["busy_s"], 2)}
+        log(f"bench: serving {st['jobs']} jobs: p50 {st['p50_latency_s']:.3f} s, p99 "
+            f"{st['p99_latency_s']:.3f} s, {st['atom_steps_per_s']:.4e} atom-steps/s")
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -391,6 +442,7 @@ def main():
             "plain_fp64_path": f"one system per launch, skin {cfg['skin']} A",
             "list_rebuilds_per_step": round(mixed["rebuilds_per_step"], 3),
             "rebuilt_atoms_per_step": round(mixed["rebuilt_atoms_per_step"], 1),
+            "serving": serving,
         },
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -66,6 +66,12 @@ enum {
   NDG_SINE = 6, NDG_COSINE = 7, NDG_SOFT = 8, NDG_JOHNSON = 9,
   NDG_MORSE_C = 10, NDG_BUCK_C = 11, NDG_LJ_C = 12
 };
+/* Added to a kind for ndg_pair_table / ndg_pair_table_q (not in the reference, which
+ * truncates every potential at its cutoff): the energy table holds V(r) - V(rc), the force
+ * table is unchanged.  The forces, and so the trajectory, are the reference's; the potential
+ * energy no longer jumps by V(rc) when a pair crosses the cutoff, so PE + KE of an NVE run is
+ * the quantity its dynamics conserves.  Not for NDG_SPRING. */
+enum { NDG_ESHIFT = 0x100 };
 
 typedef struct ndg_system ndg_t;
 typedef struct ndg_engine ndg_engine_t;

@@ -58,6 +58,7 @@ def build_oracle():
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
     if os.path.isdir("/root/reference/src"):
         _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"])
+        _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref-fast"])
 
 
 def build_all():

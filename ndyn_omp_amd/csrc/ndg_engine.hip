@@ -193,15 +193,20 @@ static double pair_dv(int kind, double r, const double *p, double qprd = 0.0) {
 
 extern "C" int ndg_pair_table_q(int kind, const double params[4], double qprd, double ptrmin,
                                 int ptsize, double *P, double *F, double *rc2, double *dr2) {
-  if (kind < NDG_MORSE || kind > NDG_LJ_C || ptsize < 2 || !P || !F)
+  const bool eshift = (kind & NDG_ESHIFT) != 0;
+  kind &= ~NDG_ESHIFT;
+  if (kind < NDG_MORSE || kind > NDG_LJ_C || ptsize < 2 || !P || !F ||
+      (eshift && kind == NDG_SPRING))
     return fail(NDG_ERR_ARG, "ndg_pair_table: bad arguments");
   const double rc = params[0];
   const double c2 = rc * rc;
   const double h = (c2 - ptrmin) / (double)(ptsize - 1);
+  // NDG_ESHIFT: the pair's energy at its cutoff is taken off the energy table
+  const double vc = eshift ? pair_v(kind, rc, params, qprd) : 0.0;
   for (int i = 0; i <= ptsize; ++i) {
     const double s = ptrmin + (double)i * h;
     const double r = std::sqrt(s);
-    P[i] = pair_v(kind, r, params, qprd);
+    P[i] = pair_v(kind, r, params, qprd) - vc;
     F[i] = r * pair_dv(kind, r, params, qprd);
   }
   if (rc2) *rc2 = c2;
@@ -1802,8 +1807,10 @@ static int engine_admit(ndg_engine_t *e, int *slot, int np, int ntypes, int npai
   if (rc) return rc;
   if (kinds) {
     h.kinds.assign(kinds, kinds + npairs);
-    for (int v : h.kinds)
+    for (int &v : h.kinds) {
+      v &= ~NDG_ESHIFT;   // the energy shift is a table property only
       if (v < NDG_MORSE || v > NDG_LJ_C) return fail(NDG_ERR_ARG, "ndg_engine_admit_ex: unknown kind");
+    }
     if (ipos) {
       // given initial positions; without them the Run's first state is taken, as
       // initPositionSet at the start of dynamics
@@ -2151,8 +2158,10 @@ extern "C" int ndg_set_pair_kinds(ndg_t *h, const int *kinds) {
   if (!h || !kinds) return fail(NDG_ERR_ARG, "ndg_set_pair_kinds: null argument");
   HostState &hs = h->hs;
   std::vector<int> k(kinds, kinds + hs.npairs);
-  for (int v : k)
+  for (int &v : k) {
+    v &= ~NDG_ESHIFT;   // the energy shift is a table property only
     if (v < NDG_MORSE || v > NDG_LJ_C) return fail(NDG_ERR_ARG, "ndg_set_pair_kinds: unknown kind");
+  }
   if (k == hs.kinds) return NDG_OK;
   Slot &s = h->e->slots[h->slot];
   if (s.block)

@@ -16,6 +16,8 @@ module ndg_iface
   integer(c_int), parameter :: NDG_MORSE = 1, NDG_BUCK = 2, NDG_LJ = 3, NDG_MORSE_PHYS = 4, &
                                 NDG_SPRING = 5, NDG_SINE = 6, NDG_COSINE = 7, NDG_SOFT = 8, &
                                 NDG_JOHNSON = 9, NDG_MORSE_C = 10, NDG_BUCK_C = 11, NDG_LJ_C = 12
+  ! added to a kind: energy table V(r) - V(rc), force table unchanged (not in the reference)
+  integer(c_int), parameter :: NDG_ESHIFT = 256
   integer(c_int), parameter :: NDG_FRAME_XYZ = 1, NDG_FRAME_DUMP = 2
 
   type, bind(C) :: ndg_run_params

@@ -200,7 +200,7 @@ program ndyn_gpu
       ! initPositionSet (src/read_input.f90:165, src/nd.f90:1731-1763), applied to the
       ! state the next Run starts from
       if (allocated(pairkind)) then
-        if (any(pairkind == NDG_SPRING)) initpos_pending = .true.
+        if (any(iand(pairkind, not(NDG_ESHIFT)) == NDG_SPRING)) initpos_pending = .true.
       endif
     case ('GroupBufferSize')
       read(w(2),*) grpbuff
@@ -626,6 +626,8 @@ contains
   subroutine read_pairs
     integer :: k, p1, p2, q, e
     real(8) :: rcut
+    logical :: eshift
+    integer :: lw
     if (ntypes == 0) call die('ERROR: must first define particle Types before Pairs')
     if (allocated(pairmat)) deallocate(pairmat, pairkind, pairname, pairparams, ptab, ftab, rc2, &
                                        dr2, pairq)
@@ -648,6 +650,16 @@ contains
       ! (src/read_input.f90:130-133, src/nd.f90:155-163)
       if (trim(w(3)) == 'sym') pairmat(p2,p1) = k
       pairname(k) = trim(w(4))
+      ! '<name>_shift' (not in the reference): the same pair with its energy table shifted
+      ! by V(rc), NDG_ESHIFT; forces are unchanged
+      eshift = .false.
+      lw = len_trim(w(4))
+      if (lw > 6) then
+        if (w(4)(lw-5:lw) == '_shift') then
+          eshift = .true.
+          w(4) = w(4)(1:lw-6)
+        endif
+      endif
       ! the names pot/dpot accept (src/potentials.f90:81-147): 'morsec', 'buckC' and 'buckc'
       ! take their plain forms there; only MorseC/morseC, BuckC and LJC add the Coulomb term
       select case (trim(w(4)))
@@ -680,6 +692,7 @@ contains
       case default
         call die('ERROR: No potential available for Potential Type: '//trim(w(4)))
       end select
+      if (eshift) pairkind(k) = ior(pairkind(k), NDG_ESHIFT)
       do q = 5, nw
         read(w(q),*) pairparams(q-4,k)
       enddo

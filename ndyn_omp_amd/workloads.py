@@ -22,6 +22,9 @@ DT_2FS = 0.196454             # 2 fs
 # pair kinds (same ids as the C ABI / oracle)
 MORSE, BUCK, LJ, MORSE_PHYS, SPRING = 1, 2, 3, 4, 5
 SINE, COSINE, SOFT, JOHNSON, MORSE_C, BUCK_C, LJ_C = 6, 7, 8, 9, 10, 11, 12
+# added to a kind: energy table V(r) - V(rc), same forces (not in the reference, which
+# truncates every potential at its cutoff)
+ESHIFT = 0x100
 
 CU_MORSE = (7.0, 0.3429, 1.3588, 2.866)        # rc, D, alpha, r0
 # peridynamic bond of examples/testPDspringTension/input.sh: rc, C, max strain s0, Vi
@@ -96,7 +99,7 @@ class System:
 
     @property
     def springs(self) -> bool:
-        return bool(np.any(np.asarray(self.kinds) == SPRING))
+        return bool(np.any((np.asarray(self.kinds) & ~ESHIFT) == SPRING))
 
     def copy(self) -> "System":
         return dataclasses.replace(self, pos=self.pos.copy(), vel=self.vel.copy(),

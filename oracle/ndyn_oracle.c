@@ -145,18 +145,23 @@ int ndo_pair_table(int kind, const double params[4], double ptrmin, int ptsize,
   return ndo_pair_table_q(kind, params, 0.0, ptrmin, ptsize, P, F, rc2, dr2);
 }
 
+/* kind | NDO_ESHIFT (not in the reference): P holds V(r) - V(rc), F unchanged */
 int ndo_pair_table_q(int kind, const double params[4], double qprd, double ptrmin, int ptsize,
                      double *P, double *F, double *rc2, double *dr2) {
-  if (kind < NDO_MORSE || kind > NDO_LJ_C || ptsize < 2) return NDO_ERR_ARG;
+  int eshift = (kind & NDO_ESHIFT) != 0;
+  kind &= ~NDO_ESHIFT;
+  if (kind < NDO_MORSE || kind > NDO_LJ_C || ptsize < 2 || (eshift && kind == NDO_SPRING))
+    return NDO_ERR_ARG;
   double rcut = params[0];
   double c2 = rcut * rcut;
   double d2 = (c2 - ptrmin) / (double)(ptsize - 1);
+  double vc = eshift ? pot_fn(kind, rcut, params, qprd) : 0.0;
   *rc2 = c2;
   *dr2 = d2;
   for (int i = 0; i <= ptsize; ++i) {
     double s = ptrmin + (double)i * d2;
     double r = sqrt(s);
-    P[i] = pot_fn(kind, r, params, qprd);
+    P[i] = pot_fn(kind, r, params, qprd) - vc;
     F[i] = r * dpot_fn(kind, r, params, qprd);
   }
   return NDO_OK;

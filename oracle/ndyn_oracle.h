@@ -34,6 +34,8 @@ enum {
   NDO_JOHNSON = 9,                              /* 'John' :136-147, :230-241 */
   NDO_MORSE_C = 10, NDO_BUCK_C = 11, NDO_LJ_C = 12  /* + damped shifted Coulomb term */
 };
+/* added to a kind (not in the reference): energy table V(r) - V(rc), force table unchanged */
+enum { NDO_ESHIFT = 0x100 };
 
 /* list-rebuild trigger */
 enum {

@@ -22,6 +22,7 @@ ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 
 MORSE, BUCK, LJ, MORSE_PHYS, SPRING = 1, 2, 3, 4, 5
 SINE, COSINE, SOFT, JOHNSON, MORSE_C, BUCK_C, LJ_C = 6, 7, 8, 9, 10, 11, 12
+ESHIFT = 0x100
 TRIGGER_COMPAT, TRIGGER_SKIN = 0, 1
 
 

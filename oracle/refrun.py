@@ -14,12 +14,14 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 HARNESS = os.path.join(HERE, "_ref", "ndyn_ref_harness")
+# the same harness at the reference Makefile's optimisation level (timing only, no goldens)
+HARNESS_FAST = os.path.join(HERE, "_ref_fast", "ndyn_ref_harness")
 
 MODE_TABLES, MODE_LIST, MODE_FORCE, MODE_RUN = 1, 2, 3, 4
 
 
-def available() -> bool:
-    return os.path.exists(HARNESS)
+def available(fast: bool = False) -> bool:
+    return os.path.exists(HARNESS_FAST if fast else HARNESS)
 
 
 def write_input(path, mode, s, nsteps=0, outsteps=1, step0=0, neimax=0, skin=-1.0, dt=0.0,
@@ -69,14 +71,16 @@ def write_input(path, mode, s, nsteps=0, outsteps=1, step0=0, neimax=0, skin=-1.
         f.write(np.asarray(np.zeros(s.npairs) if qprd is None else qprd, "<f8").tobytes())
 
 
-def run_harness(mode, s, timeout=3600, **kw):
-    """Returns (raw output bytes, stdout text)."""
+def run_harness(mode, s, timeout=3600, fast=False, **kw):
+    """Returns (raw output bytes, stdout text).  fast=True runs the build at the reference
+    Makefile's optimisation level (for timing; goldens come from the strict build)."""
     with tempfile.TemporaryDirectory() as d:
         fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
         write_input(fin, mode, s, **kw)
         env = dict(os.environ, OMP_NUM_THREADS=str(kw.get("nthreads", 1)),
                    OMP_PROC_BIND=os.environ.get("OMP_PROC_BIND", "close"))
-        p = subprocess.run([HARNESS, fin, fout], cwd=d, env=env, capture_output=True,
+        p = subprocess.run([HARNESS_FAST if fast else HARNESS, fin, fout], cwd=d, env=env,
+                           capture_output=True,
                            text=True, timeout=timeout)
         # a Fortran STOP (e.g. 'too many neighbours in list') exits with status 0
         if p.returncode != 0 or "STOP" in p.stderr:

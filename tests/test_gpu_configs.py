@@ -6,13 +6,13 @@
   size range are compared with oracle runs of the same settings: the FP64 path (batched
   engine, one system per launch) rebuilds its lists at the same steps and ends within
   1e-9 A of the oracle; the mixed path rebuilds within one rebuild of the oracle, ends within
-  1e-4 A (max) of it, and its final forces are within 1e-5 RMS relative of the oracle's
+  1e-6 A (max) of it (measured: the same rebuild steps, 2e-8 A), and its final forces are within 1e-5 RMS relative of the oracle's
   compute0 on a fresh list at the same positions (so every rebuilt list was complete).
 - c2's operating mode (src/nd.f90:1183-1213, :1447-1499 with the skin trigger): Berendsen
   thermostat and the barostat every step on a hot Cu block, skin trigger; the box grows until
   the cell grid changes (3 -> 4 cells per axis), so rebuilds re-grid.  FP64 path: the
   oracle's rebuild count, box within 1e-10, positions within 1e-9 A, thermo PE within 1e-10;
-  mixed path: rebuilds within one, box within 1e-6, positions within 1e-4 A, PE within 1e-5.
+  mixed path: rebuilds within one, box within 1e-6, positions within 1e-6 A, PE within 1e-5.
   After the run the device list holds every pair inside the cutoff, and the last forces
   equal the oracle's compute0 on a fresh list (FP64 1e-12, mixed 1e-5 RMS relative).
 - Full size: c1 (256,000 LJ argon atoms, the configuration's own 0.005 A jitter) and c2
@@ -118,7 +118,7 @@ def test_c3_replica_mix_mixed_path(c3_oracle, oracle):
         print(f"  replica {r} np={s.np} pairs={s.npairs} rebuilds {nup}/{ref[r]['nupdates']} "
               f"max|dx| {dx:.2e} A force err {err:.2e}", flush=True)
         assert abs(nup - ref[r]["nupdates"]) <= 1
-        assert dx < 1e-4
+        assert dx < 1e-6
         assert err <= 1e-5
     eng.close()
     assert total_nup > 0, "the sample rebuilds its lists during the run"
@@ -191,7 +191,7 @@ def test_c2_mode_npt_skin_trigger(c2_mode_oracle, oracle, path_name, path):
     else:
         assert abs(nup - ref["nupdates"]) <= 1
     np.testing.assert_allclose(st["box"], ref["box"], rtol=1e-10 if fp64 else 1e-6)
-    assert dx < (1e-9 if fp64 else 1e-4)
+    assert dx < (1e-9 if fp64 else 1e-6)
     for rec, row in zip(recs, thermo):
         assert rec["step"] == row["step"]
         assert rec["pe"] == pytest.approx(row["pe"], rel=1e-10 if fp64 else 1e-5)
