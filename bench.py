@@ -55,7 +55,7 @@ def workload_systems(name):
                 "Morse Cu (a=3.615) or Buckingham alloy (a=5.6, 50/50 27/63.5 amu), "
                 "T~U[100,1200] K, dt 1 fs, NVE, list skin per size class from the autotuner "
                 "table (0.5 A without it)")
-        return systems, desc, dict(dt=W.DT_1FS, skin=0.5)
+        return systems, desc, dict(dt=W.DT_1FS, skin=0.5, skin_open=True)
     if name == "c1":
         s = W.ar_lj(40)
         return [s], ("LJ argon FCC 40^3 (256000 atoms), rc 2.5 sigma, skin 0.4, 90 K, "
@@ -63,16 +63,17 @@ def workload_systems(name):
                                                                    iso_ke=(1.5 * W.KB_EV * 90.0, W.DT_2FS, 1))
     if name == "c2":
         s = W.cu_morse(80, temp_k=600.0, seed=12347)
-        return [s], ("Morse Cu FCC 80^3 (2048000 atoms), skin 0.5, 600 K, Berendsen "
-                     "thermostat tau 100 fs + isotropic barostat 0 Pa tau 1 ps every 10 steps, "
-                     "dt 1 fs"), dict(dt=W.DT_1FS, skin=0.5,
-                                      iso_ke=(1.5 * W.KB_EV * 600.0, 100 * W.DT_1FS, 1),
-                                      iso_p=((0.0, 0.0, 0.0), 1000 * W.DT_1FS, 10))
+        desc = ("Morse Cu FCC 80^3 (2048000 atoms), 600 K, Berendsen thermostat tau 100 fs "
+                "+ isotropic barostat 0 Pa tau 1 ps every 10 steps, dt 1 fs, list skin from "
+                "the autotuner table (0.5 A without it)")
+        return [s], desc, dict(dt=W.DT_1FS, skin=0.5, skin_open=True,
+                               iso_ke=(1.5 * W.KB_EV * 600.0, 100 * W.DT_1FS, 1),
+                               iso_p=((0.0, 0.0, 0.0), 1000 * W.DT_1FS, 10))
     raise SystemExit(f"unknown workload {name}")
 
 
 def serving_run(n_jobs, capacity, size_scale=1.0, step_scale=1.0, tick_steps=50,
-                graph_steps=0, load=0.8):
+                graph_steps=0, load=0.8, realtime=True):
     """The first n_jobs of the serving stream (W.serving_jobs, seed 12349), Poisson arrivals
     at a rate that offers `load` x `capacity` atom-steps/s, served by continuous batching with
     state resident on the device.  Returns (stats, description)."""
@@ -89,7 +90,7 @@ def serving_run(n_jobs, capacity, size_scale=1.0, step_scale=1.0, tick_steps=50,
     # warm the kernels (first launches compile/load code objects)
     warm = S.jobs_from_stream(W.serving_jobs(2, seed=1, size_scale=0.01, step_scale=0.001))
     srv.run(warm, realtime=False)
-    st = srv.run(jobs, realtime=True, log=log)
+    st = srv.run(jobs, realtime=realtime, log=log)
     srv.close()
     scale = "" if size_scale == 1.0 and step_scale == 1.0 else \
         f" (sizes x{size_scale}, steps x{step_scale})"
@@ -138,11 +139,16 @@ def run_engine(systems, cfg, path, warmup, steps, timing_steps, graph_steps=0, t
         slots = [eng.admit(s, make_rp(cfg, total)) for s in systems]
     atoms = sum(s.np for s in systems)
     eng.step(warmup)
+    # atom-steps are counted from each system's own step counter: a system whose list outgrew
+    # its capacity waits for the rest of that drain interval while its list is regrown
+    before = [eng.counters(sl)["step"] for sl in slots]
     t0 = time.perf_counter()
     eng.step(steps)
     t1 = time.perf_counter()
-    res = dict(atom_steps_per_s=atoms * steps / (t1 - t0), ms_per_step=1e3 * (t1 - t0) / steps,
-               atoms=atoms)
+    after = [eng.counters(sl)["step"] for sl in slots]
+    done = sum(s.np * (b - a) for s, a, b in zip(systems, before, after))
+    res = dict(atom_steps_per_s=done / (t1 - t0), ms_per_step=1e3 * (t1 - t0) / steps,
+               atoms=atoms, atom_steps_counted=done, atom_steps_nominal=atoms * steps)
     cnt = [eng.counters(sl) for sl in slots]
     res["rebuilds_per_step"] = sum(c["nupdates"] for c in cnt) / float(warmup + steps)
     res["rebuilt_atoms_per_step"] = sum(c["nupdates"] * s.np for c, s in zip(cnt, systems)) / float(warmup + steps)
@@ -266,31 +272,61 @@ def roofline(mixed, copy_gbs):
                 frac_of_measured_copy=round(achieved / copy_gbs, 4))
 
 
+def _with_kinds(s, kinds):
+    t = s.copy()
+    t.kinds = np.asarray(kinds, dtype=np.int32)
+    return t
+
+
 def accuracy_bench(args):
-    """Mixed path vs FP64 path per configuration: force / per-atom energy / pressure
-    errors on identical positions, and NVE energy drift per 10^4 steps (both paths)."""
+    """Mixed path vs FP64 path per configuration at the configuration's own size: force /
+    per-atom energy / pressure errors on identical positions (a thermalised snapshot), and
+    NVE energy drift per 10^4 steps on both paths.  Drift is stated with the reference's
+    tables (whose energy jumps by V(rc) as pairs cross the cutoff, and whose Morse force is
+    not the derivative of its energy) and with energy-consistent tables (kind | ESHIFT;
+    morse_phys for Morse), which leave the LJ / Buckingham forces unchanged."""
     from ndyn_omp_amd import accuracy as A
     nsteps = args.drift_steps
+    mix = W.replica_mix(512)
+    order = np.argsort([r.np for r in mix])
+    buck = [mix[k] for k in order if mix[k].npairs == 3][::16][:16]
     cases = [
-        ("c0/c2 Morse Cu (reference table)", W.cu_morse(20, temp_k=600.0, seed=71), 0.5, W.DT_1FS),
-        ("c0/c2 Morse Cu (energy-consistent table)",
-         W.cu_morse(20, temp_k=300.0, seed=72, kind=W.MORSE_PHYS), 0.5, W.DT_1FS),
-        ("c1 LJ argon", W.ar_lj(20, temp_k=90.0, seed=73), 0.4, W.DT_2FS),
-        ("c3 Buckingham alloy", W.buck_alloy((14, 14, 14), temp_k=900.0, seed=74), 0.5, W.DT_1FS),
+        ("c0 Morse Cu 4000 atoms", [W.cu_morse(10, temp_k=300.0, seed=70)], 0.5, W.DT_1FS,
+         [("reference table", None), ("morse_phys + ESHIFT", [W.MORSE_PHYS | W.ESHIFT])]),
+        ("c1 LJ argon 256000 atoms, 2 fs", [W.ar_lj(40)], 0.4, W.DT_2FS,
+         [("reference table", None), ("ESHIFT", [W.LJ | W.ESHIFT])]),
+        ("c2 Morse Cu 2048000 atoms", [W.cu_morse(80, temp_k=600.0, seed=12347)], 0.5, W.DT_1FS,
+         [("reference table", None), ("morse_phys + ESHIFT", [W.MORSE_PHYS | W.ESHIFT])]),
+        (f"c3 Buckingham replicas of the mix ({len(buck)} of 512, "
+         f"{sum(r.np for r in buck)} atoms)", buck, 0.5, W.DT_1FS,
+         [("reference table", None), ("ESHIFT", [W.BUCK | W.ESHIFT] * 3)]),
     ]
     res = []
-    for name, s, skin, dt in cases:
-        r = dict(case=name, atoms=s.np, dt_fs=round(dt / W.DT_1FS, 3))
-        r.update(A.pair_errors(A.thermal_snapshot(s, 200, dt, skin), skin))
-        for path, key in ((G.PATH_MIXED, "nve_mixed"), (G.PATH_FP64, "nve_fp64")):
-            r[key] = A.nve_drift(s, path, nsteps, dt, skin)
+    for name, systems, skin, dt, tables in cases:
+        r = dict(case=name, atoms=int(sum(s.np for s in systems)), dt_fs=round(dt / W.DT_1FS, 3),
+                 drift_steps=nsteps)
+        snaps = [A.thermal_snapshot(s, 200, dt, skin) for s in systems]
+        errs = [A.pair_errors(t, skin) for t in snaps]
+        r["force_rms_rel"] = max(e["force_rms_rel"] for e in errs)
+        r["energy_per_atom_max_abs_ev"] = max(e["energy_per_atom_max_abs_ev"] for e in errs)
+        r["pressure_rel"] = max(e["pressure_rel"] for e in errs)
+        r["pe_rel"] = max(e["pe_rel"] for e in errs)
+        r["drift"] = {}
+        for label, kinds in tables:
+            runs = [s if kinds is None else _with_kinds(s, kinds) for s in systems]
+            d = {}
+            for path, key in ((G.PATH_MIXED, "mixed"), (G.PATH_FP64, "fp64")):
+                dd = A.nve_drift_batch(runs, path, nsteps, dt, skin)
+                d[key] = dd
+            r["drift"][label] = d
+            log(f"bench: {name} [{label}]: drift/1e4 mixed "
+                f"{d['mixed']['drift_per_atom_per_1e4_ev']:.2e} fp64 "
+                f"{d['fp64']['drift_per_atom_per_1e4_ev']:.2e} eV/atom")
         log(f"bench: {name}: force err {r['force_rms_rel']:.2e}, e_i err "
-            f"{r['energy_per_atom_max_abs_ev']:.2e} eV, pressure err {r['pressure_rel']:.2e}, "
-            f"drift/1e4 mixed {r['nve_mixed']['drift_per_atom_per_1e4_ev']:.2e} "
-            f"fp64 {r['nve_fp64']['drift_per_atom_per_1e4_ev']:.2e} eV/atom")
+            f"{r['energy_per_atom_max_abs_ev']:.2e} eV, pressure err {r['pressure_rel']:.2e}")
         res.append(r)
-    print(json.dumps({"metric": "mixed-path accuracy vs FP64 path: force/energy/pressure "
-                                "error and NVE drift per 1e4 steps",
+    print(json.dumps({"metric": "mixed-path accuracy vs FP64 path per configuration: force / "
+                                "energy / pressure error and NVE drift per 1e4 steps",
                       "unit": "eV/atom per 1e4 steps", "n_gpus": 1, "steps": nsteps,
                       "data": "synthetic", "cases": res}), flush=True)
 
@@ -355,8 +391,10 @@ def main():
             table = AT.TuneTable.from_json(open(saved).read())
             tune_src = os.path.relpath(saved, ROOT)
         else:
-            # the replica mix leaves the skin open; single systems keep their configured one
-            skins = AT.DEFAULT_SKINS if args.workload == "c3" and args.skin is None else (cfg["skin"],)
+            # the replica mix and the 2M-atom system leave the skin open (c2 asks only for
+            # about 20 rebuilds in 2000 steps); c1 keeps its configured 0.4 A
+            skins = (AT.DEFAULT_SKINS + (1.5,) if cfg.get("skin_open") and args.skin is None
+                     else (cfg["skin"],))
             # single systems: one class holds the whole batch
             classes = AT.DEFAULT_CLASSES if len(systems) > 1 else (1 << 26,)
             t_tune = time.perf_counter()
@@ -395,11 +433,19 @@ def main():
 
     serving = None
     if args.serve_jobs > 0 and args.workload == "c3":
-        # the served stream at the configuration's own sizes and step counts, offered 80 % of
-        # the rate the engine just sustained on the replica mix
-        st, sdesc = serving_run(args.serve_jobs, mixed["atom_steps_per_s"],
+        # the server's capacity: the first jobs of the stream all submitted at once (admission,
+        # list builds, fetches and releases included), then the stream at the configuration's
+        # own sizes and step counts offered 80 % of that capacity on the wall clock
+        ncal = min(args.serve_jobs, 40)
+        cal, _ = serving_run(ncal, mixed["atom_steps_per_s"], tick_steps=args.tick_steps,
+                             graph_steps=args.graph_steps, realtime=False)
+        log(f"bench: serving capacity {cal['atom_steps_per_s']:.4e} atom-steps/s "
+            f"({ncal} jobs submitted at once)")
+        st, sdesc = serving_run(args.serve_jobs, cal["atom_steps_per_s"],
                                 tick_steps=args.tick_steps, graph_steps=args.graph_steps)
         serving = {"workload": f"c4: {sdesc}", "jobs_completed": st["jobs"],
+                   "capacity_atom_steps_per_s": round(cal["atom_steps_per_s"], 1),
+                   "capacity_jobs": ncal,
                    "p50_latency_s": round(st["p50_latency_s"], 4),
                    "p99_latency_s": round(st["p99_latency_s"], 4),
                    "atom_steps_per_s": round(st["atom_steps_per_s"], 1),
@@ -442,6 +488,9 @@ def main():
             "plain_fp64_path": f"one system per launch, skin {cfg['skin']} A",
             "list_rebuilds_per_step": round(mixed["rebuilds_per_step"], 3),
             "rebuilt_atoms_per_step": round(mixed["rebuilt_atoms_per_step"], 1),
+            # value counts each system's own completed steps (a list regrowth pauses one)
+            "atom_steps_counted_over_nominal": round(
+                mixed["atom_steps_counted"] / mixed["atom_steps_nominal"], 6),
             "serving": serving,
         },
         "roofline": roof,

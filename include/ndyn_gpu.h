@@ -278,6 +278,17 @@ int ndg_engine_admit_ex(ndg_engine_t *e, int *slot, int np, int ntypes, int npai
  * Systems stop at the end of their Run.  *ndone = systems that are finished. */
 int ndg_engine_step(ndg_engine_t *e, int nsteps, int *ndone);
 
+/* ndg_engine_step split in two, so that the host works while the device steps (the engine
+ * counterpart of ndg_run_enqueue / ndg_run_wait; no counterpart in the reference, whose
+ * 'dynamics' runs one system to the end).
+ * ndg_engine_step_enqueue: queue nsteps (at most 512) batched steps and return.
+ * ndg_engine_step_wait: wait for them; errors and *ndone as ndg_engine_step.
+ * While steps are queued, ndg_engine_admit / ndg_engine_admit_ex may be called: the new
+ * system's uploads and list build are queued behind the steps and it joins the next ones.
+ * Every other engine call first waits for the queued steps. */
+int ndg_engine_step_enqueue(ndg_engine_t *e, int nsteps);
+int ndg_engine_step_wait(ndg_engine_t *e, int *ndone);
+
 /* Status of a slot: steps done, finished flag, last thermo record, error code. */
 int ndg_engine_status(ndg_engine_t *e, int slot, int *steps_done, int *finished,
                       ndg_thermo *last, int *err);
@@ -310,6 +321,14 @@ int ndg_engine_config(ndg_engine_t *e, int tpa, int graph_steps);
  * beyond the last class, or with n = 0, use the default.  An engine-wide tpa set by
  * ndg_engine_config takes precedence. */
 int ndg_engine_size_classes(ndg_engine_t *e, int n, const int *np_max, const int *tpa);
+/* Cell-block shape by size class (no counterpart in the reference, whose cells are visited
+ * in the linear order of buildCellList, src/nd.f90:1246): systems admitted later store
+ * their cells in blocks of 1 x by[k] x bz[k] cells (x fastest inside a block), so that the
+ * atoms the force kernel processes together, and their neighbours, lie in a compact region.
+ * Lists and forces are the same sets and sums; only which atoms share a tile changes.
+ * Classes as ndg_engine_size_classes; 1 x 1 (the default) is the linear order. */
+int ndg_engine_cell_blocks(ndg_engine_t *e, int n, const int *np_max, const int *by,
+                           const int *bz);
 /* Average device time (ms) of the force kernel over launches since the last reset,
  * measured with HIP events on the engine's stream; count = launches timed. */
 int ndg_engine_kernel_time(ndg_engine_t *e, int reset, double *avg_ms, long long *count,

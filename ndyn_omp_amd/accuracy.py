@@ -71,3 +71,35 @@ def nve_drift(s: W.System, path: int, nsteps: int, dt: float, skin: float,
                 slope_per_atom_per_1e4_ev=float(slope * 1e4 / n),
                 max_dev_per_atom_ev=float(np.abs(e - e[0]).max() / n),
                 e0_per_atom_ev=float(e[0] / n), err=int(st["err"]))
+
+
+def nve_drift_batch(systems, path: int, nsteps: int, dt: float, skin: float,
+                    out_steps: int = 100) -> dict:
+    """nve_drift of several systems advanced together in one engine; the drift is that of
+    their summed energy, per atom."""
+    eng = G.BatchEngine(path=path)
+    slots = [eng.admit(s, G.run_params(nsteps, dt, out_steps=out_steps, skin=skin,
+                                       trigger=G.TRIGGER_SKIN)) for s in systems]
+    # a system whose list outgrew its capacity pauses while the list is regrown: step until
+    # every system has run its nsteps
+    for _ in range(8):
+        if eng.step(nsteps) == len(slots):
+            break
+    e = None
+    steps = None
+    err = 0
+    for sl in slots:
+        th = eng.thermo(sl, max_rec=nsteps // out_steps + 4)
+        err = err or eng.status(sl)["err"]
+        es = np.array([r["pe"] + r["ke"] for r in th])
+        e = es if e is None else e + es
+        steps = np.array([r["step"] for r in th], dtype=np.float64)
+    eng.close()
+    n = sum(s.np for s in systems)
+    slope = np.polyfit(steps, e, 1)[0] if len(steps) > 2 else 0.0
+    return dict(steps=int(steps[-1]), records=len(steps),
+                drift_per_atom_ev=float((e[-1] - e[0]) / n),
+                drift_per_atom_per_1e4_ev=float((e[-1] - e[0]) / n * 1e4 / max(steps[-1] - steps[0], 1)),
+                slope_per_atom_per_1e4_ev=float(slope * 1e4 / n),
+                max_dev_per_atom_ev=float(np.abs(e - e[0]).max() / n),
+                e0_per_atom_ev=float(e[0] / n), err=int(err))

@@ -2,6 +2,9 @@
 kernel, TPA) and the neighbour-list skin, chosen per size class by timing the batch on
 the GPU.
 
+Cell-block shape: cells stored in blocks of 1 x by x bz instead of the linear order change
+which atoms share a force tile (ndg_engine_cell_blocks); the results are the same bits.
+
 The settings are tuned on the batch they are for (or an even subsample of it when it is
 larger than `max_atoms`), not on each class alone: in a full batch a step's rebuild time
 is set by the largest replica that rebuilds in that step, so a larger skin pays for
@@ -32,31 +35,45 @@ DEFAULT_CLASSES = (2048, 8192, 1 << 26)
 DEFAULT_TPAS = (1, 2, 4)
 DEFAULT_SKINS = (0.3, 0.4, 0.5, 0.75, 1.0, 1.25)
 START_SKIN = 0.5
+# cell-block shapes (by, bz): cells stored in blocks of 1 x by x bz (1 x 1: linear order)
+DEFAULT_BLOCKS = ((1, 1), (2, 1), (2, 2))
 
 
 @dataclasses.dataclass
 class TuneTable:
-    """Per size class: (np_max, tpa, skin), np_max ascending; plus the trial results."""
-    classes: List[Tuple[int, int, float]]
+    """Per size class: (np_max, tpa, skin, by, bz), np_max ascending -- threads per atom,
+    list skin and cell-block shape (cells stored in blocks of 1 x by x bz); plus the trial
+    results."""
+    classes: List[Tuple[int, int, float, int, int]]
     trials: List[dict] = dataclasses.field(default_factory=list)
 
+    def __post_init__(self):
+        # tables saved before the cell-block shape was tuned hold (np_max, tpa, skin)
+        self.classes = [tuple(c) + (1, 1) if len(c) == 3 else tuple(c) for c in self.classes]
+
     def tpa_classes(self):
-        return [(npm, tpa) for npm, tpa, _ in self.classes]
+        return [(c[0], c[1]) for c in self.classes]
+
+    def block_classes(self):
+        return [(c[0], c[3], c[4]) for c in self.classes]
 
     def skin_for(self, np_atoms: int, default: float = 0.5) -> float:
-        for npm, _, skin in self.classes:
-            if np_atoms <= npm:
-                return skin
+        for c in self.classes:
+            if np_atoms <= c[0]:
+                return c[2]
         return default
 
     def apply(self, eng: G.BatchEngine):
         eng.size_classes(self.tpa_classes())
+        eng.cell_blocks(self.block_classes())
 
-    def with_class(self, k: int, tpa: Optional[int] = None,
-                   skin: Optional[float] = None) -> "TuneTable":
+    def with_class(self, k: int, tpa: Optional[int] = None, skin: Optional[float] = None,
+                   block: Optional[Tuple[int, int]] = None) -> "TuneTable":
         c = list(self.classes)
-        npm, t0, s0 = c[k]
-        c[k] = (npm, t0 if tpa is None else tpa, s0 if skin is None else skin)
+        npm, t0, s0, by, bz = c[k]
+        if block is not None:
+            by, bz = block
+        c[k] = (npm, t0 if tpa is None else tpa, s0 if skin is None else skin, by, bz)
         return TuneTable(c)
 
     def to_json(self) -> str:
@@ -68,7 +85,8 @@ class TuneTable:
         return cls([tuple(c) for c in d["classes"]], d.get("trials", []))
 
     def summary(self) -> dict:
-        return {f"np<={npm}": {"tpa": tpa, "skin": skin} for npm, tpa, skin in self.classes}
+        return {f"np<={c[0]}": {"tpa": c[1], "skin": c[2], "cell_block": f"1x{c[3]}x{c[4]}"}
+                for c in self.classes}
 
 
 def subsample(systems: Sequence[W.System], max_atoms: int) -> List[W.System]:
@@ -101,15 +119,16 @@ def trial(systems: Sequence[W.System], make_rp: Callable[[float, int], G.RunPara
 
 def tune(systems: Sequence[W.System], make_rp: Callable[[float, int], G.RunParams],
          classes: Sequence[int] = DEFAULT_CLASSES, tpas: Sequence[int] = DEFAULT_TPAS,
-         skins: Sequence[float] = DEFAULT_SKINS, steps: int = 120, warmup: int = 20,
-         max_atoms: int = 4_000_000, gain: float = 0.01, path: int = G.PATH_MIXED,
-         log: Optional[Callable[[str], None]] = None) -> TuneTable:
-    """Coordinate descent over (tpa, skin) per size class on the batch itself."""
+         skins: Sequence[float] = DEFAULT_SKINS, blocks: Sequence[Tuple[int, int]] = DEFAULT_BLOCKS,
+         steps: int = 120, warmup: int = 20, max_atoms: int = 4_000_000, gain: float = 0.01,
+         path: int = G.PATH_MIXED, log: Optional[Callable[[str], None]] = None) -> TuneTable:
+    """Coordinate descent over (tpa, skin, cell-block shape) per size class on the batch
+    itself."""
     batch = subsample(systems, max_atoms)
     present = [any(lo < s.np <= hi for s in batch)
                for lo, hi in zip((0,) + tuple(classes[:-1]), classes)]
     start = START_SKIN if START_SKIN in skins else skins[0]
-    best = TuneTable([(int(c), 1, float(start)) for c in classes])
+    best = TuneTable([(int(c), 1, float(start), 1, 1) for c in classes])
     trials = []
 
     def measure(t: TuneTable, what: str) -> float:
@@ -135,6 +154,13 @@ def tune(systems: Sequence[W.System], make_rp: Callable[[float, int], G.RunParam
                 continue
             cand = best.with_class(k, skin=skin)
             r = measure(cand, f"np<={classes[k]} skin={skin}")
+            if r > best_rate * (1.0 + gain):
+                best, best_rate = cand, r
+        for blk in blocks:
+            if tuple(blk) == tuple(best.classes[k][3:5]):
+                continue
+            cand = best.with_class(k, block=tuple(blk))
+            r = measure(cand, f"np<={classes[k]} cell block 1x{blk[0]}x{blk[1]}")
             if r > best_rate * (1.0 + gain):
                 best, best_rate = cand, r
     best.trials = trials

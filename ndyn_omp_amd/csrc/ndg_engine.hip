@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <condition_variable>
 #include <cstdio>
@@ -309,7 +310,9 @@ struct ndg_engine {
   std::multimap<size_t, void *> free_blocks;
   int tpa_override = 0;
   std::vector<std::pair<int, int>> tpa_classes;   // (np_max, tpa), np_max ascending
+  std::vector<std::array<int, 3>> cb_classes;      // (np_max, cb_y, cb_z), np_max ascending
   int graph_steps = 0;
+  bool queued = false;   // steps queued by ndg_engine_step_enqueue, not yet drained
   // output snapshot: device staging and pinned host copy of one replica's output state
   double *snap_d = nullptr, *snap_h = nullptr;
   size_t snap_cap = 0, snap_hcap = 0;
@@ -889,6 +892,18 @@ static int upload_host(ndg_engine *e, Slot &s, const HostState &h, bool state_to
   for (int a = 0; a < h.ntypes; ++a)
     for (int b = 0; b < h.ntypes; ++b)
       if (h.pairmat[a + b * h.ntypes] != h.pairmat[b + a * h.ntypes]) P.asym = 1;
+  // several pairs that are the same potential (the binary alloy's A-A, A-B and B-B) and
+  // cover every type pair: the force kernel's single-pair path gives the same bits
+  P.uniform = 1;
+  for (int v : h.pairmat)
+    if (v <= 0) P.uniform = 0;
+  const size_t tab = (size_t)h.ptsize + 1;
+  for (int p = 1; p < h.npairs && P.uniform; ++p)
+    if (h.rc2[p] != h.rc2[0] || h.dr2[p] != h.dr2[0] ||
+        h.pairparams[4 * p] != h.pairparams[0] ||
+        std::memcmp(&h.F[p * tab], &h.F[0], 8 * tab) != 0 ||
+        std::memcmp(&h.P[p * tab], &h.P[0], 8 * tab) != 0)
+      P.uniform = 0;
   // displacement boundary conditions
   P.nbc = P.hbc ? (int)h.bc_dir.size() : 0;
   for (int b = 0; b < P.nbc; ++b) {
@@ -1459,6 +1474,14 @@ static int run_engine(ndg_engine *e, int nsteps, bool drain_last = true) {
                                   : engine_steps(e, nsteps, drain_last);
 }
 
+// Drain the steps ndg_engine_step_enqueue left on the stream (records, errors, step
+// counters); every entry point that reads results or changes what is stepping calls it.
+static int engine_settle(ndg_engine *e) {
+  if (!e->queued) return NDG_OK;
+  e->queued = false;
+  return drain(e);
+}
+
 static size_t snapshot_doubles(int np) { return 10 * (size_t)np + 4; }
 
 // Queue a caller-order copy of slot r's output state (snapshot_doubles(np) doubles) into the
@@ -1715,6 +1738,14 @@ static int begin_run(ndg_engine *e, int r, HostState &h, const ndg_run_params &r
       }
   }
   const int ncap = ncell_capacity(box, rcut, skin);
+  // cell-block shape of the size class (1 x 1: the reference's linear cell order)
+  s.P.cb_y = s.P.cb_z = 1;
+  for (const auto &c : e->cb_classes)
+    if (h.np <= c[0]) {
+      s.P.cb_y = c[1];
+      s.P.cb_z = c[2];
+      break;
+    }
   const bool springs = h.springs();
   if (springs && !h.have_ipos) {
     // initPositionSet at the start of dynamics when no initial positions exist yet
@@ -1865,15 +1896,35 @@ extern "C" int ndg_engine_admit_ex(ndg_engine_t *e, int *slot, int np, int ntype
                       hold, nhold, bc_disp, bc_dir, bc_parm);
 }
 
+static int count_done(ndg_engine *e) {
+  int d = 0;
+  for (auto &s : e->slots)
+    if (s.used && (s.finished || s.err)) ++d;
+  return d;
+}
+
 extern "C" int ndg_engine_step(ndg_engine_t *e, int nsteps, int *ndone) {
   if (!e) return fail(NDG_ERR_ARG, "null engine");
-  int rc = run_engine(e, nsteps);
-  if (ndone) {
-    int d = 0;
-    for (auto &s : e->slots)
-      if (s.used && (s.finished || s.err)) ++d;
-    *ndone = d;
-  }
+  int rc = engine_settle(e);
+  if (!rc) rc = run_engine(e, nsteps);
+  if (ndone) *ndone = count_done(e);
+  return rc;
+}
+
+extern "C" int ndg_engine_step_enqueue(ndg_engine_t *e, int nsteps) {
+  if (!e || nsteps < 0 || nsteps > kRing / 2)
+    return fail(NDG_ERR_ARG, "ndg_engine_step_enqueue: nsteps must be 0.." + std::to_string(kRing / 2));
+  int rc = engine_settle(e);
+  if (rc) return rc;
+  if ((rc = run_engine(e, nsteps, false))) return rc;
+  e->queued = true;
+  return NDG_OK;
+}
+
+extern "C" int ndg_engine_step_wait(ndg_engine_t *e, int *ndone) {
+  if (!e) return fail(NDG_ERR_ARG, "null engine");
+  int rc = engine_settle(e);
+  if (ndone) *ndone = count_done(e);
   return rc;
 }
 
@@ -1881,6 +1932,7 @@ extern "C" int ndg_engine_status(ndg_engine_t *e, int slot, int *steps_done, int
                                  ndg_thermo *last, int *err) {
   if (!e || slot < 0 || slot >= (int)e->slots.size() || !e->slots[slot].used)
     return fail(NDG_ERR_ARG, "bad slot");
+  if (int rq = engine_settle(e)) return rq;
   Slot &s = e->slots[slot];
   if (steps_done) *steps_done = s.step - s.run_step0;
   if (finished) *finished = s.finished ? 1 : 0;
@@ -1893,6 +1945,7 @@ extern "C" int ndg_engine_thermo(ndg_engine_t *e, int slot, ndg_thermo *rec, int
                                  int *nrec) {
   if (!e || slot < 0 || slot >= (int)e->slots.size() || !e->slots[slot].used)
     return fail(NDG_ERR_ARG, "bad slot");
+  if (int rq = engine_settle(e)) return rq;
   Slot &s = e->slots[slot];
   const int n = std::min((int)s.pending_rec.size(), max_rec);
   for (int k = 0; k < n; ++k) rec[k] = s.pending_rec[k];
@@ -1905,6 +1958,7 @@ extern "C" int ndg_engine_fetch(ndg_engine_t *e, int slot, double *pos, double *
                                 double *force, double box[3]) {
   if (!e || slot < 0 || slot >= (int)e->slots.size() || !e->slots[slot].used)
     return fail(NDG_ERR_ARG, "bad slot");
+  if (int rq = engine_settle(e)) return rq;
   EngineDev E = dev_view(e);
   launch_apply_pending(E, slot, 1, e->stream);
   HIPCHK(hipGetLastError());
@@ -1976,6 +2030,7 @@ extern "C" int ndg_engine_fetch_noalloc(ndg_engine_t *e, int slot, double *pos, 
   if (!e || slot < 0 || slot >= (int)e->slots.size() || !e->slots[slot].used ||
       !e->slots[slot].block)
     return fail(NDG_ERR_ARG, "bad slot");
+  if (int rq = engine_settle(e)) return rq;
   int cur = 0;
   HIPCHK(hipGetDevice(&cur));
   if (cur != e->device) HIPCHK(hipSetDevice(e->device));
@@ -1991,6 +2046,7 @@ extern "C" int ndg_engine_fetch_noalloc(ndg_engine_t *e, int slot, double *pos, 
 extern "C" int ndg_engine_release(ndg_engine_t *e, int slot) {
   if (!e || slot < 0 || slot >= (int)e->slots.size() || !e->slots[slot].used)
     return fail(NDG_ERR_ARG, "bad slot");
+  if (int rq = engine_settle(e)) return rq;
   HIPCHK(hipStreamSynchronize(e->stream));
   Slot &s = e->slots[slot];
   dev_release(e, s.block, s.bytes);
@@ -2013,6 +2069,7 @@ extern "C" int ndg_engine_release(ndg_engine_t *e, int slot) {
 extern "C" int ndg_engine_config(ndg_engine_t *e, int tpa, int graph_steps) {
   if (!e || (tpa != 0 && tpa != 1 && tpa != 2 && tpa != 4 && tpa != 8))
     return fail(NDG_ERR_ARG, "tpa must be 0,1,2,4,8");
+  if (int rq = engine_settle(e)) return rq;
   e->tpa_override = e->path == NDG_PATH_FP64 ? 1 : tpa;
   e->graph_steps = graph_steps < 0 ? 0 : graph_steps;
   return NDG_OK;
@@ -2033,6 +2090,22 @@ extern "C" int ndg_engine_size_classes(ndg_engine_t *e, int n, const int *np_max
   return NDG_OK;
 }
 
+extern "C" int ndg_engine_cell_blocks(ndg_engine_t *e, int n, const int *np_max, const int *by,
+                                      const int *bz) {
+  if (!e || n < 0 || n > 8 || (n > 0 && (!np_max || !by || !bz)))
+    return fail(NDG_ERR_ARG, "ndg_engine_cell_blocks: bad arguments");
+  std::vector<std::array<int, 3>> c;
+  for (int k = 0; k < n; ++k) {
+    if (by[k] < 1 || by[k] > 8 || bz[k] < 1 || bz[k] > 8)
+      return fail(NDG_ERR_ARG, "cell-block sides must be 1..8");
+    if (np_max[k] < 1 || (k > 0 && np_max[k] <= np_max[k - 1]))
+      return fail(NDG_ERR_ARG, "np_max must be positive and ascending");
+    c.push_back({np_max[k], by[k], bz[k]});
+  }
+  e->cb_classes = c;
+  return NDG_OK;
+}
+
 extern "C" int ndg_engine_timing(ndg_engine_t *e, int enable) {
   if (!e) return fail(NDG_ERR_ARG, "null engine");
   e->timing = enable != 0;
@@ -2042,6 +2115,7 @@ extern "C" int ndg_engine_timing(ndg_engine_t *e, int enable) {
 extern "C" int ndg_engine_kernel_time(ndg_engine_t *e, int reset, double *avg_ms,
                                       long long *count, double *pairs, double *atoms) {
   if (!e) return fail(NDG_ERR_ARG, "null engine");
+  if (int rq = engine_settle(e)) return rq;
   HIPCHK(hipStreamSynchronize(e->stream));
   int rc = time_collect(e);
   if (rc) return rc;
@@ -2067,6 +2141,7 @@ extern "C" int ndg_engine_counters(ndg_engine_t *e, int slot, int *nupdates, int
                                    double *mean_nbr) {
   if (!e || slot < 0 || slot >= (int)e->slots.size() || !e->slots[slot].used)
     return fail(NDG_ERR_ARG, "bad slot");
+  if (int rq = engine_settle(e)) return rq;
   RepState S;
   int rc = get_state_dev(e, slot, &S);
   if (rc) return rc;
@@ -2078,6 +2153,7 @@ extern "C" int ndg_engine_counters(ndg_engine_t *e, int slot, int *nupdates, int
 
 extern "C" int ndg_engine_list_stats(ndg_engine_t *e, double *mean_nbr, long long *atoms) {
   if (!e) return fail(NDG_ERR_ARG, "null engine");
+  if (int rq = engine_settle(e)) return rq;
   long long a = 0;
   double p = 0;
   for (auto &s : e->slots)

@@ -50,6 +50,10 @@ struct RepParams {
   int renei, renei_freq, trigger;
   int isokinetic, ke_freq, isobaric, p_freq, isoenergetic, e_freq, box_load;
   int asym;                 // PairMat not symmetric: compute1's rules (src/nd.f90:364-530)
+  int uniform;              // every type pair interacts through pairs with identical
+                            // cutoffs and tables: the force kernel treats them as one pair
+  int cb_y, cb_z;           // cell-block shape: cells are stored in blocks of 1 x cb_y x cb_z
+                            // (x fastest inside a block), see cell_rank
   int spring_mask;          // bit p: pair id p+1 is a 'spring' (compute2/3, src/nd.f90:531-933)
   double dt, skin, rcut, ptrmin;
   double ke_target, ke_tau, p_target[3], p_tau, e_tau, box_vel[3];
@@ -136,6 +140,37 @@ struct EngineDev {
 constexpr int kLdsTableBytes = 48 * 1024;       // F tables
 constexpr int kLdsTableBytesFull = 48 * 1024;   // F + P tables, energy/virial variant
 
+
+// Storage rank of cell (kx, ky, kz) of an mx x my x mz grid whose cells are stored in
+// blocks of by x bz cells across y and z: blocks ordered z-slab by z-slab, y-column by
+// y-column (the last ones thinner when by, bz do not divide my, mz), each block x by x,
+// its by bz cells of one x adjacent.  A tile of consecutive atoms then covers a compact
+// group of cells instead of a row.  by = bz = 1 is the reference's linear order
+// kx + ky mx + kz mx my (src/nd.f90:1246).  A bijection onto [0, mx my mz).
+__host__ __device__ inline int cell_rank(int kx, int ky, int kz, int mx, int my, int mz, int by,
+                                         int bz) {
+  const int zb = kz / bz, yb = ky / by;
+  const int hz = min(bz, mz - zb * bz), wy = min(by, my - yb * by);
+  return mx * my * (zb * bz) + mx * hz * (yb * by) + kx * (wy * hz) + (ky - yb * by) +
+         wy * (kz - zb * bz);
+}
+
+// Inverse of cell_rank.
+__host__ __device__ inline void cell_coords(int r, int mx, int my, int mz, int by, int bz,
+                                            int &kx, int &ky, int &kz) {
+  const int slab = mx * my * bz;
+  const int zb = r / slab;
+  int rem = r - zb * slab;
+  const int hz = min(bz, mz - zb * bz);
+  const int col = mx * hz * by;
+  const int yb = rem / col;
+  rem -= yb * col;
+  const int wy = min(by, my - yb * by);
+  kx = rem / (wy * hz);
+  const int t = rem - kx * (wy * hz);
+  ky = yb * by + t % wy;
+  kz = zb * bz + t / wy;
+}
 
 // Flags of the step a replica is about to compute (src/nd.f90:137-151).
 __host__ __device__ inline int step_flags(const RepParams &R, const RepState &S) {

@@ -12,6 +12,11 @@
 
 namespace ndg {
 
+// List entries the force kernel reads per lane and iteration.  The list builder pads every
+// atom's list with absent entries up to a whole number of such groups per lane, so the
+// kernel reads them without a bounds select.
+constexpr int kUnroll = 4;
+
 // ------------------------------------------------------------------ helpers
 
 __device__ inline void tile_range(int t0, int nt, int &begin, int &end, int &step) {
@@ -306,8 +311,6 @@ __global__ __launch_bounds__(kTile) void k_kick_mixed(EngineDev E, int t0, int n
 // every load issued first and the cutoff as a select.  A workgroup of 512 (1024 for
 // TPA 8) threads runs 512/(128 TPA) tiles at a time; TPA lanes share one atom and split
 // its list; lane sums combine in a fixed order.
-constexpr int kUnroll = 4;
-
 template <int TPA>
 struct ForceShape {
   static constexpr int kSub = kTile * TPA;                 // threads per tile
@@ -334,7 +337,6 @@ __device__ inline void force_atom(const RepParams &R, int i, int sub, const doub
   const int ti = (int)XQ[4 * i + 3] + 1;
   const int n = gmem(R.nnb)[i];
   const int nmine = n > sub ? (n - sub + TPA - 1) / TPA : 0;  // entries of this lane
-  const int nt_ = R.ntypes;
   const int tab = R.ptsize + 1;
   const float kmaxf = (float)R.ptsize;
   const double ptrmin = R.ptrmin;
@@ -346,15 +348,17 @@ __device__ inline void force_atom(const RepParams &R, int i, int sub, const doub
   // select drops them
   const unsigned pad = (unsigned)i | (63u << kCodeShift);
   // the list entries of the next iteration are loaded one iteration ahead, so only the
-  // position gathers stand between an iteration's loads and its arithmetic; the list has
-  // kListPad spare chunks, so reads past the end stay in bounds
+  // position gathers stand between an iteration's loads and its arithmetic; every list is
+  // padded with absent entries to a whole number of kUnroll chunks per lane (U divides
+  // kUnroll), so the entries of an iteration need no bounds select
+  static_assert(kUnroll % U == 0, "lists are padded to whole groups of kUnroll chunks");
   unsigned nxt[U];
 #pragma unroll
   for (int u = 0; u < U; ++u) nxt[u] = nmine > 0 ? Lg[(size_t)u * (kTile * TPA)] : pad;
   for (int c = 0; c < nmine; c += U) {
     unsigned e[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) e[u] = (c + u < nmine) ? nxt[u] : pad;
+    for (int u = 0; u < U; ++u) e[u] = nxt[u];
     uint4 pj[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -378,10 +382,10 @@ __device__ inline void force_atom(const RepParams &R, int i, int sub, const doub
       int ltp = lt_0;
       bool feel = true, bal = true;   // ASYM: the neighbour acts on this atom; balanced pair
       if (!ONE) {
-        const int pij = pm[(ti - 1) + (int)pj[u].w * nt_];
+        const int pij = pm[(ti - 1) + (int)pj[u].w * 8];
         pid = pij - 1;
         if (ASYM) {   // compute1 (src/nd.f90:445-498): the larger id of the two directions
-          const int pji = pm[(int)pj[u].w + (ti - 1) * nt_];
+          const int pji = pm[(int)pj[u].w + (ti - 1) * 8];
           pid = (pij > pji ? pij : pji) - 1;
           feel = pji > 0;
           bal = pij == pji;
@@ -464,8 +468,11 @@ __device__ inline void force_atom(const RepParams &R, int i, int sub, const doub
 
 // kick != 0: integrate2 follows in the same pass (v += dt/2 (F_new + F_old)/m, kinetic
 // energy partials), with the arithmetic and the summation order of k_kick_mixed.
+// The force-only instances fit five waves per SIMD (at most 96 VGPRs, no spills) and run
+// ~2 % faster than at four; the other instances keep the registers they need.
 template <int TPA, bool FULL, bool SPR>
-__global__ __launch_bounds__(ForceShape<TPA>::kThreads) void k_force_mixed(EngineDev E, int t0,
+__global__ __launch_bounds__(ForceShape<TPA>::kThreads)
+__attribute__((amdgpu_waves_per_eu(FULL || SPR || TPA > 1 ? 1 : 5))) void k_force_mixed(EngineDev E, int t0,
                                                                            int nt, int xflags,
                                                                            int kick) {
   using SH = ForceShape<TPA>;
@@ -528,7 +535,11 @@ __global__ __launch_bounds__(ForceShape<TPA>::kThreads) void k_force_mixed(Engin
         sh_vdr2[sg][c] = R.vdr2[c];
         sh_lt[sg][c] = R.ltab[c];
       }
-      if (c < R.ntypes * R.ntypes) sh_pm[sg][c] = R.pairmat[c];
+      // PairMat with a row stride of 8 (types are at most 8): indexed by shifts
+      {
+        const int ta = c & 7, tb = c >> 3;
+        sh_pm[sg][c] = ta < R.ntypes && tb < R.ntypes ? R.pairmat[ta + tb * R.ntypes] : 0;
+      }
       if (c == 0) {
         int use = E.tabset_n > 0;
         for (int p = 0; p < R.npairs; ++p) use = use && R.ltab[p] >= 0;
@@ -544,7 +555,10 @@ __global__ __launch_bounds__(ForceShape<TPA>::kThreads) void k_force_mixed(Engin
       const unsigned *L = R.list + ((size_t)T.ltile * R.list_chunks) * (kTile * TPA) + lt;
       // both choices are the same for every wave of the tile
       const bool asym = R.asym != 0;
-      const bool lds = sh_uselds[sg] != 0, one = R.npairs == 1 && !asym && !R.spring_mask;
+      // one pair potential (or several identical ones covering every type pair): its
+      // parameters stay in registers
+      const bool lds = sh_uselds[sg] != 0,
+                 one = (R.npairs == 1 || R.uniform) && !asym && !R.spring_mask;
       // SPR: the instance launched when some replica has spring pairs (the spring
       // variant's registers stay out of the common instance)
       if (SPR && R.spring_mask)
@@ -1140,7 +1154,7 @@ __device__ inline void place_atom(const RepParams &R, const RepState &S, int p) 
   bin_axis(px, S.mcell[0], S.brute, kx, cxa);
   bin_axis(py, S.mcell[1], S.brute, ky, cya);
   bin_axis(pz, S.mcell[2], S.brute, kz, cza);
-  const int cell = kx + ky * S.mcell[0] + kz * S.mcell[0] * S.mcell[1];
+  const int cell = cell_rank(kx, ky, kz, S.mcell[0], S.mcell[1], S.mcell[2], R.cb_y, R.cb_z);
   R.anchor[p] = make_int4(cxa, cya, cza, cell);
   const double4 x = make_double4(anchored(px, cxa, S.mcell[0], S.box[0] / S.mcell[0]),
                                  anchored(py, cya, S.mcell[1], S.box[1] / S.mcell[1]),
@@ -1156,7 +1170,7 @@ __device__ inline int cell_of(const RepParams &R, const RepState &S, int i) {
   bin_axis(R.px[i], S.mcell[0], S.brute, kx, c);
   bin_axis(R.py[i], S.mcell[1], S.brute, ky, c);
   bin_axis(R.pz[i], S.mcell[2], S.brute, kz, c);
-  return kx + ky * S.mcell[0] + kz * S.mcell[0] * S.mcell[1];
+  return cell_rank(kx, ky, kz, S.mcell[0], S.mcell[1], S.mcell[2], R.cb_y, R.cb_z);
 }
 
 // Order of atoms inside a cell: Morton index of the atom's 4 x 4 x 4 sub-cell, then
@@ -1614,13 +1628,14 @@ __device__ long long list_chunk(const RepParams &R, RepState &S, ListWave &L, co
   } else {
     int q_cnt = 0;
     if (lane < 27) {
-      const int cx = cell % mx, cy = (cell / mx) % my, cz = cell / (mx * my);
+      int cx, cy, cz;
+      cell_coords(cell, mx, my, mz, R.cb_y, R.cb_z, cx, cy, cz);
       const int dz = lane / 9 - 1, dy = (lane / 3) % 3 - 1, dx = lane % 3 - 1;
       int ux = cx + dx, uy = cy + dy, uz = cz + dz;
       ux += ux < 0 ? mx : (ux >= mx ? -mx : 0);
       uy += uy < 0 ? my : (uy >= my ? -my : 0);
       uz += uz < 0 ? mz : (uz >= mz ? -mz : 0);
-      const int cu = ux + uy * mx + uz * mx * my;
+      const int cu = cell_rank(ux, uy, uz, mx, my, mz, R.cb_y, R.cb_z);
       L.lo[lane] = R.cell_start[cu];
       q_cnt = R.cell_start[cu + 1] - R.cell_start[cu];
     }
@@ -1766,6 +1781,18 @@ __device__ long long list_chunk(const RepParams &R, RepState &S, ListWave &L, co
   wave_sync();   // the home records are read by every lane above
   const bool act = lane < cnt;
   if (act) R.nnb[lo + lane] = nv < 0 ? 0 : nv;
+  if (act && nv >= 0) {
+    // absent entries (the atom itself with the far offset code 63) up to a multiple of
+    // kUnroll chunks, so that every unrolled read of the force kernel is a list entry
+    const int ih = lo + lane;
+    const auto bh = gmem(C.list + (size_t)(ih >> 7) * C.tstride + (ih & (kTile - 1)) * C.tpa);
+    const unsigned msk = (unsigned)(C.tpa - 1), sh = (unsigned)C.tsh;
+    const int chunks = (nv + C.tpa - 1) / C.tpa;
+    const int end = (chunks + kUnroll - 1) / kUnroll * kUnroll * C.tpa;
+    const unsigned pad = (unsigned)ih | (63u << kCodeShift);
+    for (unsigned k = (unsigned)nv; k < (unsigned)end; ++k)
+      bh[((k >> sh) << (7 + sh)) | (k & msk)] = pad;
+  }
   if (ballot64(act && nv < 0) && lane == 0) atomicMax(&S.err, NDG_ERR_NEIMAX);
   long long mine = act && nv > 0 ? nv : 0;
   for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off);
@@ -1868,13 +1895,21 @@ void k_rb_list(EngineDev E) {
 
 // ------------------------------------------------------------------ bandwidth probe
 
-// Plain streaming copy (16 B per lane per iteration): the reference point for HBM
-// bandwidth and for scaling the FETCH_SIZE / WRITE_SIZE counters.
+// Plain streaming copy: the reference point for HBM bandwidth and for scaling the
+// FETCH_SIZE / WRITE_SIZE counters.  Each lane has four 16-byte loads in flight (grid-stride
+// by four), all issued before the stores.
 __global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ src,
                                               float4 *__restrict__ dst, size_t n) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x)
-    dst[i] = src[i];
+  const size_t S = (size_t)gridDim.x * blockDim.x;
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * S < n; i += 4 * S) {
+    const float4 a = src[i], b = src[i + S], c = src[i + 2 * S], d = src[i + 3 * S];
+    dst[i] = a;
+    dst[i + S] = b;
+    dst[i + 2 * S] = c;
+    dst[i + 3 * S] = d;
+  }
+  for (; i < n; i += S) dst[i] = src[i];
 }
 
 void launch_copy(const float4 *src, float4 *dst, size_t n, hipStream_t s) {

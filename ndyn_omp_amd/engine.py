@@ -107,6 +107,8 @@ def load_library():
                                 dp, dp, dp, dp, C.c_double, ip, dp, dp, dp, V, ip, dp,
                                 P(RunParams), V, V, V, V, C.c_int, V, V, V],
         "ndg_engine_step": [V, C.c_int, P(C.c_int)],
+        "ndg_engine_step_enqueue": [V, C.c_int],
+        "ndg_engine_step_wait": [V, P(C.c_int)],
         "ndg_engine_status": [V, C.c_int, P(C.c_int), P(C.c_int), P(Thermo), P(C.c_int)],
         "ndg_engine_thermo": [V, C.c_int, P(Thermo), C.c_int, P(C.c_int)],
         "ndg_engine_fetch": [V, C.c_int, V, V, V, V],
@@ -114,6 +116,7 @@ def load_library():
         "ndg_engine_release": [V, C.c_int],
         "ndg_engine_config": [V, C.c_int, C.c_int],
         "ndg_engine_size_classes": [V, C.c_int, ip, ip],
+        "ndg_engine_cell_blocks": [V, C.c_int, ip, ip, ip],
         "ndg_engine_timing": [V, C.c_int],
         "ndg_engine_kernel_time": [V, C.c_int, P(C.c_double), P(C.c_longlong), P(C.c_double),
                                    P(C.c_double)],
@@ -408,6 +411,14 @@ class BatchEngine:
         tpa = np.array([c[1] for c in classes], dtype=np.int32)
         _check(load_library().ndg_engine_size_classes(self.e, len(classes), npm, tpa))
 
+    def cell_blocks(self, classes):
+        """Cell-block shape by size class for systems admitted from now on:
+        [(np_max, by, bz), ...] with np_max ascending."""
+        npm = np.array([c[0] for c in classes], dtype=np.int32)
+        by = np.array([c[1] for c in classes], dtype=np.int32)
+        bz = np.array([c[2] for c in classes], dtype=np.int32)
+        _check(load_library().ndg_engine_cell_blocks(self.e, len(classes), npm, by, bz))
+
     def admit(self, s: W.System, rp: RunParams, neimax=0, raise_on_full=True, holds=None):
         """Admit one system; holds = (hold, bc_disp, bc_dir, bc_parm) as Simulation.set_holds.
         Spring systems carry their pair kinds and initial positions (None: the admitted
@@ -451,6 +462,16 @@ class BatchEngine:
     def step(self, nsteps):
         nd = C.c_int(0)
         _check(load_library().ndg_engine_step(self.e, int(nsteps), C.byref(nd)))
+        return nd.value
+
+    def step_enqueue(self, nsteps):
+        """Queue nsteps (<= 512) batched steps and return; admissions may follow while
+        they run (they join the next steps)."""
+        _check(load_library().ndg_engine_step_enqueue(self.e, int(nsteps)))
+
+    def step_wait(self):
+        nd = C.c_int(0)
+        _check(load_library().ndg_engine_step_wait(self.e, C.byref(nd)))
         return nd.value
 
     def status(self, slot):

@@ -360,6 +360,27 @@ module ndg_iface
       real(c_double), intent(out) :: pos(*), vel(*), force(*), box(3)
     end function
 
+    integer(c_int) function ndg_engine_step_enqueue(e, nsteps) &
+        bind(C, name='ndg_engine_step_enqueue')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), value :: nsteps
+    end function
+
+    integer(c_int) function ndg_engine_step_wait(e, ndone) bind(C, name='ndg_engine_step_wait')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), intent(out) :: ndone
+    end function
+
+    integer(c_int) function ndg_engine_cell_blocks(e, n, np_max, by, bz) &
+        bind(C, name='ndg_engine_cell_blocks')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), value :: n
+      integer(c_int), intent(in) :: np_max(*), by(*), bz(*)
+    end function
+
     ! ndg_engine_fetch without allocation (systems of at most 262144 atoms)
     integer(c_int) function ndg_engine_fetch_noalloc(e, slot, pos, vel, force, box) &
         bind(C, name='ndg_engine_fetch_noalloc')

@@ -5,9 +5,12 @@ queued job as soon as its HBM footprint fits the engine's budget, keeps every ad
 job's state (positions, velocities, neighbour lists, thermostat state) resident on the
 device, and advances all resident jobs together: each tick is one batched launch
 sequence of `tick_steps` MD steps over every resident job, each job at its own step
-count.  Finished jobs are fetched and released between ticks, which frees memory for the
-queue.  Latency = completion - arrival; throughput = atom-steps of finished work / wall
-time.  Each job's result does not depend on what else is resident (per-replica kernels,
+count.  A tick is queued on the device before the host admits the jobs that fit
+(ndg_engine_step_enqueue): their host-side preparation overlaps the tick, their uploads and
+list builds run behind it, and they join the next tick.  Finished jobs are fetched and
+released after the tick (ndg_engine_step_wait), which frees memory for the queue.
+Latency = completion - arrival; throughput = atom-steps of finished work / wall time.
+Each job's result does not depend on what else is resident (per-replica kernels,
 fixed-order reductions).
 """
 from __future__ import annotations
@@ -107,21 +110,27 @@ class Server:
         self.t0 = time.perf_counter()
         busy = 0.0
         ticks = 0
+        next_log = 5.0
         while pending or self.queue or self.active:
             t = self.now()
             while pending and (not realtime or pending[0].arrival <= t):
                 self.submit(pending.popleft())
-            self._admit()
             if not self.active:
-                if pending:
-                    time.sleep(max(0.0, min(0.05, pending[0].arrival - self.now())))
-                continue
+                self._admit()
+                if not self.active:
+                    if pending:
+                        time.sleep(max(0.0, min(0.05, pending[0].arrival - self.now())))
+                    continue
             tb = time.perf_counter()
-            self.eng.step(self.tick_steps)
+            self.eng.step_enqueue(self.tick_steps)
+            # jobs admitted now are prepared while the tick runs and join the next one
+            self._admit()
+            self.eng.step_wait()
             busy += time.perf_counter() - tb
             ticks += 1
             self._retire()
-            if log and ticks % 20 == 0:
+            if log and self.now() >= next_log:
+                next_log = self.now() + 5.0
                 log(f"serve: t={self.now():.1f}s done={len(self.done)} active={len(self.active)} "
                     f"queued={len(self.queue)} pending={len(pending)}")
         wall = self.now()

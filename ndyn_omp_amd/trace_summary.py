@@ -12,6 +12,14 @@ With --pmc, it also prints FETCH_SIZE / WRITE_SIZE per dispatch (KiB) of each ke
 two `rocprofv3 --pmc` counter collections, and HBM bytes per dispatch scaled by what the
 same counters read for `k_copy`, which reads and writes --copy-bytes per dispatch
 (ndg_copy_bandwidth's 1 GiB copy): factor = known bytes / counter bytes.
+
+With --sq COUNTERS_CSV AGENT_CSV (a collection of SQ_INSTS_VALU, SQ_WAVES, SQ_WAVE_CYCLES,
+SQ_BUSY_CYCLES, SQ_INSTS_VMEM_RD, SQ_INSTS_LDS, SQ_INSTS_SALU and GRBM_GUI_ACTIVE, and the
+run's agent info), it prints per kernel (largest-grid dispatches): wave instructions, the
+kernel's cycles (GRBM_GUI_ACTIVE over the XCDs), VALU busy = 4 cycles per wave instruction
+over SIMDs x cycles, and waves resident on average (SQ_WAVE_CYCLES counts 4-cycle units).
+--pairs N,K gives kernel K's work units per dispatch (e.g. list entries of the force
+kernel) to print VALU instructions per unit (64 units per wave instruction).
 """
 from __future__ import annotations
 
@@ -100,12 +108,59 @@ def full_grid_mean(pairs):
     return g, statistics.mean(v), len(v)
 
 
+def load_counters(path):
+    """{(kernel, dispatch): {counter: value summed over instances}} and grids."""
+    per = defaultdict(lambda: defaultdict(float))
+    grid = {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            key = (kernel_name(r["Kernel_Name"]), r["Dispatch_Id"])
+            per[key][r["Counter_Name"]] += float(r["Counter_Value"])
+            grid[key] = int(r["Grid_Size"])
+    return per, grid
+
+
+def gpu_agent(path):
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r.get("Agent_Type") == "GPU":
+                return int(r["Simd_Count"]), int(r["Num_Xcc"])
+    raise SystemExit("no GPU agent in " + path)
+
+
+def print_sq(counters_csv, agent_csv, pairs):
+    simds, xcc = gpu_agent(agent_csv)
+    per, grid = load_counters(counters_csv)
+    by = defaultdict(list)
+    for key, c in per.items():
+        by[key[0]].append((grid[key], c))
+    print(f"\nSQ counters per dispatch (largest grid; {simds} SIMDs, {xcc} XCDs): kernel, grid, "
+          f"dispatches, VALU wave-insts, cycles, VALU busy, waves resident, VMEM_RD, LDS, SALU"
+          + (", VALU insts per unit" if pairs else ""))
+    for name in sorted(by):
+        g = max(x[0] for x in by[name])
+        cs = [x[1] for x in by[name] if x[0] == g]
+        m = {k: statistics.mean(c[k] for c in cs) for k in cs[0]}
+        cyc = m.get("GRBM_GUI_ACTIVE", 0.0) / xcc
+        valu = m.get("SQ_INSTS_VALU", 0.0)
+        busy = 4.0 * valu / (simds * cyc) if cyc else 0.0
+        res = 4.0 * m.get("SQ_WAVE_CYCLES", 0.0) / cyc if cyc else 0.0
+        line = (f"{name[:28]:28s} {g:9d} {len(cs):4d} {valu:12.4e} {cyc:11.4e} {busy:7.3f} "
+                f"{res:8.0f} {m.get('SQ_INSTS_VMEM_RD', 0.0):11.4e} {m.get('SQ_INSTS_LDS', 0.0):11.4e} "
+                f"{m.get('SQ_INSTS_SALU', 0.0):11.4e}")
+        if pairs and name.startswith(pairs[1]):
+            line += f" {valu * 64.0 / pairs[0]:8.2f}"
+        print(line)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("trace")
     ap.add_argument("--anchor", default="k_drift_mixed")
     ap.add_argument("--pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
     ap.add_argument("--copy-bytes", type=float, default=float(1 << 30))
+    ap.add_argument("--sq", nargs=2, metavar=("COUNTERS_CSV", "AGENT_CSV"))
+    ap.add_argument("--pairs", default=None, help="N,KERNEL: work units per dispatch of KERNEL")
     a = ap.parse_args()
     rows = load_trace(a.trace)
     win, steps, gmax = window(rows, a.anchor)
@@ -141,6 +196,12 @@ def main():
             if kf:
                 line += f" {fv * 1024 * kf + wv * 1024 * kw:16.4e}"
             print(line)
+    if a.sq:
+        pairs = None
+        if a.pairs:
+            n, k = a.pairs.split(",", 1)
+            pairs = (float(n), k)
+        print_sq(a.sq[0], a.sq[1], pairs)
 
 
 if __name__ == "__main__":

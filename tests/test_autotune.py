@@ -23,5 +23,12 @@ def test_table_lookup_json_and_edit():
     assert t.tpa_classes() == [(2048, 2), (8192, 1), (1 << 26, 1)]
     u = AT.TuneTable.from_json(t.to_json())
     assert u.classes == t.classes
+    # tables saved without a cell-block shape read as the linear order (1 x 1)
+    assert t.classes[1] == (8192, 1, 1.0, 1, 1)
+    assert t.block_classes() == [(2048, 1, 1), (8192, 1, 1), (1 << 26, 1, 1)]
     v = t.with_class(1, skin=1.25)
-    assert v.classes[1] == (8192, 1, 1.25) and t.classes[1] == (8192, 1, 1.0)
+    assert v.classes[1] == (8192, 1, 1.25, 1, 1) and t.classes[1] == (8192, 1, 1.0, 1, 1)
+    w = v.with_class(2, block=(2, 1), tpa=2)
+    assert w.classes[2] == (1 << 26, 2, 0.5, 2, 1)
+    assert w.summary()[f"np<={1 << 26}"] == {"tpa": 2, "skin": 0.5, "cell_block": "1x2x1"}
+    assert AT.TuneTable.from_json(w.to_json()).classes == w.classes

@@ -101,3 +101,30 @@ def test_config_stream_alone_equals_mix():
         np.testing.assert_array_equal(job.result["vel"], mixed.result["vel"])
         np.testing.assert_array_equal(job.result["box"], mixed.result["box"])
         assert job.thermo[-1] == mixed.thermo[-1]
+
+
+def test_admission_while_steps_are_queued():
+    """ndg_engine_step_enqueue / ndg_engine_step_wait: a system admitted while 20 steps of
+    another are queued joins the steps that follow; both end bit-identical to runs alone
+    (50 and 30 steps), and enqueue + wait equals ndg_engine_step."""
+    x = W.cu_morse(7, temp_k=800.0, seed=41)
+    y = W.buck_alloy((6, 6, 6), temp_k=500.0, seed=42)
+    rp = lambda n: G.run_params(n, W.DT_1FS, out_steps=10, skin=0.5, trigger=G.TRIGGER_SKIN)
+    eng = G.BatchEngine(path=G.PATH_MIXED)
+    sx = eng.admit(x, rp(50))
+    eng.step_enqueue(20)
+    sy = eng.admit(y, rp(30))            # while the 20 steps are queued
+    assert eng.step_wait() == 0
+    assert eng.counters(sx)["step"] == 20 and eng.counters(sy)["step"] == 0
+    eng.step_enqueue(30)
+    assert eng.step_wait() == 2
+    got = {k: eng.fetch(s, m.np) for k, s, m in (("x", sx, x), ("y", sy, y))}
+    eng.close()
+    for k, m, n in (("x", x, 50), ("y", y, 30)):
+        e1 = G.BatchEngine(path=G.PATH_MIXED)
+        s1 = e1.admit(m, rp(n))
+        e1.step(n)
+        ref = e1.fetch(s1, m.np)
+        e1.close()
+        for q in ("pos", "vel", "box"):
+            np.testing.assert_array_equal(got[k][q], ref[q])

@@ -72,12 +72,12 @@ def test_autotune_small():
     sysl = systems()
     table = AT.tune(sysl, lambda skin, n: G.run_params(n, W.DT_1FS, out_steps=n, skin=skin,
                                                        trigger=G.TRIGGER_SKIN),
-                    classes=(1000, 1 << 26), tpas=(1, 2), skins=(0.5, 0.8), steps=20,
-                    warmup=5)
+                    classes=(1000, 1 << 26), tpas=(1, 2), skins=(0.5, 0.8),
+                    blocks=((1, 1), (2, 2)), steps=20, warmup=5)
     assert [c[0] for c in table.classes] == [1000, 1 << 26]
-    for _, tpa, skin in table.classes:
-        assert tpa in (1, 2) and skin in (0.5, 0.8)
-    assert len(table.trials) == 5          # start + (one tpa + one skin) per class
+    for _, tpa, skin, by, bz in table.classes:
+        assert tpa in (1, 2) and skin in (0.5, 0.8) and (by, bz) in ((1, 1), (2, 2))
+    assert len(table.trials) == 7          # start + (one tpa + one skin + one block) per class
     assert all(t["atom_steps_per_s"] > 0 for t in table.trials)
     eng = G.BatchEngine(path=G.PATH_MIXED)
     table.apply(eng)
