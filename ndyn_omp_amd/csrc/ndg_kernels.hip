@@ -994,14 +994,17 @@ __global__ __launch_bounds__(1024) void k_reduce(EngineDev E, int r0, int nr, in
     const int nkw = 2 * R.tpa;   // kinetic-energy slots of a mixed-path tile
     const int nthr = ntl > 1024 ? 1024 : 256;
     const int t0 = (int)threadIdx.x < nthr ? (int)threadIdx.x : ntl;   // idle threads: no tiles
-    // one quantity at a time, several tiles' loads in flight per thread
+    // one quantity at a time, eight tiles' loads in flight per thread (a large replica is
+    // summed by one workgroup, whose time is the number of round trips to memory)
     if (f & F_CK) {
       if (R.path == NDG_PATH_MIXED) {
-#pragma unroll 4
-        for (int lt = t0; lt < ntl; lt += nthr)
-          for (int w = 0; w < nkw; ++w) v[0] += p[(kPartKeW + w) * ntl + lt];
+        for (int w = 0; w < nkw; ++w) {
+          const auto pw = p + (size_t)(kPartKeW + w) * ntl;
+#pragma unroll 8
+          for (int lt = t0; lt < ntl; lt += nthr) v[0] += pw[lt];
+        }
       } else {
-#pragma unroll 4
+#pragma unroll 8
         for (int lt = t0; lt < ntl; lt += nthr) v[0] += p[kPartKe * ntl + lt];
       }
     }
@@ -1011,7 +1014,7 @@ __global__ __launch_bounds__(1024) void k_reduce(EngineDev E, int r0, int nr, in
 #pragma unroll
         for (int q = 1; q < 8; ++q) v[q] += p[q * ntl + lt];
     }
-#pragma unroll 4
+#pragma unroll 8
     for (int lt = t0; lt < ntl; lt += nthr)
       top2_merge(t1, t2, p[kPartD1 * ntl + lt], p[kPartD2 * ntl + lt]);
     block_sum<8>(v, sh);
