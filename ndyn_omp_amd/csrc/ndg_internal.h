@@ -133,7 +133,13 @@ struct EngineDev {
   int tabset_n;         // floats of the F half (0: tables are read from global memory)
   int tabset_p;         // 1: the energy/virial force variant also stages the P half
   int springs;          // some replica has spring pairs: launch the force variants with them
+  int list_nt;          // 1: the lists exceed the last-level cache; the force kernel streams
+                        // them with non-temporal loads
 };
+
+// Size of the device's last-level cache (MI355X Infinity Cache, 256 MB): lists that fit
+// stay cached from one step to the next, longer ones are streamed past it.
+constexpr double kListCacheBytes = 256.0 * 1024 * 1024;
 
 // LDS budgets of the staged tables: with the force kernel's own LDS (at most ~15 KB) a
 // workgroup stays within 64 KB

@@ -52,7 +52,8 @@ def load_trace(path):
 def window(rows, anchor="k_drift_mixed", gap_ms=50.0):
     """The last contiguous run of batched steps: largest-grid anchor launches, split where
     consecutive ones are more than gap_ms apart (admission of a new batch between runs)."""
-    drifts = [r for r in rows if r["name"] == anchor]
+    # a templated kernel matches by its name without the template arguments
+    drifts = [r for r in rows if r["name"].split("<")[0] == anchor]
     if not drifts:
         return rows, 0, 0
     gmax = max(r["grid"] for r in drifts)
