@@ -74,8 +74,19 @@ def workload_systems(name):
     raise SystemExit(f"unknown workload {name}")
 
 
+SERVE_TUNE = os.path.join(ROOT, "ndyn_omp_amd", "tuning", "c4.json")
+
+
+def serving_table():
+    """The autotuner table of the serving stream (ndyn_omp_amd/tuning/c4.json), if saved."""
+    if not os.path.exists(SERVE_TUNE):
+        return None, None
+    from ndyn_omp_amd import autotune as AT
+    return AT.TuneTable.from_json(open(SERVE_TUNE).read()), os.path.relpath(SERVE_TUNE, ROOT)
+
+
 def serving_run(n_jobs, capacity, size_scale=1.0, step_scale=1.0, tick_steps=50,
-                graph_steps=0, load=0.8, realtime=True):
+                graph_steps=0, load=0.8, realtime=True, table=None):
     """The first n_jobs of the serving stream (W.serving_jobs, seed 12349), Poisson arrivals
     at a rate that offers `load` x `capacity` atom-steps/s, served by continuous batching with
     state resident on the device.  Returns (stats, description)."""
@@ -88,7 +99,7 @@ def serving_run(n_jobs, capacity, size_scale=1.0, step_scale=1.0, tick_steps=50,
                             rate_hz=rate)
     jobs = S.jobs_from_stream(stream)
     log(f"bench: serving {len(jobs)} jobs, mean work {work:.3e} atom-steps, rate {rate:.2f}/s")
-    srv = S.Server(tick_steps=tick_steps, graph_steps=graph_steps)
+    srv = S.Server(tick_steps=tick_steps, graph_steps=graph_steps, table=table)
     # warm the kernels (first launches compile/load code objects)
     warm = S.jobs_from_stream(W.serving_jobs(2, seed=1, size_scale=0.01, step_scale=0.001))
     srv.run(warm, realtime=False)
@@ -103,10 +114,43 @@ def serving_run(n_jobs, capacity, size_scale=1.0, step_scale=1.0, tick_steps=50,
     return st, desc
 
 
+# size classes of the serving table: the replica mix's three, and jobs beyond 65536 atoms
+SERVE_CLASSES = (2048, 8192, 65536, 1 << 26)
+
+
+def tune_serving(args):
+    """Autotuner table of the serving stream: TPA, skin and cell-block shape per size class,
+    timed on the first --tune-jobs systems of the stream advanced together (a resident mix
+    at the stream's own sizes)."""
+    from ndyn_omp_amd import autotune as AT
+    stream = W.serving_jobs(args.tune_jobs, seed=12349, size_scale=args.size_scale,
+                            step_scale=args.step_scale)
+    systems = [s for _, s, _, _, _ in stream]
+
+    def rp(skin, n):
+        return G.run_params(n, W.DT_1FS, out_steps=n, skin=skin, trigger=G.TRIGGER_SKIN)
+
+    skins = (0.5, 0.75, 1.0, 1.25)
+    table = AT.tune(systems, rp, classes=SERVE_CLASSES, skins=skins, log=log)
+    log(f"bench: serving autotune {table.summary()}")
+    if args.save_tune:
+        with open(args.save_tune, "w") as f:
+            f.write(table.to_json())
+    return table
+
+
 def serve_bench(args):
     """Serving workload alone (--workload c4): its JSON line has the served rate as value."""
+    if args.tune == "run":
+        table, src = tune_serving(args), "tuned in this run"
+    elif args.tune == "off" or args.no_tune:
+        table, src = None, None
+    else:
+        table, src = serving_table()
     st, desc = serving_run(args.jobs, args.capacity, args.size_scale, args.step_scale,
-                           args.tick_steps, args.graph_steps)
+                           args.tick_steps, args.graph_steps, table=table)
+    if src:
+        desc += f", size classes from {src}"
     out = {
         "metric": METRIC, "value": round(st["atom_steps_per_s"], 1), "unit": "atom-steps/s",
         "n_gpus": 1, "steps": args.jobs, "warmup": 2, "ms_per_step": None,
@@ -117,7 +161,9 @@ def serve_bench(args):
                    "p50_latency_s": round(st["p50_latency_s"], 4),
                    "p99_latency_s": round(st["p99_latency_s"], 4),
                    "wall_s": round(st["wall_s"], 3), "busy_s": round(st["busy_s"], 3),
-                   "atom_steps_per_busy_s": round(st["atom_steps_per_busy_s"], 1)},
+                   "atom_steps_per_busy_s": round(st["atom_steps_per_busy_s"], 1),
+                   "host_s": {k: round(st[k], 3) for k in ("admit_s", "wait_s", "retire_s", "idle_s")},
+                   "ticks": st["ticks"], "mean_resident_atoms": round(st["mean_resident_atoms"])},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
@@ -354,6 +400,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--jobs", type=int, default=300)
+    ap.add_argument("--tune-jobs", type=int, default=60,
+                    help="c4 --tune run: stream systems timed together by the autotuner")
     ap.add_argument("--size-scale", type=float, default=1.0)
     ap.add_argument("--step-scale", type=float, default=1.0)
     # offered load is 80 % of this rate (atom-steps/s)
@@ -439,12 +487,16 @@ def main():
         # list builds, fetches and releases included), then the stream at the configuration's
         # own sizes and step counts offered 80 % of that capacity on the wall clock
         ncal = min(args.serve_jobs, 40)
+        stab, ssrc = serving_table()
         cal, _ = serving_run(ncal, mixed["atom_steps_per_s"], tick_steps=args.tick_steps,
-                             graph_steps=args.graph_steps, realtime=False)
+                             graph_steps=args.graph_steps, realtime=False, table=stab)
         log(f"bench: serving capacity {cal['atom_steps_per_s']:.4e} atom-steps/s "
             f"({ncal} jobs submitted at once)")
         st, sdesc = serving_run(args.serve_jobs, cal["atom_steps_per_s"],
-                                tick_steps=args.tick_steps, graph_steps=args.graph_steps)
+                                tick_steps=args.tick_steps, graph_steps=args.graph_steps,
+                                table=stab)
+        if ssrc:
+            sdesc += f", size classes from {ssrc}"
         serving = {"workload": f"c4: {sdesc}", "jobs_completed": st["jobs"],
                    "capacity_atom_steps_per_s": round(cal["atom_steps_per_s"], 1),
                    "capacity_jobs": ncal,
@@ -452,7 +504,9 @@ def main():
                    "p99_latency_s": round(st["p99_latency_s"], 4),
                    "atom_steps_per_s": round(st["atom_steps_per_s"], 1),
                    "atom_steps_per_busy_s": round(st["atom_steps_per_busy_s"], 1),
-                   "wall_s": round(st["wall_s"], 2), "busy_s": round(st["busy_s"], 2)}
+                   "wall_s": round(st["wall_s"], 2), "busy_s": round(st["busy_s"], 2),
+                   "host_s": {k: round(st[k], 3) for k in ("admit_s", "wait_s", "retire_s", "idle_s")},
+                   "mean_resident_atoms": round(st["mean_resident_atoms"])}
         log(f"bench: serving {st['jobs']} jobs: p50 {st['p50_latency_s']:.3f} s, p99 "
             f"{st['p99_latency_s']:.3f} s, {st['atom_steps_per_s']:.4e} atom-steps/s")
 

@@ -59,9 +59,16 @@ class Job:
 
 
 class Server:
+    """table: an autotuner table (ndyn_omp_amd.autotune.TuneTable) whose size classes set the
+    threads per atom and cell-block shape of admitted jobs, and whose skins replace a job's
+    list skin by its size class."""
+
     def __init__(self, hbm_budget: int = 0, tick_steps: int = 50, path: int = G.PATH_MIXED,
-                 graph_steps: int = 0):
+                 graph_steps: int = 0, table=None):
         self.eng = G.BatchEngine(path=path, hbm_budget=hbm_budget, graph_steps=graph_steps)
+        self.table = table
+        if table is not None:
+            table.apply(self.eng)
         self.tick_steps = tick_steps
         self.queue: deque = deque()
         self.active: Dict[int, Job] = {}
@@ -78,6 +85,8 @@ class Server:
         """Admit queued jobs in arrival order while their footprint fits."""
         while self.queue:
             job = self.queue[0]
+            if self.table is not None:
+                job.skin = self.table.skin_for(job.system.np, job.skin)
             slot = self.eng.admit(job.system, job.run_params(), raise_on_full=False,
                                   holds=job.holds)
             if slot is None:
@@ -110,25 +119,38 @@ class Server:
         self.t0 = time.perf_counter()
         busy = 0.0
         ticks = 0
+        # host time by phase: admissions while a tick runs, the wait for the tick, the
+        # retirement of finished jobs (status, thermo, fetch, release), idle waiting for work
+        t_admit = t_wait = t_retire = t_idle = 0.0
+        resident = 0.0   # resident atoms summed over ticks
         next_log = 5.0
         while pending or self.queue or self.active:
             t = self.now()
             while pending and (not realtime or pending[0].arrival <= t):
                 self.submit(pending.popleft())
             if not self.active:
+                ti = time.perf_counter()
                 self._admit()
                 if not self.active:
                     if pending:
                         time.sleep(max(0.0, min(0.05, pending[0].arrival - self.now())))
+                    t_idle += time.perf_counter() - ti
                     continue
+                t_idle += time.perf_counter() - ti
             tb = time.perf_counter()
+            resident += sum(j.system.np for j in self.active.values())
             self.eng.step_enqueue(self.tick_steps)
             # jobs admitted now are prepared while the tick runs and join the next one
             self._admit()
+            ta = time.perf_counter()
             self.eng.step_wait()
-            busy += time.perf_counter() - tb
+            tw = time.perf_counter()
+            busy += tw - tb
+            t_admit += ta - tb
+            t_wait += tw - ta
             ticks += 1
             self._retire()
+            t_retire += time.perf_counter() - tw
             if log and self.now() >= next_log:
                 next_log = self.now() + 5.0
                 log(f"serve: t={self.now():.1f}s done={len(self.done)} active={len(self.active)} "
@@ -137,6 +159,8 @@ class Server:
         lat = np.array([j.finished - j.arrival for j in self.done])
         atom_steps = float(sum(j.atom_steps for j in self.done))
         return dict(jobs=len(self.done), wall_s=wall, busy_s=busy, ticks=ticks,
+                    admit_s=t_admit, wait_s=t_wait, retire_s=t_retire, idle_s=t_idle,
+                    mean_resident_atoms=resident / ticks if ticks else 0.0,
                     atom_steps=atom_steps, atom_steps_per_s=atom_steps / wall,
                     atom_steps_per_busy_s=atom_steps / busy if busy > 0 else 0.0,
                     p50_latency_s=float(np.percentile(lat, 50)) if len(lat) else None,
