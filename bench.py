@@ -387,10 +387,12 @@ def main():
     ap.add_argument("--workload", default="c3")
     ap.add_argument("--timing-steps", type=int, default=100)
     ap.add_argument("--plain-steps", type=int, default=100)
-    # the batched step of the configurations runs as hipGraphs of 10 steps (measured neutral
-    # against eager launches: c3 -0.1 %, c1 -0.2 %); the server ticks run eagerly, since every
-    # admission changes the tile set and would force a new capture
-    ap.add_argument("--graph-steps", type=int, default=10)
+    # K-step hipGraphs of the batched step (--graph-steps K) measure neutral against eager
+    # launches (c3 -0.1 %, c1 -0.2 % at K = 10); the default is eager, whose runs the
+    # rocprofv3 kernel trace of this image can record (it stopped with a host segmentation
+    # fault on graph launches).  The server's ticks stay eager: every admission changes the
+    # tile set and would force a new capture.
+    ap.add_argument("--graph-steps", type=int, default=0)
     ap.add_argument("--serve-graph-steps", type=int, default=0)
     ap.add_argument("--tpa", type=int, default=0)
     ap.add_argument("--skin", type=float, default=None,
