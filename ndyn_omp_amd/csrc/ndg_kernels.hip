@@ -2062,7 +2062,10 @@ void launch_rebuild(const EngineDev &E, int nt, int r0, int nr, hipStream_t s) {
   // nt: tiles held by the replicas [r0, r0 + nr), a bound on the work list's length
   if (nt <= 0 || nr <= 0) return;
   const int gr = nr < 1024 ? nr : 1024;
-  const int g = nt;   // one workgroup per work-list entry at most
+  // the sort kernels stride over the work list: at most 2048 workgroups (eight per CU), so
+  // that a step in which few tiles rebuild does not dispatch a workgroup per tile (c2, 16000
+  // tiles rebuilt every ~45 steps: 1.017 -> 1.007 ms per step; c3 and c1 unchanged)
+  const int g = nt < 2048 ? nt : 2048;
   hipLaunchKernelGGL(k_rb_plan, dim3(1), dim3(1024), 0, s, E, r0, nr);
   hipLaunchKernelGGL(k_rb_bin, dim3(g), dim3(kTile), 0, s, E);
   hipLaunchKernelGGL(k_rb_scan, dim3(gr), dim3(1024), 0, s, E, r0, nr);
