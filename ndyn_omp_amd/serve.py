@@ -6,8 +6,9 @@ job's state (positions, velocities, neighbour lists, thermostat state) resident 
 device, and advances all resident jobs together: each tick is one batched launch
 sequence of `tick_steps` MD steps over every resident job, each job at its own step
 count.  A tick is queued on the device before the host admits the jobs that fit
-(ndg_engine_step_enqueue): their host-side preparation overlaps the tick, their uploads and
-list builds run behind it, and they join the next tick.  Finished jobs are fetched and
+(ndg_engine_step_enqueue): their host-side preparation runs while the tick executes, their
+uploads and list builds follow it on the stream (an admission returns once its build is
+checked), and they join the next tick.  Finished jobs are fetched and
 released after the tick (ndg_engine_step_wait), which frees memory for the queue.
 Latency = completion - arrival; throughput = atom-steps of finished work / wall time.
 Each job's result does not depend on what else is resident (per-replica kernels,
