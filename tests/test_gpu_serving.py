@@ -128,3 +128,33 @@ def test_admission_while_steps_are_queued():
         e1.close()
         for q in ("pos", "vel", "box"):
             np.testing.assert_array_equal(got[k][q], ref[q])
+
+
+def test_served_with_size_class_table_alone_equals_mix():
+    """A server with an autotuner table: jobs of different size classes run with different
+    threads per atom (1 and 2) and list skins (0.5 and 1.0 A) in the same ticks.  Each job
+    served alone under the same table gives bit-identical results, and the skin of its size
+    class is the one it ran with (a job's kernels depend only on its own class)."""
+    from ndyn_omp_amd import autotune as AT
+    table = AT.TuneTable([(2000, 1, 0.5, 1, 1), (1 << 26, 2, 1.0, 1, 2)])
+    alone = {}
+    for job in make_jobs():
+        print(f"case table alone job {job.jid} np={job.system.np}", flush=True)
+        srv = S.Server(tick_steps=16, table=table)
+        srv.run([job], realtime=False)
+        alone[job.jid] = job
+        srv.close()
+    jobs = make_jobs()
+    print("case table mixed", flush=True)
+    srv = S.Server(tick_steps=9, table=table)
+    stats = srv.run(jobs, realtime=False)
+    srv.close()
+    assert stats["jobs"] == 4
+    for j in jobs:
+        assert j.skin == (0.5 if j.system.np <= 2000 else 1.0)
+        a = alone[j.jid]
+        np.testing.assert_array_equal(j.result["pos"], a.result["pos"])
+        np.testing.assert_array_equal(j.result["vel"], a.result["vel"])
+        assert j.thermo[-1] == a.thermo[-1]
+    assert stats["ticks"] > 0 and stats["mean_resident_atoms"] > 0
+    assert stats["wait_s"] >= 0 and stats["admit_s"] >= 0 and stats["retire_s"] >= 0
