@@ -519,17 +519,28 @@ __attribute__((amdgpu_waves_per_eu(FULL || SPR || TPA > 1 ? 1 : 5))) void k_forc
   __shared__ int sh_lt[NSUB][16];
   __shared__ int sh_uselds[NSUB];
   __shared__ double sh_red[NW][8];
+  __shared__ int sh_flags[NSUB], sh_act[NSUB];
+  __shared__ double sh_qu[NSUB];
   // the F half of the table set; the energy/virial variant also the P half when it fits
   const int ptoff = FULL && E.tabset_p ? E.tabset_n : -1;
   const int nstage = ptoff >= 0 ? 2 * E.tabset_n : E.tabset_n;
   for (int q = threadIdx.x; q < nstage; q += blockDim.x) sh_tab[q] = E.tabset[q];
-  const int sg = threadIdx.x / SUB, lt = threadIdx.x % SUB;
+  // CACHE (TPA >= 2): the sub-group index is made wave-uniform (a wave never spans two
+  // sub-groups), so tile and replica reads use scalar addresses, and a replica's step
+  // flags and record unit are evaluated once, when its parameters are staged, instead of
+  // by a chain of dependent reads in every pass (c2 force launch 0.752 -> 0.717 ms, c1
+  // 0.0505 -> 0.0480 ms).  The TPA 1 instance keeps the per-pass form: made uniform it
+  // ran 1 % slower on the replica mix, and the LDS copies spill at its 96-VGPR budget.
+  constexpr bool CACHE = TPA >= 2;
+  const int sg = CACHE ? __builtin_amdgcn_readfirstlane(threadIdx.x / SUB) : threadIdx.x / SUB;
+  const int lt = threadIdx.x % SUB;
   const int a = lt / TPA, sub = lt % TPA;
   const int nblk = (nt + NSUB - 1) / NSUB;
   int bb, be, bs;
   tile_range(0, nblk, bb, be, bs);
-  // replica whose parameters each sub-group holds in LDS (box and grid do not change
-  // during a launch, so a sub-group keeps them while its tiles stay in one replica)
+  // replica whose parameters each sub-group holds in LDS (box, grid and step counter do
+  // not change during a launch, so a sub-group keeps them while its tiles stay in one
+  // replica)
   int held[NSUB];
 #pragma unroll
   for (int q = 0; q < NSUB; ++q) held[q] = -1;
@@ -540,7 +551,7 @@ __attribute__((amdgpu_waves_per_eu(FULL || SPR || TPA > 1 ? 1 : 5))) void k_forc
     const RepParams &R = E.reps[T.rep];
     const RepState &S = E.st[T.rep];
     int f = 0;
-    if (have) {
+    if (!CACHE && have) {
       f = active_flags(E, T.rep, xflags);
       const bool full = (f & (F_CU | F_CV)) != 0;
       have = (f & F_ACTIVE) && R.tpa == TPA && full == FULL;
@@ -574,6 +585,13 @@ __attribute__((amdgpu_waves_per_eu(FULL || SPR || TPA > 1 ? 1 : 5))) void k_forc
         const int ta = c & 7, tb = c >> 3;
         sh_pm[sg][c] = ta < R.ntypes && tb < R.ntypes ? R.pairmat[ta + tb * R.ntypes] : 0;
       }
+      if (CACHE && c == 0) {
+        const int fl = active_flags(E, T.rep, xflags);
+        const bool full = (fl & (F_CU | F_CV)) != 0;
+        sh_flags[sg] = fl;
+        sh_act[sg] = (fl & F_ACTIVE) && R.tpa == TPA && full == FULL;
+        sh_qu[sg] = xq_unit(S, R.skin);
+      }
       if (c == 0) {
         int use = E.tabset_n > 0;
         for (int p = 0; p < R.npairs; ++p) use = use && R.ltab[p] >= 0;
@@ -581,8 +599,12 @@ __attribute__((amdgpu_waves_per_eu(FULL || SPR || TPA > 1 ? 1 : 5))) void k_forc
       }
     }
     if (renew) __syncthreads();
+    if (CACHE && have) {
+      f = sh_flags[sg];
+      have = sh_act[sg] != 0;
+    }
     const int i = T.atom0 + a;
-    const double qunit = have ? xq_unit(S, R.skin) : 1.0;
+    const double qunit = have ? (CACHE ? sh_qu[sg] : xq_unit(S, R.skin)) : 1.0;
     double fx = 0.0, fy = 0.0, fz = 0.0;
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};  // e_i, vir xx yy zz xy xz yz (FULL)
     if (have && i < R.np) {
