@@ -92,6 +92,36 @@ __device__ inline void top2_merge(double &a1, double &a2, double b1, double b2) 
   a1 = hi;
 }
 
+// Block sums of NV doubles per thread and a block top-2 in one LDS exchange; valid in
+// thread 0 (the same order of additions as block_sum).  The caller synchronises before
+// sh is written again.
+template <int NV>
+__device__ inline void block_sum_top2(double (&v)[NV], double &t1, double &t2, double *sh) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] += shfl_xor_d(v[q], m);
+    const double b1 = shfl_xor_d(t1, m), b2 = shfl_xor_d(t2, m);
+    top2_merge(t1, t2, b1, b2);
+  }
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) sh[wid * (NV + 2) + q] = v[q];
+    sh[wid * (NV + 2) + NV] = t1;
+    sh[wid * (NV + 2) + NV + 1] = t2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < nw; ++w) {
+#pragma unroll
+      for (int q = 0; q < NV; ++q) v[q] += sh[w * (NV + 2) + q];
+      top2_merge(t1, t2, sh[w * (NV + 2) + NV], sh[w * (NV + 2) + NV + 1]);
+    }
+  }
+}
+
 // Block top-2 (largest, second largest); valid in thread 0.
 __device__ inline void block_top2(double &t1, double &t2, double *sh) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -1029,52 +1059,67 @@ __device__ void step_epilogue(const RepParams &R, RepState &S, int f, double t1,
   }
 }
 
+// Kinetic-energy partials (NKW slots from kbase) and displacement top-2 of a thread's
+// tiles, in tile order; the loads of a tile are issued together, four tiles at a time, so
+// the sums cost one round trip to memory per four tiles.
+template <int NKW>
+__device__ inline void ke_top2_sum(const __attribute__((address_space(1))) double *p, int kbase,
+                                   int ntl, int t0, int nthr, double &ke, double &t1,
+                                   double &t2) {
+#pragma unroll 4
+  for (int lt = t0; lt < ntl; lt += nthr) {
+    double x[NKW];
+#pragma unroll
+    for (int w = 0; w < NKW; ++w) x[w] = p[(size_t)(kbase + w) * ntl + lt];
+    const double d1 = p[(size_t)kPartD1 * ntl + lt], d2 = p[(size_t)kPartD2 * ntl + lt];
+#pragma unroll
+    for (int w = 0; w < NKW; ++w) ke += x[w];
+    top2_merge(t1, t2, d1, d2);
+  }
+}
+
 // One workgroup per replica: fixed-order sums of the tile partials, then (by mode) the
 // force / kick results, the drop-in rebuild check, or the full step epilogue.
-// One workgroup per replica.  A replica's partials are summed by the first T threads,
-// T = 1024 for a replica of more than 1024 tiles (2M atoms: 16 coalesced rounds) and 256
-// otherwise; T depends only on the replica, and the other threads of the workgroup add
-// exact zeros, so the sums do not depend on the launch's block size.
+// A replica's partials are summed by the first T threads, T = 1024 for a replica of more
+// than 1024 tiles (2M atoms: 16 tiles per thread) and 256 otherwise; T depends only on the
+// replica, and the other threads of the workgroup add exact zeros, so the sums do not
+// depend on the launch's block size.  The kernel is a chain of round trips to memory (a
+// replica of 2000 tiles is summed by one workgroup): the kinetic-energy and displacement
+// partials, which every step reads, are loaded before the step flags are known (a
+// released slot has no tiles), and the block reduction of the sums and the top-2 is one
+// LDS exchange.
 __global__ __launch_bounds__(1024) void k_reduce(EngineDev E, int r0, int nr, int mode,
                                                  int xflags, int check_trigger) {
-  __shared__ double sh[16 * 8];
+  __shared__ double sh[16 * 10];
   for (int r = r0 + blockIdx.x; r < r0 + nr; r += gridDim.x) {
     const RepParams &R = E.reps[r];
     RepState &S = E.st[r];
-    const int f = xflags ? xflags : step_flags(R, S);
-    if (!(f & F_ACTIVE)) continue;
-    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    double t1 = -1.0, t2 = -1.0;
     const int ntl = R.ntiles;
     const auto p = gmem((const double *)R.partials);
-    const int nkw = 2 * R.tpa;   // kinetic-energy slots of a mixed-path tile
     const int nthr = ntl > 1024 ? 1024 : 256;
     const int t0 = (int)threadIdx.x < nthr ? (int)threadIdx.x : ntl;   // idle threads: no tiles
-    // one quantity at a time, eight tiles' loads in flight per thread (a large replica is
-    // summed by one workgroup, whose time is the number of round trips to memory)
-    if (f & F_CK) {
-      if (R.path == NDG_PATH_MIXED) {
-        for (int w = 0; w < nkw; ++w) {
-          const auto pw = p + (size_t)(kPartKeW + w) * ntl;
-#pragma unroll 8
-          for (int lt = t0; lt < ntl; lt += nthr) v[0] += pw[lt];
-        }
-      } else {
-#pragma unroll 8
-        for (int lt = t0; lt < ntl; lt += nthr) v[0] += p[kPartKe * ntl + lt];
+    double ke = 0.0, t1 = -1.0, t2 = -1.0;
+    if (R.path == NDG_PATH_MIXED) {
+      // kinetic-energy slots of a mixed-path tile: one per wave of the force kernel's tile
+      switch (R.tpa) {
+        case 1: ke_top2_sum<2>(p, kPartKeW, ntl, t0, nthr, ke, t1, t2); break;
+        case 2: ke_top2_sum<4>(p, kPartKeW, ntl, t0, nthr, ke, t1, t2); break;
+        case 4: ke_top2_sum<8>(p, kPartKeW, ntl, t0, nthr, ke, t1, t2); break;
+        default: ke_top2_sum<16>(p, kPartKeW, ntl, t0, nthr, ke, t1, t2); break;
       }
+    } else {
+      ke_top2_sum<1>(p, kPartKe, ntl, t0, nthr, ke, t1, t2);
     }
+    const int f = xflags ? xflags : step_flags(R, S);
+    if (!(f & F_ACTIVE)) continue;
+    double v[8] = {(f & F_CK) ? ke : 0.0, 0, 0, 0, 0, 0, 0, 0};
     if (f & (F_CU | F_CV)) {
 #pragma unroll 4
       for (int lt = t0; lt < ntl; lt += nthr)
 #pragma unroll
         for (int q = 1; q < 8; ++q) v[q] += p[q * ntl + lt];
     }
-#pragma unroll 8
-    for (int lt = t0; lt < ntl; lt += nthr)
-      top2_merge(t1, t2, p[kPartD1 * ntl + lt], p[kPartD2 * ntl + lt]);
-    block_sum<8>(v, sh);
-    block_top2(t1, t2, sh);
+    block_sum_top2<8>(v, t1, t2, sh);
     if (threadIdx.x == 0) {
       if (mode & RED_FORCE) {
         S.fnew = 1;
