@@ -1415,18 +1415,77 @@ __device__ inline int cell_run(int c, int &len) {
   return 63 - __builtin_clzll(lead & (lane == 63 ? ~0ull : ((2ull << lane) - 1)));
 }
 
+// Histogram of one tile: the cell of each atom, one atomic per run of equal cells.
+__device__ inline void bin_tile(const EngineDev &E, int t) {
+  const Tile T = E.tiles[t];
+  const RepParams &R = E.reps[T.rep];
+  const RepState &S = E.st[T.rep];
+  if (!S.rebuild || S.err) return;
+  const int i = T.atom0 + threadIdx.x;
+  const int c = i < R.np ? cell_of(R, S, i) : -1;
+  if (c >= 0) R.key[i] = c;
+  int len;
+  const int ld = cell_run(c, len);
+  if (c >= 0 && ld == (int)(threadIdx.x & 63)) atomicAdd(&R.cell_count[c], len);
+}
+
 __global__ __launch_bounds__(kTile) void k_rb_bin(EngineDev E) {
-  for (int k = blockIdx.x, t; (t = rb_tile(E, k)) >= 0; k += gridDim.x) {
-    const Tile T = E.tiles[t];
-    const RepParams &R = E.reps[T.rep];
-    const RepState &S = E.st[T.rep];
-    if (!S.rebuild || S.err) continue;
-    const int i = T.atom0 + threadIdx.x;
-    const int c = i < R.np ? cell_of(R, S, i) : -1;
-    if (c >= 0) R.key[i] = c;
-    int len;
-    const int ld = cell_run(c, len);
-    if (c >= 0 && ld == (int)(threadIdx.x & 63)) atomicAdd(&R.cell_count[c], len);
+  for (int k = blockIdx.x, t; (t = rb_tile(E, k)) >= 0; k += gridDim.x) bin_tile(E, t);
+}
+
+// The histogram with the work list planned in the same launch: every workgroup scans the
+// rebuild flags of the replicas [r0, r0 + nr) into LDS (prefix of the rebuilding
+// replicas' tile counts, in slot order, as k_rb_plan) and maps its work indices to
+// tiles itself; each workgroup writes the list entries it visits and workgroup 0 the
+// count, for the kernels that follow.  One launch less per step than k_rb_plan + k_rb_bin
+// (an empty launch costs ~3 us).  Dynamic LDS: 2 nr + 1 ints, nr <= kPlanLds.
+constexpr int kPlanLds = 4096;
+
+__global__ __launch_bounds__(kTile) void k_rb_bin_plan(EngineDev E, int r0, int nr) {
+  extern __shared__ int sh_plan[];
+  __shared__ int sh_scan[kTile];
+  int *pre = sh_plan, *first = sh_plan + nr + 1;   // pre[j]: tiles of replicas before j
+  const int tid = threadIdx.x;
+  for (int j = tid; j < nr; j += kTile) {
+    const int r = r0 + j;
+    const int t0 = E.tile0[r];
+    const RepState &S = E.st[r];
+    pre[j + 1] = t0 >= 0 && S.rebuild && !S.err ? E.reps[r].ntiles : 0;
+    first[j] = t0;
+  }
+  if (tid == 0) pre[0] = 0;
+  __syncthreads();
+  // inclusive prefix over chunks of consecutive replicas, one chunk per thread
+  const int per = (nr + kTile - 1) / kTile;
+  const int c0 = min(nr, tid * per), c1 = min(nr, c0 + per);
+  int loc = 0;
+  for (int j = c0; j < c1; ++j) loc += pre[j + 1];
+  sh_scan[tid] = loc;
+  __syncthreads();
+  for (int off = 1; off < kTile; off <<= 1) {
+    const int v = tid >= off ? sh_scan[tid - off] : 0;
+    __syncthreads();
+    sh_scan[tid] += v;
+    __syncthreads();
+  }
+  int run = sh_scan[tid] - loc;
+  for (int j = c0; j < c1; ++j) {
+    run += pre[j + 1];
+    pre[j + 1] = run;
+  }
+  __syncthreads();
+  const int total = pre[nr];
+  if (blockIdx.x == 0 && tid == 0) E.rb[0] = total;
+  for (int k = blockIdx.x; k < total; k += gridDim.x) {
+    // the last replica j with pre[j] <= k (it has tiles: pre[j + 1] > k)
+    int lo = 0, hi = nr - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (pre[mid] <= k) lo = mid; else hi = mid - 1;
+    }
+    const int t = first[lo] + (k - pre[lo]);
+    if (tid == 0) E.rb[1 + k] = t;
+    bin_tile(E, t);
   }
 }
 
@@ -2133,8 +2192,13 @@ void launch_rebuild(const EngineDev &E, int nt, int r0, int nr, hipStream_t s) {
   // that a step in which few tiles rebuild does not dispatch a workgroup per tile (c2, 16000
   // tiles rebuilt every ~45 steps: 1.017 -> 1.007 ms per step; c3 and c1 unchanged)
   const int g = nt < 2048 ? nt : 2048;
-  hipLaunchKernelGGL(k_rb_plan, dim3(1), dim3(1024), 0, s, E, r0, nr);
-  hipLaunchKernelGGL(k_rb_bin, dim3(g), dim3(kTile), 0, s, E);
+  if (nr <= kPlanLds) {
+    hipLaunchKernelGGL(k_rb_bin_plan, dim3(g), dim3(kTile), sizeof(int) * (2 * (size_t)nr + 1), s,
+                       E, r0, nr);
+  } else {
+    hipLaunchKernelGGL(k_rb_plan, dim3(1), dim3(1024), 0, s, E, r0, nr);
+    hipLaunchKernelGGL(k_rb_bin, dim3(g), dim3(kTile), 0, s, E);
+  }
   hipLaunchKernelGGL(k_rb_scan, dim3(gr), dim3(1024), 0, s, E, r0, nr);
   hipLaunchKernelGGL(k_rb_scatter, dim3(g), dim3(kTile), 0, s, E);
   hipLaunchKernelGGL(k_rb_rank, dim3(g), dim3(64 * kRankWaves), 0, s, E);
