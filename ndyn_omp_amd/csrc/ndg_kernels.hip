@@ -1611,14 +1611,30 @@ struct alignas(16) HomeRec {
   int ax, ay, az, pad;           // anchor
 };
 
+// Candidates that passed the chunk's bounding-box cull, in stream order, waiting for a
+// full batch of 64: a ring of 128 (at most 63 wait while a raw batch adds up to 64 more).
+constexpr int kQueue = 128;
+
 struct ListWave {
   HomeRec home[kChunk];
   int lo[27], pref[28];
   unsigned long long mask[kChunk];   // accepted candidates of the batch, per home atom
   unsigned ent[64];                  // list entry of each candidate of the batch
+  double qx[kQueue], qy[kQueue], qz[kQueue];   // candidate in the frame of the home cell
+  int qu[kQueue];                    // slot
+  unsigned qent[kQueue];             // list entry
+  int qtj[kQueue];                   // type | 256 if in the home cell
 };
 
 __device__ inline unsigned long long ballot64(bool p) { return __ballot(p); }
+
+// A double that is the same in every lane, held in scalar registers.
+__device__ inline double uniform_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
 
 __device__ inline int lane_prefix(unsigned long long m) {
   return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
@@ -1768,10 +1784,15 @@ __device__ long long list_chunk(const RepParams &R, RepState &S, ListWave &L, co
   const int mx = S.mcell[0], my = S.mcell[1], mz = S.mcell[2];
   const double b0 = S.box[0], b1 = S.box[1], b2 = S.box[2];
   const double w0 = b0 / mx, w1 = b1 / my, w2 = b2 / mz;
+  // bounding box of the chunk's homes (cell-anchored frame), for the candidate cull
+  double bl0 = 1e300, bl1 = 1e300, bl2 = 1e300, bh0 = -1e300, bh1 = -1e300, bh2 = -1e300;
   if (lane < cnt) {
     const int ih = lo + lane;
     const double4 xa = R.x64a[ih];
     const int4 A = R.anchor[ih];
+    bl0 = bh0 = xa.x;
+    bl1 = bh1 = xa.y;
+    bl2 = bh2 = xa.z;
     HomeRec &hr = L.home[lane];
     hr.x = xa.x;
     hr.y = xa.y;
@@ -1851,35 +1872,18 @@ __device__ long long list_chunk(const RepParams &R, RepState &S, ListWave &L, co
     return make_double4(gx64a[4 * u], gx64a[4 * u + 1], gx64a[4 * u + 2], gx64a[4 * u + 3]);
   };
   int nv = 0;   // lane h: list length of home h, -1 after an overflow
-  // the next batch's candidates are located and loaded while this batch is tested
-  int u, q;
-  locate(lane, u, q);
-  double4 xu = load_x(u);
-  for (int g0 = 0; g0 < total; g0 += 64) {
-    const bool inb = g0 + lane < total;
-    const bool same = !brute && q == 13;
-    const unsigned long long valid = ballot64(inb);
-    const unsigned long long samem = ballot64(inb && same);
-    int un = lo, qn = 13;
-    double4 xn = xu;
-    if (g0 + 64 < total) {
-      locate(g0 + 64 + lane, un, qn);
-      xn = load_x(un);
-    }
-    const int tj = bits_type(xu.w);
-    if (anch) {
-      const int dxq = q % 3 - 1, dyq = (q / 3) % 3 - 1, dzq = q / 9 - 1;
-      // the candidate in the frame of the home cell
-      const double ux_ = xu.x + dxq * w0, uy_ = xu.y + dyq * w1, uz_ = xu.z + dzq * w2;
-      L.ent[lane] = (unsigned)u |
-          ((unsigned)((1 - dxq) | ((1 - dyq) << 2) | ((1 - dzq) << 4)) << kCodeShift);
+  // test and append of one batch of 64 candidates (lane = candidate: slot bu, position
+  // bx, by, bz in the frame of the home cell, type btj, entry bent)
+  auto run_batch = [&](int bu, bool bsame, double bx, double by, double bz, int btj,
+                       unsigned bent, unsigned long long bvalid, unsigned long long bsamem) {
+      L.ent[lane] = bent;
       // test: one ballot of the batch per home atom (no appends here)
       if (uni) {
-        if (samem) test_batch<true, true>(L, gpx, gpy, gpz, B, lo, cnt, u, same, ux_, uy_, uz_, tj, valid, samem);
-        else test_batch<false, true>(L, gpx, gpy, gpz, B, lo, cnt, u, same, ux_, uy_, uz_, tj, valid, samem);
+        if (bsamem) test_batch<true, true>(L, gpx, gpy, gpz, B, lo, cnt, bu, bsame, bx, by, bz, btj, bvalid, bsamem);
+        else test_batch<false, true>(L, gpx, gpy, gpz, B, lo, cnt, bu, bsame, bx, by, bz, btj, bvalid, bsamem);
       } else {
-        if (samem) test_batch<true, false>(L, gpx, gpy, gpz, B, lo, cnt, u, same, ux_, uy_, uz_, tj, valid, samem);
-        else test_batch<false, false>(L, gpx, gpy, gpz, B, lo, cnt, u, same, ux_, uy_, uz_, tj, valid, samem);
+        if (bsamem) test_batch<true, false>(L, gpx, gpy, gpz, B, lo, cnt, bu, bsame, bx, by, bz, btj, bvalid, bsamem);
+        else test_batch<false, false>(L, gpx, gpy, gpz, B, lo, cnt, bu, bsame, bx, by, bz, btj, bvalid, bsamem);
       }
       wave_sync();
       // append: lane = home atom, the set bits of its mask in lane (= stream) order, up to
@@ -1914,6 +1918,81 @@ __device__ long long list_chunk(const RepParams &R, RepState &S, ListWave &L, co
         }
       }
       wave_sync();   // the next batch rewrites mask and ent
+  };
+  // reduce the homes' bounding box to every lane; a candidate farther than the largest list
+  // cutoff from it cannot enter any of the chunk's lists (the margin covers the rounding of
+  // the box distance and the reference re-test's ~1e-12 A^2), so on the anchored path only
+  // the survivors, about half of the 27-cell stream, are tested, in full batches of 64
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    bl0 = fmin(bl0, shfl_xor_d(bl0, off));
+    bl1 = fmin(bl1, shfl_xor_d(bl1, off));
+    bl2 = fmin(bl2, shfl_xor_d(bl2, off));
+    bh0 = fmax(bh0, shfl_xor_d(bh0, off));
+    bh1 = fmax(bh1, shfl_xor_d(bh1, off));
+    bh2 = fmax(bh2, shfl_xor_d(bh2, off));
+  }
+  bl0 = uniform_d(bl0);
+  bl1 = uniform_d(bl1);
+  bl2 = uniform_d(bl2);
+  bh0 = uniform_d(bh0);
+  bh1 = uniform_d(bh1);
+  bh2 = uniform_d(bh2);
+  const double cull2 = uniform_d(c2u * (1.0 + 1e-6) + 1e-9);
+  int nq = 0, qh = 0;   // queued candidates and the ring position of the first (wave-uniform)
+  // the next batch's candidates are located and loaded while this batch is tested
+  int u, q;
+  locate(lane, u, q);
+  double4 xu = load_x(u);
+  for (int g0 = 0; g0 < total; g0 += 64) {
+    const bool inb = g0 + lane < total;
+    const bool same = !brute && q == 13;
+    const unsigned long long valid = ballot64(inb);
+    const unsigned long long samem = ballot64(inb && same);
+    int un = lo, qn = 13;
+    double4 xn = xu;
+    if (g0 + 64 < total) {
+      locate(g0 + 64 + lane, un, qn);
+      xn = load_x(un);
+    }
+    const int tj = bits_type(xu.w);
+    if (anch) {
+      const int dxq = q % 3 - 1, dyq = (q / 3) % 3 - 1, dzq = q / 9 - 1;
+      // the candidate in the frame of the home cell
+      const double ux_ = xu.x + dxq * w0, uy_ = xu.y + dyq * w1, uz_ = xu.z + dzq * w2;
+      const unsigned ent = (unsigned)u |
+          ((unsigned)((1 - dxq) | ((1 - dyq) << 2) | ((1 - dzq) << 4)) << kCodeShift);
+      const double gx = fmax(0.0, fmax(bl0 - ux_, ux_ - bh0));
+      const double gy = fmax(0.0, fmax(bl1 - uy_, uy_ - bh1));
+      const double gz = fmax(0.0, fmax(bl2 - uz_, uz_ - bh2));
+      const bool keep = inb && !(fma(gx, gx, fma(gy, gy, gz * gz)) > cull2);
+      const unsigned long long km = ballot64(keep);
+      if (keep) {
+        const int k = (qh + nq + lane_prefix(km)) & (kQueue - 1);
+        L.qx[k] = ux_;
+        L.qy[k] = uy_;
+        L.qz[k] = uz_;
+        L.qu[k] = u;
+        L.qent[k] = ent;
+        L.qtj[k] = tj | (same ? 256 : 0);
+      }
+      nq += __popcll(km);
+      // full batches of queued candidates, and after the last raw batch the rest
+      // (absent lanes of a partial batch: the first home, far away)
+      const bool last = g0 + 64 >= total;
+      while (nq >= 64 || (last && nq > 0)) {
+        wave_sync();
+        const int k = (qh + lane) & (kQueue - 1);
+        const bool inq = lane < nq;
+        const int bt = inq ? L.qtj[k] : 0;
+        const bool bs = inq && (bt & 256) != 0;
+        run_batch(inq ? L.qu[k] : lo, bs, inq ? L.qx[k] : 1e30, inq ? L.qy[k] : 0.0,
+                  inq ? L.qz[k] : 0.0, inq ? (bt & 255) : 1, inq ? L.qent[k] : 0u,
+                  ballot64(inq), ballot64(bs));
+        const int used = nq < 64 ? nq : 64;
+        qh = (qh + used) & (kQueue - 1);
+        nq -= used;
+      }
     } else {
       const double pxu = gpx[u], pyu = gpy[u], pzu = gpz[u];
       const auto ga = gmem((const int *)R.anchor) + 4 * u;
