@@ -1702,8 +1702,10 @@ struct BandCtx {
 // taken kH at a time: their LDS records are read together, then kH independent FP64
 // chains run; a candidate within the band of a cutoff is decided by the reference's
 // formula (rare, after the group).  SAME: the batch holds same-cell candidates (PTrMin
-// band); UNI: one list cutoff for every type pair.
-constexpr int kH = 4;
+// band); UNI: one list cutoff for every type pair.  Two homes per group: with the
+// candidate queue's registers, four spill 8 VGPRs at the 128-register budget (0.3 %
+// slower on c3, c2 and c1).
+constexpr int kH = 2;
 
 template <bool SAME, bool UNI>
 __device__ inline void test_batch(ListWave &L, const __attribute__((address_space(1))) double *gpx,
