@@ -1605,7 +1605,7 @@ constexpr int kListWaves = 8;    // waves per workgroup (one workgroup per tile)
 constexpr int kChunk = 32;       // home atoms per wave task (16 in brute mode)
 
 struct alignas(16) HomeRec {
-  double x, y, z;                // cell-anchored position
+  float x, y, z;                 // cell-anchored position (FP32, for the banded test)
   int ih, th;                    // slot, type
   double px, py, pz;             // scaled position (reference test)
   int ax, ay, az, pad;           // anchor
@@ -1620,7 +1620,7 @@ struct ListWave {
   int lo[27], pref[28];
   unsigned long long mask[kChunk];   // accepted candidates of the batch, per home atom
   unsigned ent[64];                  // list entry of each candidate of the batch
-  double qx[kQueue], qy[kQueue], qz[kQueue];   // candidate in the frame of the home cell
+  float qx[kQueue], qy[kQueue], qz[kQueue];    // candidate in the frame of the home cell
   int qu[kQueue];                    // slot
   unsigned qent[kQueue];             // list entry
   int qtj[kQueue];                   // type | 256 if in the home cell
@@ -1692,10 +1692,19 @@ __device__ inline void append(const ListCtx &C, int &nv, int h, int ih, unsigned
 
 // Cutoff bands and box of the anchored test.
 struct BandCtx {
-  double c2u, c2lo_u, c2hi_u;    // uniform list cutoff^2 and its band
-  double pmlo, pmhi, ptrmin;     // PTrMin band (same-cell pairs)
+  double c2u;                    // uniform (or largest) list cutoff^2
+  float c2lo_u, c2hi_u;          // its band
+  float pmlo, pmhi;              // PTrMin band (same-cell pairs)
+  double ptrmin;
   double b0, b1, b2;             // box (reference re-test)
 };
+
+// Half-width (A^2) of the bands around a list cutoff and PTrMin inside which the FP32
+// test defers to the reference's formula.  The FP32 d2 of the cell frame (|coordinate| <
+// ~16 A: rounding <= 1e-6 A per coordinate and difference, plus that of the sums) is
+// within ~1.2e-4 A^2 of the exact one; a pair outside the band is decided the same way
+// by both.
+__device__ inline double test_band(double c2) { return 1e-6 * fabs(c2) + 1e-3; }
 
 // Test phase of one candidate batch on the anchored path: L.mask[h] = the lanes
 // (candidates) that enter the list of home h, for every home of the chunk.  Homes are
@@ -1712,11 +1721,11 @@ __device__ inline void test_batch(ListWave &L, const __attribute__((address_spac
                                   const __attribute__((address_space(1))) double *gpy,
                                   const __attribute__((address_space(1))) double *gpz,
                                   const BandCtx &B, int lo, int cnt, int u, bool same,
-                                  double ux, double uy, double uz, int tj,
+                                  float ux, float uy, float uz, int tj,
                                   unsigned long long valid, unsigned long long samem) {
   const int lane = threadIdx.x & 63;
   for (int h0 = 0; h0 < cnt; h0 += kH) {
-    double hx[kH], hy[kH], hz[kH];
+    float hx[kH], hy[kH], hz[kH];
     int hth[kH];
 #pragma unroll
     for (int k = 0; k < kH; ++k) {
@@ -1730,14 +1739,14 @@ __device__ inline void test_batch(ListWave &L, const __attribute__((address_spac
 #pragma unroll
     for (int k = 0; k < kH; ++k) {
       const int ih = lo + min(h0 + k, cnt - 1);
-      const double dx = hx[k] - ux, dy = hy[k] - uy, dz = hz[k] - uz;
-      const double d2 = fma(dx, dx, fma(dy, dy, dz * dz));
+      const float dx = hx[k] - ux, dy = hy[k] - uy, dz = hz[k] - uz;
+      const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
       unsigned long long live = valid & ~ballot64(u == ih);
-      double c2lo = B.c2lo_u, c2hi = B.c2hi_u;
+      float c2lo = B.c2lo_u, c2hi = B.c2hi_u;
       if (!UNI) {
         const double c2 = s_lcut2[(hth[k] - 1) * 8 + tj - 1];
-        c2lo = c2 - 1e-9 * fabs(c2);
-        c2hi = c2 + 1e-9 * fabs(c2);
+        c2lo = (float)(c2 - test_band(c2));
+        c2hi = (float)(c2 + test_band(c2));
         live &= ballot64(c2 >= 0.0);
       }
       const unsigned long long lo_m = ballot64(d2 < c2lo), hi_m = ballot64(d2 <= c2hi);
@@ -1796,9 +1805,9 @@ __device__ long long list_chunk(const RepParams &R, RepState &S, ListWave &L, co
     bl1 = bh1 = xa.y;
     bl2 = bh2 = xa.z;
     HomeRec &hr = L.home[lane];
-    hr.x = xa.x;
-    hr.y = xa.y;
-    hr.z = xa.z;
+    hr.x = (float)xa.x;
+    hr.y = (float)xa.y;
+    hr.z = (float)xa.z;
     hr.ih = ih;
     hr.th = bits_type(xa.w);
     hr.px = C.px[ih];
@@ -1841,11 +1850,11 @@ __device__ long long list_chunk(const RepParams &R, RepState &S, ListWave &L, co
   const double c2u = C.c2u;
   BandCtx B;
   B.c2u = c2u;
-  B.c2lo_u = c2u - 1e-9 * fabs(c2u);
-  B.c2hi_u = c2u + 1e-9 * fabs(c2u);
+  B.c2lo_u = (float)(c2u - test_band(c2u));
+  B.c2hi_u = (float)(c2u + test_band(c2u));
   // PTrMin band: the same width as the cutoff band of the largest cutoff
-  B.pmlo = ptrmin - 1e-9 * fabs(c2u);
-  B.pmhi = ptrmin + 1e-9 * fabs(c2u);
+  B.pmlo = (float)(ptrmin - test_band(c2u));
+  B.pmhi = (float)(ptrmin + test_band(c2u));
   B.ptrmin = ptrmin;
   B.b0 = b0;
   B.b1 = b1;
@@ -1876,7 +1885,7 @@ __device__ long long list_chunk(const RepParams &R, RepState &S, ListWave &L, co
   int nv = 0;   // lane h: list length of home h, -1 after an overflow
   // test and append of one batch of 64 candidates (lane = candidate: slot bu, position
   // bx, by, bz in the frame of the home cell, type btj, entry bent)
-  auto run_batch = [&](int bu, bool bsame, double bx, double by, double bz, int btj,
+  auto run_batch = [&](int bu, bool bsame, float bx, float by, float bz, int btj,
                        unsigned bent, unsigned long long bvalid, unsigned long long bsamem) {
       L.ent[lane] = bent;
       // test: one ballot of the batch per home atom (no appends here)
@@ -1940,7 +1949,10 @@ __device__ long long list_chunk(const RepParams &R, RepState &S, ListWave &L, co
   bh0 = uniform_d(bh0);
   bh1 = uniform_d(bh1);
   bh2 = uniform_d(bh2);
-  const double cull2 = uniform_d(c2u * (1.0 + 1e-6) + 1e-9);
+  // FP32 box distance of the FP32 candidate: within ~1e-4 A^2 of the exact one
+  const float cull2 = (float)uniform_d(c2u * (1.0 + 1e-6) + 1e-3);
+  const float fl0 = (float)bl0, fl1 = (float)bl1, fl2 = (float)bl2;
+  const float fh0 = (float)bh0, fh1 = (float)bh1, fh2 = (float)bh2;
   int nq = 0, qh = 0;   // queued candidates and the ring position of the first (wave-uniform)
   // the next batch's candidates are located and loaded while this batch is tested
   int u, q;
@@ -1961,13 +1973,14 @@ __device__ long long list_chunk(const RepParams &R, RepState &S, ListWave &L, co
     if (anch) {
       const int dxq = q % 3 - 1, dyq = (q / 3) % 3 - 1, dzq = q / 9 - 1;
       // the candidate in the frame of the home cell
-      const double ux_ = xu.x + dxq * w0, uy_ = xu.y + dyq * w1, uz_ = xu.z + dzq * w2;
+      const float ux_ = (float)(xu.x + dxq * w0), uy_ = (float)(xu.y + dyq * w1),
+                  uz_ = (float)(xu.z + dzq * w2);
       const unsigned ent = (unsigned)u |
           ((unsigned)((1 - dxq) | ((1 - dyq) << 2) | ((1 - dzq) << 4)) << kCodeShift);
-      const double gx = fmax(0.0, fmax(bl0 - ux_, ux_ - bh0));
-      const double gy = fmax(0.0, fmax(bl1 - uy_, uy_ - bh1));
-      const double gz = fmax(0.0, fmax(bl2 - uz_, uz_ - bh2));
-      const bool keep = inb && !(fma(gx, gx, fma(gy, gy, gz * gz)) > cull2);
+      const float gx = fmaxf(0.f, fmaxf(fl0 - ux_, ux_ - fh0));
+      const float gy = fmaxf(0.f, fmaxf(fl1 - uy_, uy_ - fh1));
+      const float gz = fmaxf(0.f, fmaxf(fl2 - uz_, uz_ - fh2));
+      const bool keep = inb && !(fmaf(gx, gx, fmaf(gy, gy, gz * gz)) > cull2);
       const unsigned long long km = ballot64(keep);
       if (keep) {
         const int k = (qh + nq + lane_prefix(km)) & (kQueue - 1);
@@ -1988,8 +2001,8 @@ __device__ long long list_chunk(const RepParams &R, RepState &S, ListWave &L, co
         const bool inq = lane < nq;
         const int bt = inq ? L.qtj[k] : 0;
         const bool bs = inq && (bt & 256) != 0;
-        run_batch(inq ? L.qu[k] : lo, bs, inq ? L.qx[k] : 1e30, inq ? L.qy[k] : 0.0,
-                  inq ? L.qz[k] : 0.0, inq ? (bt & 255) : 1, inq ? L.qent[k] : 0u,
+        run_batch(inq ? L.qu[k] : lo, bs, inq ? L.qx[k] : 1e30f, inq ? L.qy[k] : 0.f,
+                  inq ? L.qz[k] : 0.f, inq ? (bt & 255) : 1, inq ? L.qent[k] : 0u,
                   ballot64(inq), ballot64(bs));
         const int used = nq < 64 ? nq : 64;
         qh = (qh + used) & (kQueue - 1);
