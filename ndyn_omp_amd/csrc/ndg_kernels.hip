@@ -1595,10 +1595,12 @@ __global__ __launch_bounds__(kTile) void k_rb_place(EngineDev E) {
 // grid guarantees that the stencil representation of a pair is its minimum image for
 // every pair inside the cutoff ((M-2) w - 2.5e-4 box > cut on each axis; the 2.5e-4
 // covers the 0.4999 of the binning), d2 is formed instead from the cell-anchored
-// coordinates, candidate moved by its stencil offset: three subtractions and three
-// multiply-adds per pair.  The two forms differ by a few 1e-12 A^2; a pair whose anchored
-// d2 lies within 1e-9 c2 of the list cutoff (or of PTrMin for same-cell pairs) is decided
-// by the reference's own formula, so the accepted set is the reference's.  Other grids
+// coordinates in FP32, candidate moved by its stencil offset into the home cell's frame:
+// three subtractions and three multiply-adds per pair.  That d2 is within ~1.2e-4 A^2 of
+// the reference's; a pair whose FP32 d2 lies within test_band (1e-3 A^2 + 1e-6 c2) of the
+// list cutoff (or of PTrMin for same-cell pairs) is decided by the reference's own
+// formula, so the accepted set is the reference's.  Candidates farther than the cutoff
+// from the chunk's bounding box are culled before the test.  Other grids
 // (brute mode, M = 3 with w close to the cutoff) use the reference's formula for every
 // pair.  Entries carry the cell/image offset code of the mixed force kernel (ox = -dx).
 constexpr int kListWaves = 8;    // waves per workgroup (one workgroup per tile)
