@@ -1596,8 +1596,8 @@ __global__ __launch_bounds__(kTile) void k_rb_place(EngineDev E) {
 // every pair inside the cutoff ((M-2) w - 2.5e-4 box > cut on each axis; the 2.5e-4
 // covers the 0.4999 of the binning), d2 is formed instead from the cell-anchored
 // coordinates in FP32, candidate moved by its stencil offset into the home cell's frame:
-// three subtractions and three multiply-adds per pair.  That d2 is within ~1.2e-4 A^2 of
-// the reference's; a pair whose FP32 d2 lies within test_band (1e-3 A^2 + 1e-6 c2) of the
+// three subtractions and three multiply-adds per pair.  That d2 is within ~2e-6 c2 of
+// the reference's; a pair whose FP32 d2 lies within test_band (1e-3 A^2 + 1e-5 c2) of the
 // list cutoff (or of PTrMin for same-cell pairs) is decided by the reference's own
 // formula, so the accepted set is the reference's.  Candidates farther than the cutoff
 // from the chunk's bounding box are culled before the test.  Other grids
@@ -1702,11 +1702,12 @@ struct BandCtx {
 };
 
 // Half-width (A^2) of the bands around a list cutoff and PTrMin inside which the FP32
-// test defers to the reference's formula.  The FP32 d2 of the cell frame (|coordinate| <
-// ~16 A: rounding <= 1e-6 A per coordinate and difference, plus that of the sums) is
-// within ~1.2e-4 A^2 of the exact one; a pair outside the band is decided the same way
-// by both.
-__device__ inline double test_band(double c2) { return 1e-6 * fabs(c2) + 1e-3; }
+// test defers to the reference's formula.  In the home cell's frame a coordinate is at
+// most X ~ 2 cut + skin + 1 (cells are at most 4/3 of the cut wide on the anchored path);
+// rounding it, the difference and the sums to FP32 leaves d2 within ~2e-6 cut^2 (+ a few
+// 1e-5 A^2 for small cuts) of the exact value, so a pair outside the band is decided the
+// same way by both.
+__device__ inline double test_band(double c2) { return 1e-5 * fabs(c2) + 1e-3; }
 
 // Test phase of one candidate batch on the anchored path: L.mask[h] = the lanes
 // (candidates) that enter the list of home h, for every home of the chunk.  Homes are
@@ -1951,8 +1952,8 @@ __device__ long long list_chunk(const RepParams &R, RepState &S, ListWave &L, co
   bh0 = uniform_d(bh0);
   bh1 = uniform_d(bh1);
   bh2 = uniform_d(bh2);
-  // FP32 box distance of the FP32 candidate: within ~1e-4 A^2 of the exact one
-  const float cull2 = (float)uniform_d(c2u * (1.0 + 1e-6) + 1e-3);
+  // FP32 box distance of the FP32 candidate: within the same bound as the test's d2
+  const float cull2 = (float)uniform_d(c2u + test_band(c2u));
   const float fl0 = (float)bl0, fl1 = (float)bl1, fl2 = (float)bl2;
   const float fh0 = (float)bh0, fh1 = (float)bh1, fh2 = (float)bh2;
   int nq = 0, qh = 0;   // queued candidates and the ring position of the first (wave-uniform)

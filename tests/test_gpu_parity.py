@@ -304,3 +304,21 @@ def test_other_potentials_through_the_device(kind_name, kind, prm, q, oracle):
         else:
             assert rms_rel(f, f_o) <= 1e-5
             assert pe == pytest.approx(pe_o, rel=1e-6, abs=1e-9)
+
+
+def test_long_cutoff_neighbour_sets(oracle):
+    """A 14 A Morse cutoff (4 x 4 x 4 cells of 15.4 A, about 1080 neighbours per atom):
+    the list build's FP32 test in the cell frame, whose rounding grows with the cell size,
+    still yields the reference's neighbour sets bit for bit."""
+    prm = (14.0,) + W.CU_MORSE[1:]
+    s = W.cu_morse(17, temp_k=600.0, seed=17, jitter_a=0.1, params=prm, ptsize=20000)
+    print(f"case long cutoff np={s.np}", flush=True)
+    _, vnl, _ = oracle.half_list(s, 0.5)
+    ref = oracle.neighbour_sets_from_half(vnl)
+    for path in (G.PATH_FP64, G.PATH_MIXED):
+        sim = G.Simulation(s, path=path)
+        sim.build_lists(0.5)
+        counts, nbr = sim.neighbours()
+        for i in range(s.np):
+            np.testing.assert_array_equal(nbr[i, :counts[i]], ref[i])
+        sim.close()
