@@ -1620,8 +1620,9 @@ constexpr int kQueue = 128;
 struct ListWave {
   HomeRec home[kChunk];
   int lo[27], pref[28];
-  unsigned long long mask[kChunk];   // accepted candidates of the batch, per home atom
-  unsigned ent[64];                  // list entry of each candidate of the batch
+  // two tested batches, appended together (lanes 0-31 the first, 32-63 the second)
+  unsigned long long mask[2][kChunk];   // accepted candidates of the batch, per home atom
+  unsigned ent[2][64];                  // list entry of each candidate of the batch
   float qx[kQueue], qy[kQueue], qz[kQueue];    // candidate in the frame of the home cell
   int qu[kQueue];                    // slot
   unsigned qent[kQueue];             // list entry
@@ -1709,7 +1710,7 @@ struct BandCtx {
 // same way by both.
 __device__ inline double test_band(double c2) { return 1e-5 * fabs(c2) + 1e-3; }
 
-// Test phase of one candidate batch on the anchored path: L.mask[h] = the lanes
+// Test phase of one candidate batch on the anchored path: mrow[h] = the lanes
 // (candidates) that enter the list of home h, for every home of the chunk.  Homes are
 // taken kH at a time: their LDS records are read together, then kH independent FP64
 // chains run; a candidate within the band of a cutoff is decided by the reference's
@@ -1720,7 +1721,8 @@ __device__ inline double test_band(double c2) { return 1e-5 * fabs(c2) + 1e-3; }
 constexpr int kH = 2;
 
 template <bool SAME, bool UNI>
-__device__ inline void test_batch(ListWave &L, const __attribute__((address_space(1))) double *gpx,
+__device__ inline void test_batch(ListWave &L, unsigned long long *mrow,
+                                  const __attribute__((address_space(1))) double *gpx,
                                   const __attribute__((address_space(1))) double *gpy,
                                   const __attribute__((address_space(1))) double *gpz,
                                   const BandCtx &B, int lo, int cnt, int u, bool same,
@@ -1784,7 +1786,7 @@ __device__ inline void test_batch(ListWave &L, const __attribute__((address_spac
     if (lane == 0) {
 #pragma unroll
       for (int k = 0; k < kH; ++k)
-        if (h0 + k < cnt) L.mask[h0 + k] = acc[k];
+        if (h0 + k < cnt) mrow[h0 + k] = acc[k];
     }
   }
 }
@@ -1886,52 +1888,71 @@ __device__ long long list_chunk(const RepParams &R, RepState &S, ListWave &L, co
     return make_double4(gx64a[4 * u], gx64a[4 * u + 1], gx64a[4 * u + 2], gx64a[4 * u + 3]);
   };
   int nv = 0;   // lane h: list length of home h, -1 after an overflow
-  // test and append of one batch of 64 candidates (lane = candidate: slot bu, position
-  // bx, by, bz in the frame of the home cell, type btj, entry bent)
+  // test of one batch of 64 candidates into mask/entry slot ts (lane = candidate: slot bu,
+  // position bx, by, bz in the frame of the home cell, type btj, entry bent)
+  auto test_slot = [&](int ts, int bu, bool bsame, float bx, float by, float bz, int btj,
+                       unsigned bent, unsigned long long bvalid, unsigned long long bsamem) {
+    L.ent[ts][lane] = bent;
+    unsigned long long *mrow = L.mask[ts];
+    // one ballot of the batch per home atom (no appends here)
+    if (uni) {
+      if (bsamem) test_batch<true, true>(L, mrow, gpx, gpy, gpz, B, lo, cnt, bu, bsame, bx, by, bz, btj, bvalid, bsamem);
+      else test_batch<false, true>(L, mrow, gpx, gpy, gpz, B, lo, cnt, bu, bsame, bx, by, bz, btj, bvalid, bsamem);
+    } else {
+      if (bsamem) test_batch<true, false>(L, mrow, gpx, gpy, gpz, B, lo, cnt, bu, bsame, bx, by, bz, btj, bvalid, bsamem);
+      else test_batch<false, false>(L, mrow, gpx, gpy, gpz, B, lo, cnt, bu, bsame, bx, by, bz, btj, bvalid, bsamem);
+    }
+  };
+  // append of the ns (1 or 2) tested batches: lane = home atom h of batch half = lane / 32
+  // (the second batch's entries follow the first's), the set bits of its mask in lane
+  // (= stream) order, up to four entries per step (their LDS reads in flight together)
+  auto append_slots = [&](int ns) {
+    wave_sync();
+    const int half = lane >> 5, h = lane & 31;
+    const int nvh = __shfl(nv, h);   // the home's list length, held by lane h
+    const bool act = h < cnt && half < ns;
+    const unsigned long long m0 = act ? L.mask[0][h] : 0ull;
+    const unsigned long long m1 = act && ns > 1 ? L.mask[1][h] : 0ull;
+    const int tot = __popcll(m0) + __popcll(m1);
+    const bool ovf = nvh < 0 || nvh + tot > C.neimax;
+    if (act && !ovf) {
+      unsigned long long m = half ? m1 : m0;
+      const int ih = lo + h;
+      const auto bh = gmem(C.list + (size_t)(ih >> 7) * C.tstride + (ih & (kTile - 1)) * C.tpa);
+      const unsigned msk = (unsigned)(C.tpa - 1), sh = (unsigned)C.tsh;
+      const unsigned *er = L.ent[half];
+      unsigned k = (unsigned)(nvh + (half ? __popcll(m0) : 0));
+      while (m) {
+        int b[4];
+        int nb = 0;
+#pragma unroll
+        for (int z = 0; z < 4; ++z) {
+          b[z] = m ? __builtin_ctzll(m) : 0;
+          nb += m ? 1 : 0;
+          m &= m - 1;
+        }
+        unsigned ev[4];
+#pragma unroll
+        for (int z = 0; z < 4; ++z) ev[z] = er[b[z]];
+#pragma unroll
+        for (int z = 0; z < 4; ++z)
+          if (z < nb) bh[(((k + z) >> sh) << (7 + sh)) | ((k + z) & msk)] = ev[z];
+        k += (unsigned)nb;
+      }
+    }
+    if (lane < cnt) nv = ovf ? -1 : nvh + tot;
+    wave_sync();   // the next batches rewrite mask and ent
+  };
+  int ns = 0;   // tested batches waiting for their append (wave-uniform)
   auto run_batch = [&](int bu, bool bsame, float bx, float by, float bz, int btj,
                        unsigned bent, unsigned long long bvalid, unsigned long long bsamem) {
-      L.ent[lane] = bent;
-      // test: one ballot of the batch per home atom (no appends here)
-      if (uni) {
-        if (bsamem) test_batch<true, true>(L, gpx, gpy, gpz, B, lo, cnt, bu, bsame, bx, by, bz, btj, bvalid, bsamem);
-        else test_batch<false, true>(L, gpx, gpy, gpz, B, lo, cnt, bu, bsame, bx, by, bz, btj, bvalid, bsamem);
-      } else {
-        if (bsamem) test_batch<true, false>(L, gpx, gpy, gpz, B, lo, cnt, bu, bsame, bx, by, bz, btj, bvalid, bsamem);
-        else test_batch<false, false>(L, gpx, gpy, gpz, B, lo, cnt, bu, bsame, bx, by, bz, btj, bvalid, bsamem);
-      }
-      wave_sync();
-      // append: lane = home atom, the set bits of its mask in lane (= stream) order, up to
-      // four entries per step (their LDS reads in flight together)
-      if (lane < cnt && nv >= 0) {
-        unsigned long long m = L.mask[lane];
-        if (nv + __popcll(m) > C.neimax) {
-          nv = -1;
-        } else {
-          const int ih = lo + lane;
-          const auto bh = gmem(C.list + (size_t)(ih >> 7) * C.tstride + (ih & (kTile - 1)) * C.tpa);
-          const unsigned msk = (unsigned)(C.tpa - 1), sh = (unsigned)C.tsh;
-          unsigned k = (unsigned)nv;
-          while (m) {
-            int b[4];
-            int nb = 0;
-#pragma unroll
-            for (int z = 0; z < 4; ++z) {
-              b[z] = m ? __builtin_ctzll(m) : 0;
-              nb += m ? 1 : 0;
-              m &= m - 1;
-            }
-            unsigned ev[4];
-#pragma unroll
-            for (int z = 0; z < 4; ++z) ev[z] = L.ent[b[z]];
-#pragma unroll
-            for (int z = 0; z < 4; ++z)
-              if (z < nb) bh[(((k + z) >> sh) << (7 + sh)) | ((k + z) & msk)] = ev[z];
-            k += (unsigned)nb;
-          }
-          nv = (int)k;
-        }
-      }
-      wave_sync();   // the next batch rewrites mask and ent
+    test_slot(ns, bu, bsame, bx, by, bz, btj, bent, bvalid, bsamem);
+    if (ns == 1) {
+      append_slots(2);
+      ns = 0;
+    } else {
+      ns = 1;
+    }
   };
   // reduce the homes' bounding box to every lane; a candidate farther than the largest list
   // cutoff from it cannot enter any of the chunk's lists (the margin covers the rounding of
@@ -2038,6 +2059,7 @@ __device__ long long list_chunk(const RepParams &R, RepState &S, ListWave &L, co
     q = qn;
     xu = xn;
   }
+  if (anch && ns > 0) append_slots(ns);
   wave_sync();   // the home records are read by every lane above
   const bool act = lane < cnt;
   if (act) R.nnb[lo + lane] = nv < 0 ? 0 : nv;
