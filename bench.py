@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE x 2.000, WRITE_SIZE x 1.000.  Source: profiles/k_force_mixed_c3_traffic.txt.
 # HBM bytes per force launch of each workload's default run, from FETCH_SIZE / WRITE_SIZE
 # counter runs calibrated on k_copy (profiles/k_force_mixed_<workload>_traffic.txt)
-TRAFFIC_BYTES_PER_LAUNCH = {"c3": 1.0366e9, "c2": 1.3886e9, "c1": 8.8227e7}
+TRAFFIC_BYTES_PER_LAUNCH = {"c3": 1.0365e9, "c2": 1.3886e9, "c1": 8.8220e7}
 METRIC = "atom-steps/s vs plain FP64 GPU path at <=1e-5 rel force err; p50/p99 job latency"
 
 
