@@ -739,10 +739,16 @@ static EngineDev dev_view(ndg_engine *e) {
   E.tabset_p = e->tabset_p;
   E.springs = 0;
   double list_bytes = 0.0;
+  int nused = 0, min_tiles = 1 << 30;
   for (const Slot &sl : e->slots) {
     if (sl.used && sl.P.spring_mask) E.springs = 1;
-    if (sl.used) list_bytes += 4.0 * sl.ntiles * sl.P.list_chunks * kTile * sl.tpa;
+    if (sl.used) {
+      list_bytes += 4.0 * sl.ntiles * sl.P.list_chunks * kTile * sl.tpa;
+      ++nused;
+      min_tiles = std::min(min_tiles, sl.ntiles);
+    }
   }
+  E.list_big = nused > 0 && min_tiles >= kBigTiles ? 1 : 0;
   // non-temporal list loads measured: c3 (1.5 GB of lists) +1.4 %, c2 +2.9 %, and c1
   // (0.1 GB, which the cache keeps between steps) -4 %
   E.list_nt = list_bytes > kListCacheBytes ? 1 : 0;

@@ -135,7 +135,13 @@ struct EngineDev {
   int springs;          // some replica has spring pairs: launch the force variants with them
   int list_nt;          // 1: the lists exceed the last-level cache; the force kernel streams
                         // them with non-temporal loads
+  int list_big;         // 1: every replica has at least kBigTiles tiles; the list build takes
+                        // chunks of up to 64 home atoms of a cell instead of 32
 };
+
+// Replicas this large rebuild enough tiles at once to fill the GPU with the list build's
+// larger chunks.
+constexpr int kBigTiles = 512;
 
 // Size of the device's last-level cache (MI355X Infinity Cache, 256 MB): lists that fit
 // stay cached from one step to the next, longer ones are streamed past it.

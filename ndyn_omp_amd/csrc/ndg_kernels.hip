@@ -1604,7 +1604,13 @@ __global__ __launch_bounds__(kTile) void k_rb_place(EngineDev E) {
 // (brute mode, M = 3 with w close to the cutoff) use the reference's formula for every
 // pair.  Entries carry the cell/image offset code of the mixed force kernel (ox = -dx).
 constexpr int kListWaves = 8;    // waves per workgroup (one workgroup per tile)
-constexpr int kChunk = 32;       // home atoms per wave task (16 in brute mode)
+// Home atoms per wave task: at most kChunk; 64 when every replica of the engine has at least
+// kBigTiles tiles (the k_rb_list<true> instance), 32 otherwise (16 in brute mode).  A large
+// replica rebuilds many tiles at once, enough to fill the GPU with either chunk size, and a
+// cell's one chunk then streams its 27 cells once instead of twice (c2 +4 %, c1 +3 %
+// atom-steps/s); the replica mix, whose few rebuilding tiles per step need the waves of the
+// smaller chunks to hide their latency, runs 6 % slower with 64.
+constexpr int kChunk = 64;
 
 struct alignas(16) HomeRec {
   float x, y, z;                 // cell-anchored position (FP32, for the banded test)
@@ -1620,7 +1626,8 @@ constexpr int kQueue = 128;
 struct ListWave {
   HomeRec home[kChunk];
   int lo[27], pref[28];
-  // two tested batches, appended together (lanes 0-31 the first, 32-63 the second)
+  // two tested batches, appended together (lanes 0-31 the first, 32-63 the second) when a
+  // chunk has at most 32 homes
   unsigned long long mask[2][kChunk];   // accepted candidates of the batch, per home atom
   unsigned ent[2][64];                  // list entry of each candidate of the batch
   float qx[kQueue], qy[kQueue], qz[kQueue];    // candidate in the frame of the home cell
@@ -1792,7 +1799,7 @@ __device__ inline void test_batch(ListWave &L, unsigned long long *mrow,
 }
 
 __device__ long long list_chunk(const RepParams &R, RepState &S, ListWave &L, const ListCtx &C,
-                                bool anch, int lo, int cnt, int cell) {
+                                bool anch, int lo, int cnt, int cell, bool pair) {
   const int lane = threadIdx.x & 63;
   lo = __builtin_amdgcn_readfirstlane(lo);
   cnt = __builtin_amdgcn_readfirstlane(cnt);
@@ -1908,7 +1915,7 @@ __device__ long long list_chunk(const RepParams &R, RepState &S, ListWave &L, co
   // (= stream) order, up to four entries per step (their LDS reads in flight together)
   auto append_slots = [&](int ns) {
     wave_sync();
-    const int half = lane >> 5, h = lane & 31;
+    const int half = pair ? lane >> 5 : 0, h = pair ? lane & 31 : lane;
     const int nvh = __shfl(nv, h);   // the home's list length, held by lane h
     const bool act = h < cnt && half < ns;
     const unsigned long long m0 = act ? L.mask[0][h] : 0ull;
@@ -1947,7 +1954,9 @@ __device__ long long list_chunk(const RepParams &R, RepState &S, ListWave &L, co
   auto run_batch = [&](int bu, bool bsame, float bx, float by, float bz, int btj,
                        unsigned bent, unsigned long long bvalid, unsigned long long bsamem) {
     test_slot(ns, bu, bsame, bx, by, bz, btj, bent, bvalid, bsamem);
-    if (ns == 1) {
+    if (!pair) {
+      append_slots(1);   // chunks of more than 32 homes: one batch per append
+    } else if (ns == 1) {
       append_slots(2);
       ns = 0;
     } else {
@@ -2083,6 +2092,7 @@ __device__ long long list_chunk(const RepParams &R, RepState &S, ListWave &L, co
 
 // One tile of the list build, by the whole workgroup: the chunks of the cells whose first
 // slot lies in the tile (the tile's own slots in brute mode), dealt to its waves in turn.
+template <bool BIG>
 __device__ inline void list_tile(const EngineDev &E, int t, ListWave &L, double &s_cmax,
                                  double &s_cmin) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -2132,10 +2142,10 @@ __device__ inline void list_tile(const EngineDev &E, int t, ListWave &L, double 
   C.c2u = s_cmax;
   long long mine = 0;
   if (S.brute) {
-    constexpr int kb = kChunk / 2;
+    constexpr int kb = 16;
     for (int k = wid; k < kTile / kb; k += kListWaves) {
       const int lo = T.atom0 + kb * k;
-      if (lo < R.np) mine += list_chunk(R, S, L, C, false, lo, min(kb, R.np - lo), -1);
+      if (lo < R.np) mine += list_chunk(R, S, L, C, false, lo, min(kb, R.np - lo), -1, true);
     }
   } else {
     const int mx = S.mcell[0], my = S.mcell[1], mz = S.mcell[2];
@@ -2144,6 +2154,7 @@ __device__ inline void list_tile(const EngineDev &E, int t, ListWave &L, double 
                       (my - 2) * (S.box[1] / my) - 2.5e-4 * S.box[1] > cut &&
                       (mz - 2) * (S.box[2] / mz) - 2.5e-4 * S.box[2] > cut;
     const int ncell = mx * my * mz;
+    constexpr int chunk = BIG ? 64 : 32;
     int c = __builtin_amdgcn_readfirstlane(R.anchor[T.atom0].w);
     if (R.cell_start[c] < T.atom0) ++c;
     int k = 0;
@@ -2151,13 +2162,13 @@ __device__ inline void list_tile(const EngineDev &E, int t, ListWave &L, double 
       const int c_lo = __builtin_amdgcn_readfirstlane(R.cell_start[c]);
       if (c_lo >= T.atom0 + kTile) break;
       const int c_hi = __builtin_amdgcn_readfirstlane(R.cell_start[c + 1]);
-      const int nch = (c_hi - c_lo + kChunk - 1) / kChunk;
+      const int nch = (c_hi - c_lo + chunk - 1) / chunk;
       for (int j = 0; j < nch; ++j, ++k) {
         if (k % kListWaves != wid) continue;
         // near-equal chunks of the cell
         const int a0 = c_lo + (int)((long long)(c_hi - c_lo) * j / nch);
         const int a1 = c_lo + (int)((long long)(c_hi - c_lo) * (j + 1) / nch);
-        mine += list_chunk(R, S, L, C, anch, a0, a1 - a0, c);
+        mine += list_chunk(R, S, L, C, anch, a0, a1 - a0, c, !BIG);
       }
     }
   }
@@ -2165,6 +2176,8 @@ __device__ inline void list_tile(const EngineDev &E, int t, ListWave &L, double 
 }
 
 // At most 128 VGPRs (four waves per SIMD): two workgroups of a CU build lists at once.
+// BIG: chunks of up to 64 home atoms (every replica has at least kBigTiles tiles).
+template <bool BIG>
 __global__ __launch_bounds__(64 * kListWaves) __attribute__((amdgpu_waves_per_eu(4)))
 void k_rb_list(EngineDev E) {
   __shared__ ListWave s_wave[kListWaves];
@@ -2172,7 +2185,7 @@ void k_rb_list(EngineDev E) {
   // one workgroup per entry of the work list (the grid covers every tile): the hardware
   // hands tiles to CUs as they free up, and a wave without a chunk ends at once
   const int t = rb_tile(E, blockIdx.x);
-  if (t >= 0) list_tile(E, t, s_wave[threadIdx.x >> 6], s_cmax, s_cmin);
+  if (t >= 0) list_tile<BIG>(E, t, s_wave[threadIdx.x >> 6], s_cmax, s_cmin);
 }
 
 // ------------------------------------------------------------------ bandwidth probe
@@ -2322,7 +2335,10 @@ void launch_rebuild(const EngineDev &E, int nt, int r0, int nr, hipStream_t s) {
   hipLaunchKernelGGL(k_rb_scatter, dim3(g), dim3(kTile), 0, s, E);
   hipLaunchKernelGGL(k_rb_rank, dim3(g), dim3(64 * kRankWaves), 0, s, E);
   hipLaunchKernelGGL(k_rb_place, dim3(g), dim3(kTile), 0, s, E);
-  hipLaunchKernelGGL(k_rb_list, dim3(nt), dim3(64 * kListWaves), 0, s, E);
+  if (E.list_big)
+    hipLaunchKernelGGL(k_rb_list<true>, dim3(nt), dim3(64 * kListWaves), 0, s, E);
+  else
+    hipLaunchKernelGGL(k_rb_list<false>, dim3(nt), dim3(64 * kListWaves), 0, s, E);
 }
 
 }  // namespace ndg
