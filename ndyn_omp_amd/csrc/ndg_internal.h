@@ -136,7 +136,8 @@ struct EngineDev {
   int list_nt;          // 1: the lists exceed the last-level cache; the force kernel streams
                         // them with non-temporal loads
   int list_big;         // 1: every replica has at least kBigTiles tiles; the list build takes
-                        // chunks of up to 64 home atoms of a cell instead of 32
+                        // chunks of up to 64 home atoms of a cell instead of 32 and places
+                        // the state arrays from tmp itself (no k_rb_place launch)
 };
 
 // Replicas this large rebuild enough tiles at once to fill the GPU with the list build's

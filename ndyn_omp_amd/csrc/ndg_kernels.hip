@@ -1205,10 +1205,20 @@ __global__ __launch_bounds__(256) void k_snapshot(EngineDev E, int r, double *ou
 // Counting sort of atoms by reference cell (canonical order: cell, then caller index),
 // then a full Verlet list from the 27 surrounding cells in a fixed stencil order.
 
-__device__ inline void permute_atom(const RepParams &R, int src, int dst, int parity) {
-  R.tmpd[dst] = R.px[src];
-  R.tmpd[R.np + dst] = R.py[src];
-  R.tmpd[2 * R.np + dst] = R.pz[src];
+__device__ inline void derive_atom(const RepParams &R, const RepState &S, int dst, double px,
+                                   double py, double pz, int ty);
+
+// Copies atom src of the state to slot dst of the tmp arrays.  DERIVED: also writes the
+// arrays derived from its position at slot dst directly (nothing reads them between this
+// kernel and the list build, which then places the state arrays from tmp itself);
+// otherwise k_rb_place writes both.
+template <bool DERIVED>
+__device__ inline void permute_atom(const RepParams &R, const RepState &S, int src, int dst) {
+  const double px = R.px[src], py = R.py[src], pz = R.pz[src];
+  const int ty = R.typ[src];
+  R.tmpd[dst] = px;
+  R.tmpd[R.np + dst] = py;
+  R.tmpd[2 * R.np + dst] = pz;
   R.tmpd[3 * R.np + dst] = R.vx[src];
   R.tmpd[4 * R.np + dst] = R.vy[src];
   R.tmpd[5 * R.np + dst] = R.vz[src];
@@ -1226,18 +1236,42 @@ __device__ inline void permute_atom(const RepParams &R, int src, int dst, int pa
     R.tmpd[11 * R.np + dst] = R.ipz[src];
   }
   if (R.nbc) R.tmpi[2 * R.np + dst] = R.hbc[src];
-  R.tmpi[dst] = R.typ[src];
+  R.tmpi[dst] = ty;
   R.tmpi[R.np + dst] = R.oid[src];
+  if (DERIVED) derive_atom(R, S, dst, px, py, pz, ty);
 }
 
-__device__ inline void place_atom(const RepParams &R, const RepState &S, int p) {
-  const double px = R.tmpd[p], py = R.tmpd[R.np + p], pz = R.tmpd[2 * R.np + p];
-  R.px[p] = px; R.py[p] = py; R.pz[p] = pz;
+// The arrays derived from the position of slot dst: anchor, anchored and packed records,
+// build-time copies.
+__device__ inline void derive_atom(const RepParams &R, const RepState &S, int dst, double px,
+                                   double py, double pz, int ty) {
+  if (R.path == NDG_PATH_FP64) {
+    R.pbx[dst] = px; R.pby[dst] = py; R.pbz[dst] = pz;
+  }
+  int kx, ky, kz, cxa, cya, cza;
+  bin_axis(px, S.mcell[0], S.brute, kx, cxa);
+  bin_axis(py, S.mcell[1], S.brute, ky, cya);
+  bin_axis(pz, S.mcell[2], S.brute, kz, cza);
+  const int cell = cell_rank(kx, ky, kz, S.mcell[0], S.mcell[1], S.mcell[2], R.cb_y, R.cb_z);
+  R.anchor[dst] = make_int4(cxa, cya, cza, cell);
+  const double4 x = make_double4(anchored(px, cxa, S.mcell[0], S.box[0] / S.mcell[0]),
+                                 anchored(py, cya, S.mcell[1], S.box[1] / S.mcell[1]),
+                                 anchored(pz, cza, S.mcell[2], S.box[2] / S.mcell[2]),
+                                 type_bits(ty));
+  R.x64a[dst] = x;
+  R.xq[dst] = xq_pack(x.x, x.y, x.z, ty, xq_unit(S, R.skin));
+  R.x32b[dst] = make_float4((float)x.x, (float)x.y, (float)x.z, 0.f);
+}
+
+// Slot p of the state arrays from tmp (the permuted copy written by permute_atom).
+__device__ inline void place_state(const RepParams &R, int p) {
+  R.px[p] = R.tmpd[p];
+  R.py[p] = R.tmpd[R.np + p];
+  R.pz[p] = R.tmpd[2 * R.np + p];
   R.vx[p] = R.tmpd[3 * R.np + p];
   R.vy[p] = R.tmpd[4 * R.np + p];
   R.vz[p] = R.tmpd[5 * R.np + p];
-  const int ty = R.tmpi[p];
-  R.typ[p] = ty;
+  R.typ[p] = R.tmpi[p];
   R.oid[p] = R.tmpi[R.np + p];
   if (R.nbc) R.hbc[p] = R.tmpi[2 * R.np + p];
   if (R.spring_mask) {
@@ -1249,24 +1283,10 @@ __device__ inline void place_atom(const RepParams &R, const RepState &S, int p) 
     R.ax[p] = R.tmpd[6 * R.np + p];
     R.ay[p] = R.tmpd[7 * R.np + p];
     R.az[p] = R.tmpd[8 * R.np + p];
-    R.pbx[p] = px; R.pby[p] = py; R.pbz[p] = pz;
   } else {
     R.f32a[p] = R.tmpf[p];
     R.f32b[p] = R.tmpf[R.np + p];
   }
-  int kx, ky, kz, cxa, cya, cza;
-  bin_axis(px, S.mcell[0], S.brute, kx, cxa);
-  bin_axis(py, S.mcell[1], S.brute, ky, cya);
-  bin_axis(pz, S.mcell[2], S.brute, kz, cza);
-  const int cell = cell_rank(kx, ky, kz, S.mcell[0], S.mcell[1], S.mcell[2], R.cb_y, R.cb_z);
-  R.anchor[p] = make_int4(cxa, cya, cza, cell);
-  const double4 x = make_double4(anchored(px, cxa, S.mcell[0], S.box[0] / S.mcell[0]),
-                                 anchored(py, cya, S.mcell[1], S.box[1] / S.mcell[1]),
-                                 anchored(pz, cza, S.mcell[2], S.box[2] / S.mcell[2]),
-                                 type_bits(ty));
-  R.x64a[p] = x;
-  R.xq[p] = xq_pack(x.x, x.y, x.z, ty, xq_unit(S, R.skin));
-  R.x32b[p] = make_float4((float)x.x, (float)x.y, (float)x.z, 0.f);
 }
 
 __device__ inline int cell_of(const RepParams &R, const RepState &S, int i) {
@@ -1300,8 +1320,9 @@ __device__ inline unsigned order_key(const RepParams &R, const RepState &S, int 
 // global memory.  The atoms are then copied to their final slots in tmp.
 constexpr int kCellLds = 256;
 
+template <bool DERIVED>
 __device__ inline void sort_cell_wave(const RepParams &R, const RepState &S, int lo, int hi,
-                                      int parity, int *wsrc, unsigned *wkey) {
+                                      int *wsrc, unsigned *wkey) {
   const int lane = threadIdx.x & 63;
   const int cnt = hi - lo;
   if (cnt <= 64) {
@@ -1318,7 +1339,7 @@ __device__ inline void sort_cell_wave(const RepParams &R, const RepState &S, int
           u = ou;
         }
       }
-    if (lane < cnt) permute_atom(R, u, lo + lane, parity);
+    if (lane < cnt) permute_atom<DERIVED>(R, S, u, lo + lane);
   } else if (cnt <= kCellLds) {
     for (int v = lane; v < cnt; v += 64) {
       const int u = R.slot_src[lo + v];
@@ -1330,7 +1351,7 @@ __device__ inline void sort_cell_wave(const RepParams &R, const RepState &S, int
       const unsigned me = wkey[v];
       int rank = 0;
       for (int w = 0; w < cnt; ++w) rank += wkey[w] < me;
-      permute_atom(R, wsrc[v], lo + rank, parity);
+      permute_atom<DERIVED>(R, S, wsrc[v], lo + rank);
     }
     wave_sync();
   } else {
@@ -1339,7 +1360,7 @@ __device__ inline void sort_cell_wave(const RepParams &R, const RepState &S, int
       const unsigned me = order_key(R, S, i);
       int rank = 0;
       for (int w = lo; w < hi; ++w) rank += order_key(R, S, R.slot_src[w]) < me;
-      permute_atom(R, i, lo + rank, parity);
+      permute_atom<DERIVED>(R, S, i, lo + rank);
     }
   }
 }
@@ -1546,6 +1567,8 @@ __global__ __launch_bounds__(kTile) void k_rb_scatter(EngineDev E) {
 // Each tile sorts the cells whose first slot lies in it (one wave per cell, kRankWaves
 // waves per tile, so that a tile's few cells are sorted side by side).
 constexpr int kRankWaves = 8;
+// BIG: the derived arrays are written here (permute_atom<true>).
+template <bool BIG>
 __global__ __launch_bounds__(64 * kRankWaves) void k_rb_rank(EngineDev E) {
   __shared__ int sh_sort[kRankWaves][2 * kCellLds];
   for (int k = blockIdx.x, t; (t = rb_tile(E, k)) >= 0; k += gridDim.x) {
@@ -1555,7 +1578,7 @@ __global__ __launch_bounds__(64 * kRankWaves) void k_rb_rank(EngineDev E) {
     if (!S.rebuild || S.err) continue;
     if (S.brute) {
       const int i = T.atom0 + threadIdx.x;
-      if (threadIdx.x < kTile && i < R.np) permute_atom(R, i, R.oid[i], S.parity);
+      if (threadIdx.x < kTile && i < R.np) permute_atom<BIG>(R, S, i, R.oid[i]);
       continue;
     }
     const int ncell = S.mcell[0] * S.mcell[1] * S.mcell[2];
@@ -1565,11 +1588,13 @@ __global__ __launch_bounds__(64 * kRankWaves) void k_rb_rank(EngineDev E) {
     for (c += wid; c < ncell && R.cell_start[c] < T.atom0 + kTile; c += nwave) {
       const int lo = R.cell_start[c], hi = R.cell_start[c + 1];
       if (hi > lo)
-        sort_cell_wave(R, S, lo, hi, S.parity, sh_sort[wid], (unsigned *)(sh_sort[wid] + kCellLds));
+        sort_cell_wave<BIG>(R, S, lo, hi, sh_sort[wid], (unsigned *)(sh_sort[wid] + kCellLds));
     }
   }
 }
 
+// The state arrays from tmp and the derived arrays, for engines whose order kernel and list
+// build do not write them (see list_tile).
 __global__ __launch_bounds__(kTile) void k_rb_place(EngineDev E) {
   for (int k = blockIdx.x, t; (t = rb_tile(E, k)) >= 0; k += gridDim.x) {
     const Tile T = E.tiles[t];
@@ -1578,7 +1603,8 @@ __global__ __launch_bounds__(kTile) void k_rb_place(EngineDev E) {
     if (!S.rebuild || S.err) continue;
     const int p = T.atom0 + threadIdx.x;
     if (p >= R.np) continue;
-    place_atom(R, S, p);
+    place_state(R, p);
+    derive_atom(R, S, p, R.tmpd[p], R.tmpd[R.np + p], R.tmpd[2 * R.np + p], R.tmpi[p]);
   }
 }
 
@@ -2101,6 +2127,17 @@ __device__ inline void list_tile(const EngineDev &E, int t, ListWave &L, double 
   RepState &S = E.st[T.rep];
   // S.rebuild does not change during this kernel (S.err can be raised meanwhile)
   if (!S.rebuild) return;
+  // BIG: the tile's slots of the state arrays from tmp, here instead of in k_rb_place
+  // (one launch less per step; c1 +5.8 % atom-steps/s).  Every replica on the work list
+  // had no error when it was planned, and no kernel since raises one, so the order kernel
+  // has permuted all of its atoms into tmp.  A mix of small replicas keeps k_rb_place: its
+  // list build is bound by the latency of each workgroup's chain, which the copy would
+  // lengthen (-0.5 %), and its order kernel leaves the derived arrays to k_rb_place
+  // (-0.3 % when written there).  The pair tests read the positions from tmp either way.
+  if (BIG) {
+    const int p = T.atom0 + (int)threadIdx.x;
+    if (threadIdx.x < kTile && p < R.np) place_state(R, p);
+  }
   if (threadIdx.x < 64) {
     // the reference's per-pair list cutoff (src/nd.f90:1361,1375-1377), per type pair
     const int ti = lane / 8 + 1, tj = lane % 8 + 1;
@@ -2135,9 +2172,9 @@ __device__ inline void list_tile(const EngineDev &E, int t, ListWave &L, double 
   C.tstride = (size_t)R.list_chunks * C.kstride;
   C.neimax = R.neimax;
   C.ptrmin = R.ptrmin;
-  C.px = R.px;
-  C.py = R.py;
-  C.pz = R.pz;
+  C.px = R.tmpd;
+  C.py = R.tmpd + R.np;
+  C.pz = R.tmpd + 2 * R.np;
   C.uni = s_cmax == s_cmin;
   C.c2u = s_cmax;
   long long mine = 0;
@@ -2176,7 +2213,8 @@ __device__ inline void list_tile(const EngineDev &E, int t, ListWave &L, double 
 }
 
 // At most 128 VGPRs (four waves per SIMD): two workgroups of a CU build lists at once.
-// BIG: chunks of up to 64 home atoms (every replica has at least kBigTiles tiles).
+// BIG: chunks of up to 64 home atoms, and the state arrays placed from tmp here (every
+// replica has at least kBigTiles tiles).
 template <bool BIG>
 __global__ __launch_bounds__(64 * kListWaves) __attribute__((amdgpu_waves_per_eu(4)))
 void k_rb_list(EngineDev E) {
@@ -2333,12 +2371,14 @@ void launch_rebuild(const EngineDev &E, int nt, int r0, int nr, hipStream_t s) {
   }
   hipLaunchKernelGGL(k_rb_scan, dim3(gr), dim3(1024), 0, s, E, r0, nr);
   hipLaunchKernelGGL(k_rb_scatter, dim3(g), dim3(kTile), 0, s, E);
-  hipLaunchKernelGGL(k_rb_rank, dim3(g), dim3(64 * kRankWaves), 0, s, E);
-  hipLaunchKernelGGL(k_rb_place, dim3(g), dim3(kTile), 0, s, E);
-  if (E.list_big)
+  if (E.list_big) {
+    hipLaunchKernelGGL(k_rb_rank<true>, dim3(g), dim3(64 * kRankWaves), 0, s, E);
     hipLaunchKernelGGL(k_rb_list<true>, dim3(nt), dim3(64 * kListWaves), 0, s, E);
-  else
+  } else {
+    hipLaunchKernelGGL(k_rb_rank<false>, dim3(g), dim3(64 * kRankWaves), 0, s, E);
+    hipLaunchKernelGGL(k_rb_place, dim3(g), dim3(kTile), 0, s, E);
     hipLaunchKernelGGL(k_rb_list<false>, dim3(nt), dim3(64 * kListWaves), 0, s, E);
+  }
 }
 
 }  // namespace ndg
