@@ -2128,7 +2128,7 @@ __device__ inline void list_tile(const EngineDev &E, int t, ListWave &L, double 
   // S.rebuild does not change during this kernel (S.err can be raised meanwhile)
   if (!S.rebuild) return;
   // BIG: the tile's slots of the state arrays from tmp, here instead of in k_rb_place
-  // (one launch less per step; c1 +5.8 % atom-steps/s).  Every replica on the work list
+  // (one launch less per step; c1 +6.4 % atom-steps/s).  Every replica on the work list
   // had no error when it was planned, and no kernel since raises one, so the order kernel
   // has permuted all of its atoms into tmp.  A mix of small replicas keeps k_rb_place: its
   // list build is bound by the latency of each workgroup's chain, which the copy would
